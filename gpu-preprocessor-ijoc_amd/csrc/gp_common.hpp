@@ -1,0 +1,301 @@
+// gp_common.hpp -- shared host/device infrastructure of the MI355X presolve
+// engine: error handling, a size-class device memory pool, device vectors,
+// launch helpers and the numerics shared by every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gpupre.h"
+
+namespace gp {
+
+// ---- errors -----------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw Error(e == hipErrorOutOfMemory ? GP_ERR_ALLOCATION : GP_ERR_RUNTIME,
+                std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+#define GP_CHECK(expr) ::gp::check((expr), #expr)
+#define GP_LAUNCH_CHECK(name) ::gp::check(hipGetLastError(), name)
+
+void set_last_error(const std::string& msg);
+
+template <class F>
+int guarded(F&& body) {
+  try {
+    body();
+    return GP_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::invalid_argument& e) {
+    set_last_error(e.what());
+    return GP_ERR_INVALID_ARGUMENT;
+  } catch (const std::length_error& e) {
+    set_last_error(e.what());
+    return GP_ERR_LENGTH;
+  } catch (const std::bad_alloc& e) {
+    set_last_error("host allocation failed");
+    return GP_ERR_ALLOCATION;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return GP_ERR_RUNTIME;
+  }
+}
+
+// ---- device memory pool -------------------------------------------------------
+// Size-class (power of two, >= 256 B) cache of hipMalloc blocks.  Presolve
+// rounds allocate and release plans, maps and scratch many times; the pool
+// turns those into free-list operations after the first round.
+class Pool {
+ public:
+  static Pool& instance();
+  void* get(size_t bytes);
+  void put(void* p);
+  void release_cached();  // hipFree every cached block (no live block touched)
+  size_t cached_bytes() const { return cached_; }
+  ~Pool();
+
+ private:
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> size_of_;
+  size_t cached_ = 0;
+};
+
+inline size_t size_class(size_t bytes) {
+  size_t s = 256;
+  while (s < bytes) s <<= 1;
+  return s;
+}
+
+template <class T>
+struct DVec {
+  T* p = nullptr;
+  size_t n = 0;    // logical size
+  size_t cap = 0;  // elements available
+  DVec() = default;
+  explicit DVec(size_t count) { resize(count); }
+  DVec(const DVec&) = delete;
+  DVec& operator=(const DVec&) = delete;
+  DVec(DVec&& o) noexcept : p(o.p), n(o.n), cap(o.cap) { o.p = nullptr; o.n = o.cap = 0; }
+  DVec& operator=(DVec&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p; n = o.n; cap = o.cap;
+      o.p = nullptr; o.n = o.cap = 0;
+    }
+    return *this;
+  }
+  ~DVec() { reset(); }
+  void reset() {
+    if (p) Pool::instance().put(p);
+    p = nullptr;
+    n = cap = 0;
+  }
+  // Discard contents and make room for `count` elements.
+  void resize(size_t count) {
+    if (count > cap) {
+      reset();
+      const size_t bytes = size_class(sizeof(T) * (count ? count : 1));
+      p = static_cast<T*>(Pool::instance().get(bytes));
+      cap = bytes / sizeof(T);
+    }
+    n = count;
+  }
+  // Grow keeping the first `keep` elements.
+  void grow_keep(size_t count, size_t keep) {
+    if (count <= cap) {
+      n = count;
+      return;
+    }
+    const size_t bytes = size_class(sizeof(T) * count);
+    T* q = static_cast<T*>(Pool::instance().get(bytes));
+    if (keep) GP_CHECK(hipMemcpyAsync(q, p, sizeof(T) * keep, hipMemcpyDeviceToDevice, 0));
+    if (p) Pool::instance().put(p);
+    p = q;
+    cap = bytes / sizeof(T);
+    n = count;
+  }
+  // Hand the block to an owner outside the pool's RAII (freed via pool_free).
+  T* release() {
+    T* q = p;
+    p = nullptr;
+    n = cap = 0;
+    return q;
+  }
+  T* data() const { return p; }
+  size_t size() const { return n; }
+};
+
+inline void pool_free(void* p) {
+  if (p) Pool::instance().put(p);
+}
+
+template <class T>
+T* pool_alloc(size_t count) {
+  return static_cast<T*>(Pool::instance().get(size_class(sizeof(T) * (count ? count : 1))));
+}
+
+template <class T>
+void d2h(T* dst, const T* src, size_t count) {
+  if (count) GP_CHECK(hipMemcpy(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost));
+}
+template <class T>
+void h2d(T* dst, const T* src, size_t count) {
+  if (count) GP_CHECK(hipMemcpy(dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+}
+template <class T>
+void d2d(T* dst, const T* src, size_t count) {
+  if (count) GP_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToDevice, 0));
+}
+template <class T>
+T read1(const T* src) {
+  T v{};
+  GP_CHECK(hipMemcpy(&v, src, sizeof(T), hipMemcpyDeviceToHost));
+  return v;
+}
+
+// ---- launch helpers -----------------------------------------------------------
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+// ---- kernel-family timing ---------------------------------------------------------
+// When enabled, every launch family is bracketed by HIP events on the null
+// stream; elapsed time and algorithmic bytes accumulate per family.
+struct KernelClock {
+  bool enabled = false;
+  double ms[GP_KT_COUNT] = {};
+  int64_t launches[GP_KT_COUNT] = {};
+  double bytes[GP_KT_COUNT] = {};
+  std::vector<hipEvent_t> pool;
+  struct Pending { int family; hipEvent_t a, b; double bytes; };
+  std::vector<Pending> pending;
+  hipEvent_t take();
+  int begin(int family);                // returns a token (or -1)
+  void end(int token, double algorithmic_bytes);
+  void flush();                          // sync + accumulate
+  ~KernelClock();
+};
+KernelClock& kclock();
+
+struct TimedScope {
+  int token;
+  double bytes;
+  TimedScope(int family, double b) : token(kclock().begin(family)), bytes(b) {}
+  ~TimedScope() { kclock().end(token, bytes); }
+};
+
+// ---- numerics shared by all kernels (mirrored by the CPU restatement) --------------
+constexpr int kSumBlock = 256;  // stored entries per Neumaier block in row sums
+
+struct Neu {
+  double s, c;
+};
+
+__host__ __device__ inline void neu_add(Neu& a, double x) {
+  const double t = a.s + x;
+  if (fabs(a.s) >= fabs(x))
+    a.c += (a.s - t) + x;
+  else
+    a.c += (x - t) + a.s;
+  a.s = t;
+}
+__host__ __device__ inline void neu_merge(Neu& a, Neu b) {
+  neu_add(a, b.s);
+  a.c += b.c;
+}
+__host__ __device__ inline double neu_value(Neu a) { return a.s + a.c; }
+
+// Row accumulator: pos(k) for every stored position k of the row before add().
+struct RowAcc {
+  Neu total{0.0, 0.0}, blk{0.0, 0.0};
+  __host__ __device__ inline void pos(int64_t k) {
+    if (k > 0 && k % kSumBlock == 0) {
+      neu_merge(total, blk);
+      blk.s = blk.c = 0.0;
+    }
+  }
+  __host__ __device__ inline void add(double x) { neu_add(blk, x); }
+  __host__ __device__ inline double value() const {
+    Neu t = total;
+    neu_merge(t, blk);
+    return neu_value(t);
+  }
+};
+
+// fmax / fmin with C library semantics: ties return the first argument.
+__host__ __device__ inline double cmax(double x, double y) { return (x >= y || y != y) ? x : y; }
+__host__ __device__ inline double cmin(double x, double y) { return (x <= y || y != y) ? x : y; }
+
+// Max / min under the IEEE total order of non-NaN values (-0 < +0).
+__host__ __device__ inline bool total_greater(double a, double b) {
+  return a > b || (a == b && std::signbit(b) && !std::signbit(a));
+}
+
+__device__ inline void atomic_total_max(double* slot, double v) {
+  unsigned long long* addr = reinterpret_cast<unsigned long long*>(slot);
+  unsigned long long old = __atomic_load_n(addr, __ATOMIC_RELAXED);
+  while (total_greater(v, __longlong_as_double(static_cast<long long>(old)))) {
+    const unsigned long long want = static_cast<unsigned long long>(__double_as_longlong(v));
+    if (__atomic_compare_exchange_n(addr, &old, want, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+      break;
+  }
+}
+__device__ inline void atomic_total_min(double* slot, double v) {
+  unsigned long long* addr = reinterpret_cast<unsigned long long*>(slot);
+  unsigned long long old = __atomic_load_n(addr, __ATOMIC_RELAXED);
+  while (total_greater(__longlong_as_double(static_cast<long long>(old)), v)) {
+    const unsigned long long want = static_cast<unsigned long long>(__double_as_longlong(v));
+    if (__atomic_compare_exchange_n(addr, &old, want, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+      break;
+  }
+}
+
+// ---- primitives (gp_primitives.hip) ---------------------------------------------
+// In-place inclusive / exclusive scans; `total` receives the sum when non-null.
+void inclusive_scan_i32(int32_t* data, int64_t n);
+void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total);
+int64_t exclusive_scan_i32_total(int32_t* data, int64_t n);  // returns total
+// old_to_new[i] = keep[i] ? rank : -1 ; returns number kept.
+int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new);
+// Sum in the fixed pairwise order (chunks of 1024, stride halving, recursive).
+double det_sum(const double* x, int64_t n);
+void fill_u8(uint8_t* p, uint8_t v, int64_t n);
+void gather_compact(double* dst, const double* src, const int32_t* old_to_new, int32_t n_old);
+void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* old_to_new, int32_t n_old);
+int32_t count_u8(const uint8_t* mask, int32_t n);
+
+// Matrices.
+void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
+void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col,
+                 const int32_t* row_old_to_new, const int32_t* col_old_to_new, int32_t m_new,
+                 int32_t n_new, gp_csr& out);
+void csr_pool_free(gp_csr& A);
+
+// Statistics.
+void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
+                        int64_t* device_total);
+void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
+                          const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
+                          double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
+
+}  // namespace gp

@@ -1,0 +1,596 @@
+// gp_primitives.hip -- device-wide primitives of the presolve engine:
+// scans, prefix maps, the fixed-order global sum, CSR transpose and
+// compaction, live statistics and row activity summaries.
+#include "gp_common.hpp"
+
+#include <mutex>
+
+namespace gp {
+
+// ---- errors / pool / clock ---------------------------------------------------------
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+Pool& Pool::instance() {
+  static Pool pool;
+  return pool;
+}
+
+void* Pool::get(size_t bytes) {
+  auto it = free_.find(bytes);
+  if (it != free_.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    cached_ -= bytes;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // Release cached blocks and retry once before reporting exhaustion.
+    (void)hipGetLastError();
+    release_cached();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(GP_ERR_ALLOCATION, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+  }
+  size_of_[p] = bytes;
+  return p;
+}
+
+void Pool::put(void* p) {
+  if (!p) return;
+  auto it = size_of_.find(p);
+  if (it == size_of_.end()) return;  // not a pool block
+  free_[it->second].push_back(p);
+  cached_ += it->second;
+}
+
+void Pool::release_cached() {
+  if (cached_ == 0) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_) {
+    for (void* p : kv.second) {
+      (void)hipFree(p);
+      size_of_.erase(p);
+    }
+    kv.second.clear();
+  }
+  cached_ = 0;
+}
+
+Pool::~Pool() {}
+
+KernelClock& kclock() {
+  static KernelClock c;
+  return c;
+}
+
+hipEvent_t KernelClock::take() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  GP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+int KernelClock::begin(int family) {
+  if (!enabled) return -1;
+  Pending p{family, take(), take(), 0.0};
+  GP_CHECK(hipEventRecord(p.a, 0));
+  pending.push_back(p);
+  return static_cast<int>(pending.size()) - 1;
+}
+
+void KernelClock::end(int token, double algorithmic_bytes) {
+  if (token < 0 || !enabled) return;
+  Pending& p = pending[static_cast<size_t>(token)];
+  p.bytes = algorithmic_bytes;
+  (void)hipEventRecord(p.b, 0);
+}
+
+void KernelClock::flush() {
+  if (pending.empty()) return;
+  GP_CHECK(hipDeviceSynchronize());
+  for (auto& p : pending) {
+    float t = 0.f;
+    GP_CHECK(hipEventElapsedTime(&t, p.a, p.b));
+    ms[p.family] += t;
+    launches[p.family] += 1;
+    bytes[p.family] += p.bytes;
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  pending.clear();
+}
+
+KernelClock::~KernelClock() {}
+
+// ---- scans -------------------------------------------------------------------------
+namespace {
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <class T>
+__device__ T block_exclusive_scan(T v, T* lds, T* total) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    T add = t >= off ? lds[t - off] : T(0);
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  const T incl = lds[t];
+  *total = lds[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+template <class T>
+__global__ void k_tile_reduce(const T* in, int64_t n, T* sums) {
+  __shared__ T lds[kScanThreads];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  T acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = lds[0];
+}
+
+// Scan one tile in place; `prefix` (exclusive, per tile) may be null.
+template <class T>
+__global__ void k_tile_scan(T* data, int64_t n, const T* prefix, int inclusive) {
+  __shared__ T tile[kScanTile];
+  __shared__ T lds[kScanThreads];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    tile[k * kScanThreads + threadIdx.x] = i < n ? data[i] : T(0);
+  }
+  __syncthreads();
+  T local[kScanItems];
+  T sum = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    local[k] = tile[threadIdx.x * kScanItems + k];
+    sum += local[k];
+  }
+  T total;
+  T run = block_exclusive_scan<T>(sum, lds, &total);
+  if (prefix) run += prefix[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const T v = local[k];
+    tile[threadIdx.x * kScanItems + k] = inclusive ? run + v : run;
+    run += v;
+  }
+  __syncthreads();
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) data[i] = tile[k * kScanThreads + threadIdx.x];
+  }
+}
+
+template <class T>
+void scan_impl(T* data, int64_t n, bool inclusive) {
+  if (n <= 0) return;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kScanThreads), 0, 0, data, n,
+                       static_cast<const T*>(nullptr), inclusive ? 1 : 0);
+    GP_LAUNCH_CHECK("tile scan");
+    return;
+  }
+  DVec<T> sums(static_cast<size_t>(tiles));
+  hipLaunchKernelGGL(k_tile_reduce<T>, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, 0,
+                     static_cast<const T*>(data), n, sums.data());
+  GP_LAUNCH_CHECK("tile reduce");
+  scan_impl<T>(sums.data(), tiles, false);
+  hipLaunchKernelGGL(k_tile_scan<T>, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, 0,
+                     data, n, static_cast<const T*>(sums.data()), inclusive ? 1 : 0);
+  GP_LAUNCH_CHECK("tile scan");
+}
+
+__global__ void k_mask_to_i32(const uint8_t* keep, int32_t n, int32_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = keep[i] != 0;
+}
+
+__global__ void k_old_to_new(const uint8_t* keep, int32_t n, int32_t* map) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) map[i] = keep[i] ? map[i] : -1;
+}
+}  // namespace
+
+void inclusive_scan_i32(int32_t* data, int64_t n) { scan_impl<int32_t>(data, n, true); }
+
+void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total) {
+  if (n <= 0) {
+    if (total) *total = 0;
+    return;
+  }
+  int64_t last = 0;
+  if (total) last = read1(data + n - 1);
+  scan_impl<int64_t>(data, n, false);
+  if (total) *total = read1(data + n - 1) + last;
+}
+
+int64_t exclusive_scan_i32_total(int32_t* data, int64_t n) {
+  if (n <= 0) return 0;
+  const int32_t last = read1(data + n - 1);
+  scan_impl<int32_t>(data, n, false);
+  return static_cast<int64_t>(read1(data + n - 1)) + last;
+}
+
+int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_mask_to_i32, dim3(grid_for(n)), dim3(kBlock), 0, 0, keep, n, old_to_new);
+  GP_LAUNCH_CHECK("mask to i32");
+  const int64_t total = exclusive_scan_i32_total(old_to_new, n);
+  hipLaunchKernelGGL(k_old_to_new, dim3(grid_for(n)), dim3(kBlock), 0, 0, keep, n, old_to_new);
+  GP_LAUNCH_CHECK("old to new");
+  return static_cast<int32_t>(total);
+}
+
+// ---- fixed-order global sum ------------------------------------------------------------
+namespace {
+__global__ void k_det_sum_chunks(const double* x, int64_t n, double* partial) {
+  __shared__ double y[512];
+  const int t = threadIdx.x;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * 1024;
+  const double a = base + t < n ? x[base + t] : 0.0;
+  const double b = base + t + 512 < n ? x[base + t + 512] : 0.0;
+  y[t] = a + b;
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (t < s) y[t] = y[t] + y[t + s];
+    __syncthreads();
+  }
+  if (t == 0) partial[blockIdx.x] = y[0];
+}
+}  // namespace
+
+double det_sum(const double* x, int64_t n) {
+  if (n <= 0) return 0.0;
+  int64_t chunks = (n + 1023) / 1024;
+  DVec<double> a(static_cast<size_t>(chunks));
+  hipLaunchKernelGGL(k_det_sum_chunks, dim3(static_cast<unsigned>(chunks)), dim3(512), 0, 0, x, n, a.data());
+  GP_LAUNCH_CHECK("det sum");
+  DVec<double> b;
+  while (chunks > 1) {
+    const int64_t next = (chunks + 1023) / 1024;
+    b.resize(static_cast<size_t>(next));
+    hipLaunchKernelGGL(k_det_sum_chunks, dim3(static_cast<unsigned>(next)), dim3(512), 0, 0,
+                       static_cast<const double*>(a.data()), chunks, b.data());
+    GP_LAUNCH_CHECK("det sum");
+    std::swap(a, b);
+    chunks = next;
+  }
+  return read1(a.data());
+}
+
+// ---- small vector kernels ----------------------------------------------------------------
+namespace {
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+template <class T>
+__global__ void k_gather_compact(T* dst, const T* src, const int32_t* o2n, int32_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t k = o2n[i];
+    if (k >= 0) dst[k] = src[i];
+  }
+}
+__global__ void k_count_u8(const uint8_t* mask, int32_t n, int32_t* out) {
+  __shared__ int32_t lds[kBlock];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  lds[threadIdx.x] = (i < n && mask[i]) ? 1 : 0;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && lds[0]) atomicAdd(out, lds[0]);
+}
+}  // namespace
+
+void fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n)), dim3(kBlock), 0, 0, p, v, n);
+  GP_LAUNCH_CHECK("fill u8");
+}
+
+void gather_compact(double* dst, const double* src, const int32_t* o2n, int32_t n_old) {
+  if (n_old <= 0) return;
+  hipLaunchKernelGGL(k_gather_compact<double>, dim3(grid_for(n_old)), dim3(kBlock), 0, 0, dst, src, o2n, n_old);
+  GP_LAUNCH_CHECK("gather compact");
+}
+
+void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* o2n, int32_t n_old) {
+  if (n_old <= 0) return;
+  hipLaunchKernelGGL(k_gather_compact<int32_t>, dim3(grid_for(n_old)), dim3(kBlock), 0, 0, dst, src, o2n, n_old);
+  GP_LAUNCH_CHECK("gather compact i32");
+}
+
+int32_t count_u8(const uint8_t* mask, int32_t n) {
+  if (n <= 0) return 0;
+  DVec<int32_t> c(1);
+  GP_CHECK(hipMemsetAsync(c.data(), 0, sizeof(int32_t), 0));
+  hipLaunchKernelGGL(k_count_u8, dim3(grid_for(n)), dim3(kBlock), 0, 0, mask, n, c.data());
+  GP_LAUNCH_CHECK("count u8");
+  return read1(c.data());
+}
+
+// ---- transpose ----------------------------------------------------------------------------
+// Histogram of column counts, exclusive scan, atomic scatter, then each output
+// row is sorted by input row: the unique CSR of A^T with increasing row order.
+namespace {
+__global__ void k_col_hist(const int32_t* col, int64_t nnz, int32_t* cnt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < nnz) atomicAdd(cnt + col[p], 1);
+}
+__global__ void k_scatter_rows(const gp_offset* rp, const int32_t* col, const double* val, int32_t m,
+                               int32_t* cursor, int32_t* at_col, double* at_val) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) {
+    const int32_t q = atomicAdd(cursor + col[p], 1);
+    at_col[q] = static_cast<int32_t>(i);
+    at_val[q] = val[p];
+  }
+}
+__global__ void k_sort_segments(const gp_offset* rp, int32_t n, int32_t* idx, double* val) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const gp_offset a = rp[j], b = rp[j + 1];
+  for (gp_offset p = a + 1; p < b; ++p) {
+    const int32_t key = idx[p];
+    const double v = val[p];
+    gp_offset q = p - 1;
+    while (q >= a && idx[q] > key) {
+      idx[q + 1] = idx[q];
+      val[q + 1] = val[q];
+      --q;
+    }
+    idx[q + 1] = key;
+    val[q + 1] = v;
+  }
+}
+__global__ void k_i32_to_offset_shift(const int32_t* src, int32_t n, gp_offset* dst) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n) dst[j] = src[j];
+}
+}  // namespace
+
+void transpose_csr(const gp_csr& A, gp_csr& AT) {
+  TimedScope ts(GP_KT_TRANSPOSE, 2.0 * (12.0 * A.nnz) + 4.0 * (A.rows + 1) + 4.0 * (A.cols + 1));
+  const int32_t m = A.rows, n = A.cols;
+  const int64_t nnz = A.nnz;
+  AT.rows = n;
+  AT.cols = m;
+  AT.nnz = A.nnz;
+  AT.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(n) + 1);
+  AT.col_idx = pool_alloc<int32_t>(static_cast<size_t>(nnz));
+  AT.vals = pool_alloc<double>(static_cast<size_t>(nnz));
+  DVec<int32_t> cnt(static_cast<size_t>(n) + 1);
+  GP_CHECK(hipMemsetAsync(cnt.data(), 0, sizeof(int32_t) * (static_cast<size_t>(n) + 1), 0));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_col_hist, dim3(grid_for(nnz)), dim3(kBlock), 0, 0, A.col_idx, nnz, cnt.data());
+    GP_LAUNCH_CHECK("col hist");
+  }
+  exclusive_scan_i32_total(cnt.data(), static_cast<int64_t>(n) + 1);
+  hipLaunchKernelGGL(k_i32_to_offset_shift, dim3(grid_for(n + 1)), dim3(kBlock), 0, 0, cnt.data(), n + 1, AT.row_ptr);
+  GP_LAUNCH_CHECK("row ptr");
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
+                       cnt.data(), AT.col_idx, AT.vals);
+    GP_LAUNCH_CHECK("scatter rows");
+    hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n)), dim3(kBlock), 0, 0, AT.row_ptr, n, AT.col_idx, AT.vals);
+    GP_LAUNCH_CHECK("sort segments");
+  }
+}
+
+void csr_pool_free(gp_csr& A) {
+  pool_free(A.row_ptr);
+  pool_free(A.col_idx);
+  pool_free(A.vals);
+  A.row_ptr = nullptr;
+  A.col_idx = nullptr;
+  A.vals = nullptr;
+  A.rows = A.cols = 0;
+  A.nnz = 0;
+}
+
+// ---- compaction ------------------------------------------------------------------------------
+namespace {
+__global__ void k_compact_count(const gp_offset* rp, const int32_t* col, int32_t m, const uint8_t* keep_row,
+                                const uint8_t* keep_col, int64_t* cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int64_t c = 0;
+  if (!keep_row || keep_row[i])
+    for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) c += (!keep_col || keep_col[col[p]]) ? 1 : 0;
+  cnt[i] = c;
+}
+__global__ void k_compact_rowptr(const int64_t* excl, const int32_t* row_o2n, int32_t m, gp_offset* out_rp) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t r = row_o2n ? row_o2n[i] : static_cast<int32_t>(i);
+  if (r >= 0) out_rp[r] = static_cast<gp_offset>(excl[i]);
+}
+__global__ void k_compact_copy(const gp_offset* rp, const int32_t* col, const double* val, int32_t m,
+                               const int32_t* row_o2n, const int32_t* col_o2n, const int64_t* excl,
+                               int32_t* out_col, double* out_val) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (row_o2n && row_o2n[i] < 0) return;
+  int64_t q = excl[i];
+  for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) {
+    const int32_t j = col_o2n ? col_o2n[col[p]] : col[p];
+    if (j < 0) continue;
+    out_col[q] = j;
+    out_val[q] = val[p];
+    ++q;
+  }
+}
+__global__ void k_set_last(gp_offset* rp, int32_t at, gp_offset v) { rp[at] = v; }
+}  // namespace
+
+void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col,
+                 const int32_t* row_o2n, const int32_t* col_o2n, int32_t m_new, int32_t n_new,
+                 gp_csr& out) {
+  const int32_t m = A.rows;
+  DVec<int64_t> cnt(static_cast<size_t>(m) + 1);
+  int64_t total = 0;
+  {
+    TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_compact_count, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m,
+                         keep_row, keep_col, cnt.data());
+      GP_LAUNCH_CHECK("compact count");
+    }
+    exclusive_scan_i64(cnt.data(), m, &total);
+  }
+  if (total > INT32_MAX) throw std::length_error("compacted nnz exceeds int32 offsets");
+  out.rows = m_new;
+  out.cols = n_new;
+  out.nnz = static_cast<gp_offset>(total);
+  out.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(m_new) + 1);
+  out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(total));
+  out.vals = pool_alloc<double>(static_cast<size_t>(total));
+  TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + 12.0 * total + 4.0 * (m_new + 1));
+  if (m > 0) {
+    hipLaunchKernelGGL(k_compact_rowptr, dim3(grid_for(m)), dim3(kBlock), 0, 0, cnt.data(), row_o2n, m, out.row_ptr);
+    GP_LAUNCH_CHECK("compact rowptr");
+    hipLaunchKernelGGL(k_compact_copy, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
+                       row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals);
+    GP_LAUNCH_CHECK("compact copy");
+  }
+  hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, 0, out.row_ptr, m_new, static_cast<gp_offset>(total));
+  GP_LAUNCH_CHECK("compact last");
+}
+
+// ---- live statistics (one thread per row; integer results are order free) ----------
+namespace {
+__global__ void k_live_stats(gp_csr M, const uint8_t* keep_outer, const uint8_t* keep_inner, int32_t* degree,
+                             uint8_t* empty, uint8_t* singleton, int32_t* support, double* coefficient,
+                             unsigned long long* total) {
+  __shared__ unsigned long long lds[kBlock];
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int32_t count = 0;
+  if (r < M.rows) {
+    const bool active = !keep_outer || keep_outer[r];
+    gp_offset position = -1;
+    if (active) {
+      for (gp_offset p = M.row_ptr[r]; p < M.row_ptr[r + 1]; ++p) {
+        if (keep_inner && !keep_inner[M.col_idx[p]]) continue;
+        ++count;
+        position = p;
+      }
+    }
+    degree[r] = count;
+    empty[r] = active && count == 0;
+    singleton[r] = active && count == 1;
+    support[r] = count == 1 ? M.col_idx[position] : -1;
+    coefficient[r] = count == 1 ? M.vals[position] : 0.0;
+  }
+  if (total) {
+    lds[threadIdx.x] = static_cast<unsigned long long>(count);
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && lds[0]) atomicAdd(total, lds[0]);
+  }
+}
+}  // namespace
+
+void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, int64_t* device_total) {
+  const double bytes = 4.0 * (lp.A.rows + 1) + 4.0 * lp.A.nnz + 4.0 * (lp.AT.rows + 1) + 4.0 * lp.AT.nnz +
+                       2.0 * (lp.A.rows + lp.A.cols) + 17.0 * (lp.A.rows + lp.A.cols);
+  TimedScope ts(GP_KT_LIVE_STATS, bytes);
+  unsigned long long* total = reinterpret_cast<unsigned long long*>(device_total);
+  if (total) GP_CHECK(hipMemsetAsync(total, 0, sizeof(unsigned long long), 0));
+  if (lp.A.rows > 0) {
+    hipLaunchKernelGGL(k_live_stats, dim3(grid_for(lp.A.rows)), dim3(kBlock), 0, 0, lp.A,
+                       static_cast<const uint8_t*>(plan.keep_row_mask), static_cast<const uint8_t*>(plan.keep_col_mask),
+                       s.row_nnz, s.empty_row_mask, s.singleton_row_mask, s.singleton_row_col,
+                       s.singleton_row_val, total);
+    GP_LAUNCH_CHECK("live stats rows");
+  }
+  if (lp.AT.rows > 0) {
+    hipLaunchKernelGGL(k_live_stats, dim3(grid_for(lp.AT.rows)), dim3(kBlock), 0, 0, lp.AT,
+                       static_cast<const uint8_t*>(plan.keep_col_mask), static_cast<const uint8_t*>(plan.keep_row_mask),
+                       s.col_nnz, s.empty_col_mask, s.singleton_column_mask, s.singleton_column_row,
+                       s.singleton_column_val, static_cast<unsigned long long*>(nullptr));
+    GP_LAUNCH_CHECK("live stats cols");
+  }
+  s.live_masks = 1;
+}
+
+// ---- row activity summary ------------------------------------------------------------------
+namespace {
+__global__ void k_activity(gp_csr A, const double* l, const double* u, double zero_tol, const uint8_t* keep_row,
+                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
+                           int32_t* max_pinf) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.rows) return;
+  RowAcc lo, hi;
+  int32_t ni = 0, pi = 0;
+  const bool live = !keep_row || keep_row[i];
+  if (live) {
+    const gp_offset p0 = A.row_ptr[i], p1 = A.row_ptr[i + 1];
+    for (gp_offset p = p0; p < p1; ++p) {
+      lo.pos(p - p0);
+      hi.pos(p - p0);
+      const int32_t j = A.col_idx[p];
+      if (keep_col && !keep_col[j]) continue;
+      const double a = A.vals[p];
+      if (fabs(a) <= zero_tol) continue;
+      const double lj = l[j], uj = u[j];
+      const double tlo = a >= 0.0 ? a * lj : a * uj;
+      const double thi = a >= 0.0 ? a * uj : a * lj;
+      if (isfinite(tlo)) lo.add(tlo);
+      else if (tlo < 0.0) ++ni;
+      if (isfinite(thi)) hi.add(thi);
+      else if (thi > 0.0) ++pi;
+    }
+  }
+  min_fin[i] = lo.value();
+  max_fin[i] = hi.value();
+  min_ninf[i] = ni;
+  max_pinf[i] = pi;
+}
+}  // namespace
+
+void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
+                          const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin, double* max_fin,
+                          int32_t* min_ninf, int32_t* max_pinf) {
+  if (A.rows <= 0) return;
+  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (double)(A.rows + A.cols) + 24.0 * A.rows;
+  TimedScope ts(GP_KT_ACTIVITY, bytes);
+  hipLaunchKernelGGL(k_activity, dim3(grid_for(A.rows)), dim3(kBlock), 0, 0, A, l, u, zero_tol, keep_row, keep_col,
+                     min_fin, max_fin, min_ninf, max_pinf);
+  GP_LAUNCH_CHECK("activity summary");
+}
+
+}  // namespace gp

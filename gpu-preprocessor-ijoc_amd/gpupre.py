@@ -1,0 +1,513 @@
+"""ctypes bindings of the MI355X presolve engine (libgpupre.so) and of the
+synthetic LP generator (liblpgen.so).
+
+The engine's C ABI is declared in include/gpupre.h.  This module is plumbing
+for tests, the bench and Python callers: it mirrors the C structs, uploads
+host LPs and exposes preprocess / postprocess / free_summary with the
+reference's semantics (gpu_preprocessor.hpp:22-39).  It never falls back to
+CPU code: if the shared library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpupre.so")
+LPGEN_PATH = os.path.join(HERE, "liblpgen.so")
+
+OFFSET = C.c_int32  # gp_offset (NnzOffset, int32 build)
+NP_OFFSET = np.int32
+
+RULES = [
+    "fixed_variables", "empty_columns", "empty_rows", "singleton_rows", "singleton_columns",
+    "bound_strengthening", "redundant_rows", "dual_fix", "doubleton_equations",
+    "parallel_rows", "parallel_columns",
+]
+RULE = {name: i for i, name in enumerate(RULES)}
+PHASES = ["cleanup", "propagation", "aggregation", "parallel", "end"]
+KERNEL_FAMILIES = ["live_stats", "activity", "bound_candidates", "compact", "transpose",
+                   "rules_other", "hash_sort"]
+
+
+class GpCsr(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("nnz", OFFSET),
+                ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("vals", C.c_void_p)]
+
+
+class GpDeviceLp(C.Structure):
+    _fields_ = [("A", GpCsr), ("AT", GpCsr), ("c", C.c_void_p), ("AL", C.c_void_p),
+                ("AU", C.c_void_p), ("l", C.c_void_p), ("u", C.c_void_p),
+                ("obj_constant", C.c_double)]
+
+
+class GpParams(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_int32), ("max_iters", C.c_int32), ("max_time", C.c_double),
+        ("verbose", C.c_int32), ("record_postsolve_tape", C.c_int32),
+        ("record_postsolve_tape_cpu", C.c_int32), ("enable_presolve", C.c_int32),
+        ("enable_folding", C.c_int32), ("folding_tolerance", C.c_double),
+        ("feasibility_tol", C.c_double), ("bound_tol", C.c_double), ("zero_tol", C.c_double),
+        ("bound_strengthening_upper_relaxation_factor", C.c_double),
+        ("bound_strengthening_max_rounds", C.c_int32),
+        ("bound_strengthening_max_bound_only_rounds", C.c_int32),
+        ("bound_strengthening_preserve_free_zero_cost", C.c_int32),
+        ("doubleton_equations_max_fill_in_proxy", C.c_int32),
+        ("doubleton_equations_scan", C.c_int32),
+        ("doubleton_equations_min_selected_per_batch", C.c_int32),
+        ("doubleton_equations_min_selected_ratio", C.c_double),
+        ("doubleton_equations_max_batch_rounds", C.c_int32),
+        ("doubleton_equations_max_time", C.c_double),
+        ("enable_infeasible_and_fixed_variables", C.c_int32), ("enable_empty_rows", C.c_int32),
+        ("enable_singleton_rows", C.c_int32), ("enable_infeasible_and_redundant_rows", C.c_int32),
+        ("enable_bound_strengthening", C.c_int32), ("enable_parallel_rows", C.c_int32),
+        ("enable_empty_columns", C.c_int32), ("enable_singleton_columns", C.c_int32),
+        ("enable_doubleton_equations", C.c_int32), ("enable_dual_fix", C.c_int32),
+        ("dual_fix_max_nonzero_cost_col_degree", C.c_int32),
+        ("enable_parallel_columns", C.c_int32), ("parallel_columns_max_abs_ratio", C.c_double),
+        ("parallel_columns_scale_guard_max_group_size", C.c_int32),
+        ("parallel_columns_scale_guard_min_groups", C.c_int32),
+        ("enable_structure_specific_rules", C.c_int32), ("enable_startup", C.c_int32),
+        ("bound_strengthening_bound_only_nnz_round_budget", C.c_int64),
+        ("collect_trace", C.c_int32), ("learned_schedule", C.c_int32),
+        ("scheduler_weights", C.c_void_p), ("kernel_timing", C.c_int32),
+    ]
+
+
+class GpRoundTrace(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("next_phase", C.c_int32), ("nnz_before", C.c_int64),
+                ("nnz_after", C.c_int64), ("rows_live", C.c_int32), ("cols_live", C.c_int32),
+                ("fired_mask", C.c_int32), ("skipped_mask", C.c_int32), ("seconds", C.c_double)]
+
+
+GP_KT_COUNT = 7
+
+
+class GpSummary(C.Structure):
+    _fields_ = [
+        ("preprocessing_seconds", C.c_double), ("workspace_setup_seconds", C.c_double),
+        ("workspace_release_seconds", C.c_double), ("presolve_enabled", C.c_int32),
+        ("folding_applied", C.c_int32), ("original_rows", C.c_int32), ("original_cols", C.c_int32),
+        ("reduced_rows", C.c_int32), ("reduced_cols", C.c_int32), ("reduced_nnz", OFFSET),
+        ("iterations", C.c_int32), ("stopped_at_iteration_limit", C.c_int32),
+        ("has_change", C.c_int32), ("elapsed_seconds", C.c_double),
+        ("obj_constant_delta", C.c_double), ("has_infeasible", C.c_int32),
+        ("has_unbounded", C.c_int32), ("reduced_lp", GpDeviceLp), ("owns_reduced_lp", C.c_int32),
+        ("record", C.c_void_p), ("trace_count", C.c_int32), ("trace", C.POINTER(GpRoundTrace)),
+        ("kernel_ms", C.c_double * GP_KT_COUNT), ("kernel_launches", C.c_int64 * GP_KT_COUNT),
+        ("kernel_bytes", C.c_double * GP_KT_COUNT),
+    ]
+
+
+class GpRecordSizes(C.Structure):
+    _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
+                ("fixed_count", C.c_int32), ("record_count", C.c_int64),
+                ("index_count", C.c_int64), ("value_count", C.c_int64)]
+
+
+class GpPlan(C.Structure):
+    _fields_ = [
+        ("keep_row_mask", C.c_void_p), ("keep_col_mask", C.c_void_p), ("new_c", C.c_void_p),
+        ("new_l", C.c_void_p), ("new_u", C.c_void_p), ("new_AL", C.c_void_p),
+        ("new_AU", C.c_void_p), ("rule_entry_keep_col", C.c_void_p),
+        ("propagation_reference_l", C.c_void_p), ("propagation_reference_u", C.c_void_p),
+        ("obj_constant_delta", C.c_double), ("has_change", C.c_int32),
+        ("has_row_action", C.c_int32), ("has_col_action", C.c_int32),
+        ("has_redundant_row_deletion", C.c_int32), ("has_row_deletion", C.c_int32),
+        ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32), ("has_new_A", C.c_int32),
+        ("new_A", GpCsr), ("tape", C.c_void_p),
+    ]
+
+
+class GpStats(C.Structure):
+    _fields_ = [
+        ("live_masks", C.c_int32), ("live_nnz", C.c_int64), ("row_nnz", C.c_void_p),
+        ("empty_row_mask", C.c_void_p), ("singleton_row_mask", C.c_void_p),
+        ("singleton_row_col", C.c_void_p), ("singleton_row_val", C.c_void_p),
+        ("col_nnz", C.c_void_p), ("empty_col_mask", C.c_void_p),
+        ("singleton_column_mask", C.c_void_p), ("singleton_column_row", C.c_void_p),
+        ("singleton_column_val", C.c_void_p),
+    ]
+
+
+PLAN_FLAGS = ["has_change", "has_row_action", "has_col_action", "has_redundant_row_deletion",
+              "has_row_deletion", "has_infeasible", "has_unbounded"]
+
+
+# ---------------------------------------------------------------- host LP container
+@dataclass
+class HostLP:
+    """CSR LP on the host: min c'x + obj_constant, AL <= Ax <= AU, l <= x <= u."""
+    m: int
+    n: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    vals: np.ndarray
+    c: np.ndarray
+    l: np.ndarray
+    u: np.ndarray
+    AL: np.ndarray
+    AU: np.ndarray
+    obj_constant: float = 0.0
+    x0: np.ndarray | None = None
+    y0: np.ndarray | None = None
+    at: tuple | None = field(default=None, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def transpose(self):
+        """(row_ptr, col_idx, vals) of A^T with rows of each column in increasing order."""
+        if self.at is None:
+            rows = np.repeat(np.arange(self.m, dtype=np.int32), np.diff(self.row_ptr))
+            order = np.argsort(self.col_idx, kind="stable")
+            counts = np.bincount(self.col_idx, minlength=self.n)
+            rp = np.zeros(self.n + 1, dtype=NP_OFFSET)
+            np.cumsum(counts, out=rp[1:])
+            self.at = (rp, rows[order].astype(np.int32), self.vals[order].copy())
+        return self.at
+
+    @staticmethod
+    def from_dense(A, c, l, u, AL, AU, obj_constant=0.0) -> "HostLP":
+        A = np.asarray(A, dtype=np.float64)
+        m, n = A.shape
+        rp = [0]
+        ci, va = [], []
+        for i in range(m):
+            nz = np.nonzero(A[i])[0]
+            ci.extend(nz.tolist())
+            va.extend(A[i, nz].tolist())
+            rp.append(len(ci))
+        f = lambda v, k: np.asarray(v, dtype=np.float64).reshape(k).copy()
+        return HostLP(m, n, np.asarray(rp, dtype=NP_OFFSET), np.asarray(ci, dtype=np.int32),
+                      np.asarray(va, dtype=np.float64), f(c, n), f(l, n), f(u, n), f(AL, m), f(AU, m),
+                      float(obj_constant))
+
+
+# ---------------------------------------------------------------- library loading
+_lib = None
+_lpgen = None
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "gp_last_error": (C.c_char_p, []),
+        "gp_abi_version": (C.c_int, []),
+        "gp_default_params": (None, [C.POINTER(GpParams)]),
+        "gp_preprocess": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpParams), C.POINTER(GpSummary)]),
+        "gp_postprocess": (C.c_int, [vp, vp, vp, C.POINTER(GpSummary), C.POINTER(GpDeviceLp), vp, vp, vp]),
+        "gp_free_summary": (None, [C.POINTER(GpSummary)]),
+        "gp_record_get_sizes": (C.c_int, [C.POINTER(GpSummary), C.POINTER(GpRecordSizes)]),
+        "gp_record_copy": (C.c_int, [C.POINTER(GpSummary)] + [vp] * 10),
+        "gp_plan_create": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpPlan)]),
+        "gp_plan_destroy": (None, [C.POINTER(GpPlan)]),
+        "gp_plan_enable_tape": (C.c_int, [C.POINTER(GpPlan)]),
+        "gp_stats_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(GpStats)]),
+        "gp_stats_destroy": (None, [C.POINTER(GpStats)]),
+        "gp_refresh_live_stats": (C.c_int, [C.POINTER(GpStats), C.POINTER(GpDeviceLp), C.POINTER(GpPlan)]),
+        "gp_rule_apply": (C.c_int, [C.c_int32, C.POINTER(GpPlan), C.POINTER(GpDeviceLp), C.POINTER(GpStats),
+                                     C.POINTER(GpParams)]),
+        "gp_plan_tape_sizes": (C.c_int, [C.POINTER(GpPlan), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64)]),
+        "gp_plan_tape_copy": (C.c_int, [C.POINTER(GpPlan)] + [vp] * 6),
+        "gp_row_activity_summary": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp, vp]),
+        "gp_transpose_csr": (C.c_int, [C.POINTER(GpCsr), C.POINTER(GpCsr)]),
+        "gp_compact_csr": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.POINTER(GpCsr)]),
+        "gp_csr_free": (None, [C.POINTER(GpCsr)]),
+        "gp_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+        "gp_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+        "gp_free": (C.c_int, [vp]),
+        "gp_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "gp_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+        "gp_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "gp_synchronize": (C.c_int, []),
+        "gp_lp_upload": (C.c_int, [C.c_int32, C.c_int32, OFFSET, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
+                                    C.POINTER(GpDeviceLp)]),
+        "gp_lp_free": (None, [C.POINTER(GpDeviceLp)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gpupre error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int):
+    if code != 0:
+        raise GpError(code, load().gp_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    check(load().gp_device_count(C.byref(n)))
+    return int(n.value)
+
+
+def default_params(**overrides) -> GpParams:
+    p = GpParams()
+    load().gp_default_params(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def parity_params(**overrides) -> GpParams:
+    """Reference defaults with folding and structure-specific rules off."""
+    base = dict(enable_folding=0, enable_structure_specific_rules=0)
+    base.update(overrides)
+    return default_params(**base)
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceArray:
+    """A device allocation owned by Python (gp_malloc / gp_free)."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        check(load().gp_malloc(C.byref(self.ptr), max(1, self.nbytes)))
+
+    @staticmethod
+    def from_host(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.nbytes)
+        if a.nbytes:
+            check(load().gp_memcpy_h2d(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return d
+
+    def free(self):
+        if self.ptr:
+            check(load().gp_free(self.ptr))
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_host(ptr, count: int, dtype) -> np.ndarray:
+    out = np.empty(int(count), dtype=dtype)
+    if count:
+        check(load().gp_memcpy_d2h(out.ctypes.data_as(C.c_void_p), ptr, out.nbytes))
+    return out
+
+
+def to_device(ptr, a: np.ndarray):
+    a = np.ascontiguousarray(a)
+    if a.nbytes:
+        check(load().gp_memcpy_h2d(ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+
+class DeviceLP:
+    """A host LP uploaded to HBM (A by copy, A^T built on the device)."""
+
+    def __init__(self, lp: HostLP):
+        self.lp = GpDeviceLp()
+        self.m, self.n = lp.m, lp.n
+        check(load().gp_lp_upload(lp.m, lp.n, lp.nnz, _ptr(lp.row_ptr), _ptr(lp.col_idx), _ptr(lp.vals),
+                                  _ptr(lp.c), _ptr(lp.AL), _ptr(lp.AU), _ptr(lp.l), _ptr(lp.u),
+                                  float(lp.obj_constant), C.byref(self.lp)))
+
+    def free(self):
+        if self.lp.A.row_ptr:
+            load().gp_lp_free(C.byref(self.lp))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_lp_to_host(d: GpDeviceLp) -> HostLP:
+    m, n, nnz = d.A.rows, d.A.cols, d.A.nnz
+    h = HostLP(m, n, to_host(d.A.row_ptr, m + 1, NP_OFFSET), to_host(d.A.col_idx, nnz, np.int32),
+               to_host(d.A.vals, nnz, np.float64), to_host(d.c, n, np.float64), to_host(d.l, n, np.float64),
+               to_host(d.u, n, np.float64), to_host(d.AL, m, np.float64), to_host(d.AU, m, np.float64),
+               float(d.obj_constant))
+    h.at = (to_host(d.AT.row_ptr, n + 1, NP_OFFSET), to_host(d.AT.col_idx, d.AT.nnz, np.int32),
+            to_host(d.AT.vals, d.AT.nnz, np.float64))
+    return h
+
+
+# ---------------------------------------------------------------- public pipeline
+@dataclass
+class Record:
+    m0: int
+    n0: int
+    m1: int
+    n1: int
+    row_red2org: np.ndarray
+    col_red2org: np.ndarray
+    fixed_idx: np.ndarray
+    fixed_val: np.ndarray
+    types: np.ndarray
+    index_starts: np.ndarray
+    value_starts: np.ndarray
+    dual_modes: np.ndarray
+    indices: np.ndarray
+    vals: np.ndarray
+
+
+class Summary:
+    """gp_summary wrapper (FoldingPipelineSummary); call free() when done."""
+
+    def __init__(self):
+        self.s = GpSummary()
+
+    def free(self):
+        load().gp_free_summary(C.byref(self.s))
+
+    def __getattr__(self, name):
+        return getattr(self.s, name)
+
+    def record(self) -> Record:
+        sz = GpRecordSizes()
+        check(load().gp_record_get_sizes(C.byref(self.s), C.byref(sz)))
+        a = lambda k, dt: np.empty(int(k), dtype=dt)
+        rec = Record(sz.m0, sz.n0, sz.m1, sz.n1, a(sz.m1, np.int32), a(sz.n1, np.int32),
+                     a(sz.fixed_count, np.int32), a(sz.fixed_count, np.float64),
+                     a(sz.record_count, np.int32), a(sz.record_count + 1, NP_OFFSET),
+                     a(sz.record_count + 1, NP_OFFSET), a(sz.record_count, np.uint8),
+                     a(sz.index_count, np.int32), a(sz.value_count, np.float64))
+        check(load().gp_record_copy(C.byref(self.s), _ptr(rec.row_red2org), _ptr(rec.col_red2org),
+                                    _ptr(rec.fixed_idx), _ptr(rec.fixed_val), _ptr(rec.types),
+                                    _ptr(rec.index_starts), _ptr(rec.value_starts), _ptr(rec.dual_modes),
+                                    _ptr(rec.indices), _ptr(rec.vals)))
+        return rec
+
+    def reduced(self) -> HostLP:
+        return device_lp_to_host(self.s.reduced_lp)
+
+    def trace(self):
+        return [self.s.trace[i] for i in range(self.s.trace_count)]
+
+    def kernel_stats(self):
+        return {name: dict(ms=self.s.kernel_ms[k], launches=int(self.s.kernel_launches[k]),
+                           bytes=self.s.kernel_bytes[k]) for k, name in enumerate(KERNEL_FAMILIES)}
+
+
+def preprocess(dlp: DeviceLP, params: GpParams | None = None) -> Summary:
+    if params is None:
+        params = default_params()
+    s = Summary()
+    check(load().gp_preprocess(C.byref(dlp.lp), C.byref(params), C.byref(s.s)))
+    return s
+
+
+def postprocess(summary: Summary, x: np.ndarray, y: np.ndarray, z: np.ndarray, original: DeviceLP | None):
+    """x, y, z in reduced order (host arrays, uploaded here); returns original-order host arrays."""
+    rec = GpRecordSizes()
+    check(load().gp_record_get_sizes(C.byref(summary.s), C.byref(rec)))
+    dx, dy, dz = (DeviceArray.from_host(np.asarray(v, dtype=np.float64)) for v in (x, y, z))
+    xo = np.empty(rec.n0)
+    yo = np.empty(rec.m0)
+    zo = np.empty(rec.n0)
+    check(load().gp_postprocess(dx.ptr, dy.ptr, dz.ptr, C.byref(summary.s),
+                                C.byref(original.lp) if original is not None else None,
+                                _ptr(xo), _ptr(yo), _ptr(zo)))
+    return xo, yo, zo
+
+
+# ---------------------------------------------------------------- generator
+class LpgenSpec(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("seed", C.c_uint64), ("degree_mode", C.c_int32),
+                ("mean_degree", C.c_double), ("zipf_s", C.c_double), ("zipf_cap", C.c_int32),
+                ("frac_equality", C.c_double), ("frac_free", C.c_double),
+                ("frac_singleton_rows", C.c_double), ("frac_doubleton_rows", C.c_double),
+                ("frac_parallel_rows", C.c_double), ("frac_duplicate_cols", C.c_double),
+                ("frac_fixed_cols", C.c_double), ("frac_empty_cols", C.c_double)]
+
+
+class LpgenLp(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int32), ("row_ptr", C.c_void_p),
+                ("col_idx", C.c_void_p), ("vals", C.c_void_p), ("c", C.c_void_p), ("l", C.c_void_p),
+                ("u", C.c_void_p), ("AL", C.c_void_p), ("AU", C.c_void_p), ("x0", C.c_void_p),
+                ("y0", C.c_void_p), ("obj_constant", C.c_double), ("zipf_scale", C.c_double)]
+
+
+def _load_lpgen():
+    global _lpgen
+    if _lpgen is None:
+        if not os.path.exists(LPGEN_PATH):
+            raise RuntimeError(f"{LPGEN_PATH} is not built; run __graft_entry__.build()")
+        g = C.CDLL(LPGEN_PATH)
+        g.lpgen_default_spec.argtypes = [C.POINTER(LpgenSpec), C.c_int32, C.c_int32, C.c_uint64]
+        g.lpgen_default_spec.restype = None
+        g.lpgen_generate.argtypes = [C.POINTER(LpgenSpec), C.POINTER(LpgenLp)]
+        g.lpgen_generate.restype = C.c_int
+        g.lpgen_free.argtypes = [C.POINTER(LpgenLp)]
+        g.lpgen_free.restype = None
+        _lpgen = g
+    return _lpgen
+
+
+# Workload definitions (sizes of the synthetic presolve benchmarks).
+WORKLOADS = {
+    # 2,000 x 3,000, ~6 nnz per column, uniform rows
+    "small": dict(m=2000, n=3000, degree_mode=0, mean_degree=6.0),
+    # 200,000 x 300,000, ~2M nnz, Zipf(1.5) column degree capped at 500, 1% empty columns
+    "medium": dict(m=200_000, n=300_000, degree_mode=1, mean_degree=2_000_000 / 300_000, frac_empty_cols=0.01),
+    # 5M x 8M, ~60M nnz, same recipe
+    "large": dict(m=5_000_000, n=8_000_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
+                  frac_empty_cols=0.01),
+}
+
+
+def generate(m: int, n: int, seed: int = 0, **spec_overrides) -> HostLP:
+    g = _load_lpgen()
+    spec = LpgenSpec()
+    g.lpgen_default_spec(C.byref(spec), int(m), int(n), int(seed))
+    for k, v in spec_overrides.items():
+        setattr(spec, k, v)
+    raw = LpgenLp()
+    if g.lpgen_generate(C.byref(spec), C.byref(raw)) != 0:
+        raise RuntimeError("lpgen_generate failed")
+
+    def arr(p, k, dt):
+        if k == 0:
+            return np.empty(0, dtype=dt)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(k,)).copy()
+
+    lp = HostLP(raw.m, raw.n, arr(raw.row_ptr, raw.m + 1, np.int32), arr(raw.col_idx, raw.nnz, np.int32),
+                arr(raw.vals, raw.nnz, np.float64), arr(raw.c, raw.n, np.float64), arr(raw.l, raw.n, np.float64),
+                arr(raw.u, raw.n, np.float64), arr(raw.AL, raw.m, np.float64), arr(raw.AU, raw.m, np.float64),
+                float(raw.obj_constant), arr(raw.x0, raw.n, np.float64), arr(raw.y0, raw.m, np.float64))
+    g.lpgen_free(C.byref(raw))
+    return lp
+
+
+def generate_workload(name: str, seed: int = 0) -> HostLP:
+    w = dict(WORKLOADS[name])
+    m, n = w.pop("m"), w.pop("n")
+    return generate(m, n, seed, **w)

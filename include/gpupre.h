@@ -1,0 +1,358 @@
+/*
+ * gpupre.h -- C ABI of the MI355X LP presolve / postsolve engine.
+ *
+ * Every entry point is extern "C" with plain pointers and sizes.  Matrices and
+ * vectors passed as `gp_device_lp` live in device (HBM) memory and stay owned
+ * by the caller; the library never modifies them.  Errors are integer status
+ * codes; gp_last_error() returns a thread-local message for the last failure.
+ * Infeasible / unbounded models are flags in gp_summary, not errors.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   gp_preprocess      <- gpu_preprocessor::preprocess()
+ *                         cpp/include/gpu_preprocessor/gpu_preprocessor.hpp:22-24
+ *   gp_postprocess     <- gpu_preprocessor::postprocess()
+ *                         cpp/include/gpu_preprocessor/gpu_preprocessor.hpp:30-32
+ *   gp_free_summary    <- gpu_preprocessor::free_summary()
+ *                         cpp/include/gpu_preprocessor/gpu_preprocessor.hpp:35-39
+ *   gp_device_lp       <- presolve::LPInfoGpu / DeviceCsrMatrix
+ *                         cpp/include/gpu_preprocessor/presolve/presolve_structs.hpp:160-180
+ *   gp_params          <- presolve::PresolveParams   (same field names, same defaults)
+ *                         cpp/include/gpu_preprocessor/presolve/presolve_structs.hpp:28-158
+ *   gp_summary         <- folding::FoldingPipelineSummary + presolve::GpuPresolveSummary
+ *                         cpp/include/gpu_preprocessor/folding/folding_structs.hpp:94-110,
+ *                         cpp/include/gpu_preprocessor/presolve/gpu_presolve.hpp:14-30
+ *   gp_plan / gp_stats <- presolve::PresolvePlanGpu / PresolveStatsGpu
+ *                         cpp/include/gpu_preprocessor/presolve/presolve_structs.hpp:182-205,378-439
+ *   gp_rule_*          <- apply_rule_*(PresolvePlanGpu&, const LPInfoGpu&, const PresolveStatsGpu&,
+ *                         const PresolveParams&) in cpp/src/presolve/rules/rule_<name>.cu
+ *   gp_refresh_live_stats <- refresh_live_stats  cpp/src/presolve/gpu_presolve_kernels.cu:493-517
+ *   gp_transpose_csr   <- transpose_csr64        cpp/src/presolve/transpose64.cu:69-117
+ *   gp_compact_csr     <- compact_csr_by_masks_gpu cpp/src/presolve/gpu_presolve.cu:1793-1930
+ */
+#ifndef GPUPRE_H
+#define GPUPRE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_ABI_VERSION 1
+
+/* NnzOffset of the reference build (int32 default, offset.hpp:7-19). */
+typedef int32_t gp_offset;
+
+/* Status codes. */
+enum {
+  GP_OK = 0,
+  GP_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference       */
+  GP_ERR_LENGTH = 2,           /* std::length_error (offset overflow)           */
+  GP_ERR_RUNTIME = 3,          /* std::runtime_error (HIP failure, bad tape)    */
+  GP_ERR_NO_DEVICE = 4,
+  GP_ERR_ALLOCATION = 5
+};
+
+/* Postsolve reduction types (PostsolveReductionType, presolve_structs.hpp:268-282). */
+enum {
+  GP_RED_FIXED_VARIABLE = 0,
+  GP_RED_INFINITE_BOUND_DUAL_FIX = 1,
+  GP_RED_SINGLETON_COLUMN_SUBSTITUTION = 2,
+  GP_RED_PARALLEL_COLUMN = 3,
+  GP_RED_PARALLEL_ROW = 4,
+  GP_RED_DELETED_ROW = 5,
+  GP_RED_ROW_LOWER_BOUND_CHANGE = 8,
+  GP_RED_ROW_UPPER_BOUND_CHANGE = 9,
+  GP_RED_ROW_DUAL_SHIFT = 10,
+  GP_RED_VARIABLE_BOUND_CHANGE = 11,
+  GP_RED_ROW_IMPLIED_BOUND_CHANGE = 12,
+  GP_RED_DOUBLETON_EQUATION = 13,
+  GP_RED_FME_COL = 14
+};
+
+/* PostsolveDualMode (presolve_structs.hpp:284-288). */
+enum { GP_DUAL_NONE = 0, GP_DUAL_EXACT = 1, GP_DUAL_MINIMAL = 2 };
+
+/* Rule identifiers used by gp_rule_apply, the per-round trace and the learned
+ * pass scheduler's output layer. */
+enum {
+  GP_RULE_FIXED_VARIABLES = 0,
+  GP_RULE_EMPTY_COLUMNS = 1,
+  GP_RULE_EMPTY_ROWS = 2,
+  GP_RULE_SINGLETON_ROWS = 3,
+  GP_RULE_SINGLETON_COLUMNS = 4,
+  GP_RULE_BOUND_STRENGTHENING = 5,
+  GP_RULE_REDUNDANT_ROWS = 6,
+  GP_RULE_DUAL_FIX = 7,
+  GP_RULE_DOUBLETON_EQUATIONS = 8,
+  GP_RULE_PARALLEL_ROWS = 9,
+  GP_RULE_PARALLEL_COLUMNS = 10,
+  GP_RULE_COUNT = 11
+};
+
+/* Scheduler phases (gpu_presolve.cu:2466-2475). */
+enum { GP_PHASE_CLEANUP = 0, GP_PHASE_PROPAGATION = 1, GP_PHASE_AGGREGATION = 2,
+       GP_PHASE_PARALLEL = 3, GP_PHASE_END = 4 };
+
+/* DeviceCsrMatrix: rows x cols, row_ptr[rows+1], col_idx/vals[nnz]. */
+typedef struct gp_csr {
+  int32_t rows;
+  int32_t cols;
+  gp_offset nnz;
+  gp_offset* row_ptr;
+  int32_t* col_idx;
+  double* vals;
+} gp_csr;
+
+/* LPInfoGpu: min c'x + obj_constant  s.t.  AL <= A x <= AU,  l <= x <= u.
+ * AT is A transposed, stored as CSR (rows = columns of A). */
+typedef struct gp_device_lp {
+  gp_csr A;
+  gp_csr AT;
+  double* c;
+  double* AL;
+  double* AU;
+  double* l;
+  double* u;
+  double obj_constant;
+} gp_device_lp;
+
+/* PresolveParams with the reference's field names and defaults.  Booleans are
+ * int32 (0/1).  Fields of rules this engine does not build (folding and the
+ * structure-specific Startup rules) are accepted and ignored. */
+typedef struct gp_params {
+  int32_t struct_size; /* sizeof(gp_params); set by gp_default_params */
+  int32_t max_iters;
+  double max_time;
+  int32_t verbose;
+  int32_t record_postsolve_tape;
+  int32_t record_postsolve_tape_cpu;
+  int32_t enable_presolve;
+  int32_t enable_folding;
+  double folding_tolerance;
+  double feasibility_tol;
+  double bound_tol;
+  double zero_tol;
+  double bound_strengthening_upper_relaxation_factor;
+  int32_t bound_strengthening_max_rounds;
+  int32_t bound_strengthening_max_bound_only_rounds;
+  int32_t bound_strengthening_preserve_free_zero_cost;
+  int32_t doubleton_equations_max_fill_in_proxy;
+  int32_t doubleton_equations_scan;
+  int32_t doubleton_equations_min_selected_per_batch;
+  double doubleton_equations_min_selected_ratio;
+  int32_t doubleton_equations_max_batch_rounds;
+  double doubleton_equations_max_time;
+  int32_t enable_infeasible_and_fixed_variables;
+  int32_t enable_empty_rows;
+  int32_t enable_singleton_rows;
+  int32_t enable_infeasible_and_redundant_rows;
+  int32_t enable_bound_strengthening;
+  int32_t enable_parallel_rows;
+  int32_t enable_empty_columns;
+  int32_t enable_singleton_columns;
+  int32_t enable_doubleton_equations;
+  int32_t enable_dual_fix;
+  int32_t dual_fix_max_nonzero_cost_col_degree;
+  int32_t enable_parallel_columns;
+  double parallel_columns_max_abs_ratio;
+  int32_t parallel_columns_scale_guard_max_group_size;
+  int32_t parallel_columns_scale_guard_min_groups;
+  int32_t enable_structure_specific_rules;
+  int32_t enable_startup;
+  int64_t bound_strengthening_bound_only_nnz_round_budget;
+  /* Engine extensions (no reference counterpart). */
+  int32_t collect_trace;     /* record one gp_round_trace per scheduler step    */
+  int32_t learned_schedule;  /* 1: consult the learned pass scheduler           */
+  const float* scheduler_weights; /* host MLP weights when learned_schedule=1    */
+  int32_t kernel_timing;     /* 1: time every kernel family with HIP events     */
+} gp_params;
+
+/* One scheduler execution step (the reference prints these as
+ * ">>> [schedule-progress]" lines, gpu_presolve.cu:2500-2592). */
+typedef struct gp_round_trace {
+  int32_t phase;
+  int32_t next_phase;
+  int64_t nnz_before;
+  int64_t nnz_after;
+  int32_t rows_live;
+  int32_t cols_live;
+  int32_t fired_mask;    /* bit r set when rule r changed the plan */
+  int32_t skipped_mask;  /* bit r set when the learned scheduler skipped rule r */
+  double seconds;
+} gp_round_trace;
+
+/* Per-kernel-family device time accumulated when params.kernel_timing=1. */
+enum {
+  GP_KT_LIVE_STATS = 0,
+  GP_KT_ACTIVITY = 1,
+  GP_KT_BOUND_CANDIDATES = 2,
+  GP_KT_COMPACT = 3,
+  GP_KT_TRANSPOSE = 4,
+  GP_KT_RULES_OTHER = 5,
+  GP_KT_HASH_SORT = 6,
+  GP_KT_COUNT = 7
+};
+
+/* FoldingPipelineSummary + GpuPresolveSummary, flattened.  Value semantics:
+ * gp_preprocess fills a caller-provided struct; gp_free_summary releases its
+ * device memory and zeroes it (idempotent). */
+typedef struct gp_summary {
+  double preprocessing_seconds;
+  double workspace_setup_seconds;
+  double workspace_release_seconds;
+  int32_t presolve_enabled;
+  int32_t folding_applied;
+  int32_t original_rows;
+  int32_t original_cols;
+  int32_t reduced_rows;
+  int32_t reduced_cols;
+  gp_offset reduced_nnz;
+  int32_t iterations;
+  int32_t stopped_at_iteration_limit;
+  int32_t has_change;
+  double elapsed_seconds;
+  double obj_constant_delta;
+  int32_t has_infeasible;
+  int32_t has_unbounded;
+  gp_device_lp reduced_lp;
+  int32_t owns_reduced_lp;
+  void* record; /* opaque recovery record (device resident) */
+  int32_t trace_count;
+  gp_round_trace* trace; /* host array, trace_count entries */
+  double kernel_ms[GP_KT_COUNT];
+  int64_t kernel_launches[GP_KT_COUNT];
+  double kernel_bytes[GP_KT_COUNT]; /* algorithmic bytes per family */
+} gp_summary;
+
+/* Host view of a recovery record. */
+typedef struct gp_record_sizes {
+  int32_t m0, n0, m1, n1;
+  int32_t fixed_count;
+  int64_t record_count;
+  int64_t index_count;
+  int64_t value_count;
+} gp_record_sizes;
+
+/* ---- public pipeline ---------------------------------------------------- */
+
+const char* gp_last_error(void);
+int gp_abi_version(void);
+void gp_default_params(gp_params* params);
+
+/* preprocess(): presolve the device LP `input` (caller-owned, unmodified).
+ * On a run that changes nothing the reduced LP borrows the input arrays. */
+int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary* out);
+
+/* postprocess(): x, y, z are device arrays in reduced order (n1, m1, n1);
+ * x_out (n0), y_out (m0), z_out (n0) are host arrays in original order.
+ * Convention c - A'y - z = 0.  `original` may be NULL (no dual projection). */
+int gp_postprocess(const double* x, const double* y, const double* z,
+                   const gp_summary* summary, const gp_device_lp* original,
+                   double* x_out, double* y_out, double* z_out);
+
+/* free_summary(): safe on NULL and on repeated calls. */
+void gp_free_summary(gp_summary* summary);
+
+/* Recovery record export (tests, diagnostics).  Any output pointer may be NULL. */
+int gp_record_get_sizes(const gp_summary* summary, gp_record_sizes* sizes);
+int gp_record_copy(const gp_summary* summary,
+                   int32_t* row_red2org, int32_t* col_red2org,
+                   int32_t* fixed_idx, double* fixed_val,
+                   int32_t* types, gp_offset* index_starts, gp_offset* value_starts,
+                   uint8_t* dual_modes, int32_t* indices, double* vals);
+
+/* ---- rule-level operator API (PresolvePlanGpu / PresolveStatsGpu) -------- */
+
+/* Deferred plan: masks and replacement vectors over the physical LP. */
+typedef struct gp_plan {
+  uint8_t* keep_row_mask;
+  uint8_t* keep_col_mask;
+  double* new_c;
+  double* new_l;
+  double* new_u;
+  double* new_AL;
+  double* new_AU;
+  const uint8_t* rule_entry_keep_col;
+  const double* propagation_reference_l;
+  const double* propagation_reference_u;
+  double obj_constant_delta;
+  int32_t has_change;
+  int32_t has_row_action;
+  int32_t has_col_action;
+  int32_t has_redundant_row_deletion;
+  int32_t has_row_deletion;
+  int32_t has_infeasible;
+  int32_t has_unbounded;
+  int32_t has_new_A;
+  gp_csr new_A;   /* replacement matrix built by doubleton substitution */
+  void* tape;     /* opaque device tape (local indices); NULL = do not record */
+} gp_plan;
+
+typedef struct gp_stats {
+  int32_t live_masks;
+  int64_t live_nnz;
+  int32_t* row_nnz;
+  uint8_t* empty_row_mask;
+  uint8_t* singleton_row_mask;
+  int32_t* singleton_row_col;
+  double* singleton_row_val;
+  int32_t* col_nnz;
+  uint8_t* empty_col_mask;
+  uint8_t* singleton_column_mask;
+  int32_t* singleton_column_row;
+  double* singleton_column_val;
+} gp_stats;
+
+/* allocate_plan(): masks = 1, new_* = copies of lp vectors (device). */
+int gp_plan_create(const gp_device_lp* lp, gp_plan* plan);
+void gp_plan_destroy(gp_plan* plan);
+int gp_stats_create(int32_t m, int32_t n, gp_stats* stats);
+void gp_stats_destroy(gp_stats* stats);
+int gp_refresh_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan* plan);
+
+/* Apply one general rule (GP_RULE_*) to the plan. */
+int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp,
+                  const gp_stats* stats, const gp_params* params);
+/* Export the plan tape (local indices). */
+int gp_plan_tape_sizes(const gp_plan* plan, int64_t* records, int64_t* indices, int64_t* values);
+int gp_plan_tape_copy(const gp_plan* plan, int32_t* types, gp_offset* index_starts,
+                      gp_offset* value_starts, uint8_t* dual_modes, int32_t* indices,
+                      double* vals);
+int gp_plan_enable_tape(gp_plan* plan);
+
+/* Row activity summary (compute_row_activity_summary): finite min/max sums
+ * (compensated, fixed order) and counts of infinite contributions. */
+int gp_row_activity_summary(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                            const uint8_t* keep_row, const uint8_t* keep_col,
+                            double* min_fin, double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
+
+/* ---- matrix primitives --------------------------------------------------- */
+
+/* AT = transpose(A); AT arrays are allocated by the library (gp_csr_free). */
+int gp_transpose_csr(const gp_csr* A, gp_csr* AT);
+/* Stable compaction by keep masks (NULL = keep all). */
+int gp_compact_csr(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, gp_csr* out);
+void gp_csr_free(gp_csr* matrix);
+
+/* ---- device memory helpers (for harnesses without a GPU runtime) ---------- */
+int gp_device_count(int32_t* count);
+int gp_malloc(void** ptr, size_t bytes);
+int gp_free(void* ptr);
+int gp_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int gp_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int gp_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int gp_synchronize(void);
+/* Upload a host CSR LP; AT is built on the device by gp_transpose_csr. */
+int gp_lp_upload(int32_t m, int32_t n, gp_offset nnz, const gp_offset* row_ptr,
+                 const int32_t* col_idx, const double* vals, const double* c,
+                 const double* AL, const double* AU, const double* l, const double* u,
+                 double obj_constant, gp_device_lp* out);
+void gp_lp_free(gp_device_lp* lp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUPRE_H */
