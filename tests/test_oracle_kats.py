@@ -10,3 +10,11 @@ CASES = load_kats()
 def test_oracle_reference_kat(case):
     b, states = run_kat(case, "oracle")
     check_kat(case, b, states)
+
+
+from kat_programs import PROGRAMS  # noqa: E402
+
+
+@pytest.mark.parametrize("name,program", PROGRAMS, ids=[p[0] for p in PROGRAMS])
+def test_oracle_reference_program(name, program):
+    program("oracle")
