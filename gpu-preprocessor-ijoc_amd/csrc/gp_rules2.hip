@@ -1,13 +1,1820 @@
 // gp_rules2.hip -- HIP kernels of the column-side and structural rules:
-// singleton columns, dual fix, doubleton equations, parallel rows/columns.
+//   singleton columns    rule_singleton_columns.cu:518-1190, 1562-1827
+//   dual fix             rule_dual_fix.cu:186-296, 325-645
+//   doubleton equations  rule_doubleton_equations.cu:150-272, 824-1340, 1512-1812 (batch form)
+//   parallel rows        rule_parallel_rows.cu:141-466, 648-888
+//   parallel columns     rule_parallel_columns.cu:189-612, 636-1127
+// Execution-order policy as in gp_rules.hip: row shifts are blocked Neumaier
+// sums in CSR order (RowAcc), objective deltas use det_sum, and every work
+// unit the reference runs per hash group / per selected pair / per row owner
+// runs sequentially inside one thread, so results do not depend on scheduling.
+#include <climits>
+
 #include "gp_tape.hpp"
 
 namespace gp {
 
-void rule_singleton_columns(gp_plan&, const gp_device_lp&, const gp_stats&, const gp_params&) {}
-void rule_dual_fix(gp_plan&, const gp_device_lp&, const gp_stats&, const gp_params&) {}
-void rule_doubleton_equations(gp_plan&, const gp_device_lp&, const gp_stats&, const gp_params&) {}
-void rule_parallel_rows(gp_plan&, const gp_device_lp&, const gp_stats&, const gp_params&) {}
-void rule_parallel_columns(gp_plan&, const gp_device_lp&, const gp_stats&, const gp_params&) {}
+namespace {
+inline bool tape_on(const gp_plan& plan, const gp_params& p) { return plan.tape && p.record_postsolve_tape; }
+
+struct Flags {
+  DVec<int32_t> d;
+  int32_t h[8] = {};
+  Flags() : d(8) { GP_CHECK(hipMemsetAsync(d.data(), 0, sizeof(int32_t) * 8, 0)); }
+  const int32_t* read() {
+    d2h(h, d.data(), 8);
+    return h;
+  }
+};
+
+__device__ inline int64_t gtid() { return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; }
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = gtid();
+  if (i < n) p[i] = v;
+}
+
+// AL/AU -= sum over stored entries of kept row i whose column has sel[col] of
+// a * val[col] (blocked CSR order).
+__global__ void k_shift_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* sel, const double* val, double* AL,
+                             double* AU) {
+  const int64_t i = gtid();
+  if (i >= A.rows || !keep_row[i]) return;
+  RowAcc acc;
+  const gp_offset p0 = A.row_ptr[i];
+  for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+    acc.pos(q - p0);
+    const int32_t j = A.col_idx[q];
+    if (sel[j]) acc.add(A.vals[q] * val[j]);
+  }
+  const double s = acc.value();
+  AL[i] -= s;
+  AU[i] -= s;
+}
+
+// ============================================================== singleton columns
+enum : uint8_t { SC_NONE = 0, SC_SET_ROW_SIDE = 1, SC_TIGHTEN_LHS = 2, SC_TIGHTEN_RHS = 3, SC_ELIM_FREE = 4,
+                 SC_ELIM_ABOVE = 5, SC_ELIM_BELOW = 6 };
+
+struct ScAct {
+  double* fmin;
+  double* fmax;
+  int32_t* cnt;  // 4 per row: min_neg, min_pos, max_neg, max_pos
+  uint8_t* invalid;
+};
+
+__global__ void k_sc_count(int32_t n, int32_t m, const uint8_t* keep_col, const uint8_t* keep_row, const uint8_t* smask,
+                           const int32_t* srow, uint8_t* needed, int32_t* flags) {
+  const int64_t j = gtid();
+  if (j >= n || !keep_col[j] || !smask[j]) return;
+  const int32_t r = srow[j];
+  if (r >= 0 && r < m && keep_row[r]) {
+    needed[r] = 1;
+    atomicAdd(&flags[0], 1);
+  }
+}
+
+// _kernel_singleton_row_activity_cache: live entries, no zero filter.
+__global__ void k_sc_activity(gp_csr A, const uint8_t* needed, const uint8_t* keep_row, const uint8_t* keep_col,
+                              const double* l, const double* u, ScAct act) {
+  const int64_t i = gtid();
+  if (i >= A.rows || !needed[i] || !keep_row[i]) return;
+  RowAcc mn, mx;
+  int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  uint8_t bad = 0;
+  const gp_offset p0 = A.row_ptr[i];
+  for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+    mn.pos(q - p0);
+    mx.pos(q - p0);
+    const int32_t j = A.col_idx[q];
+    if (!keep_col[j]) continue;
+    const double a = A.vals[q];
+    const double tmin = a >= 0.0 ? a * l[j] : a * u[j];
+    const double tmax = a >= 0.0 ? a * u[j] : a * l[j];
+    if (isnan(tmin) || isnan(tmax)) {
+      bad = 1;
+      continue;
+    }
+    if (isinf(tmin)) {
+      if (tmin < 0.0) ++c0;
+      else ++c1;
+    } else {
+      mn.add(tmin);
+    }
+    if (isinf(tmax)) {
+      if (tmax < 0.0) ++c2;
+      else ++c3;
+    } else {
+      mx.add(tmax);
+    }
+  }
+  act.fmin[i] = mn.value();
+  act.fmax[i] = mx.value();
+  act.cnt[4 * i + 0] = c0;
+  act.cnt[4 * i + 1] = c1;
+  act.cnt[4 * i + 2] = c2;
+  act.cnt[4 * i + 3] = c3;
+  act.invalid[i] = bad;
+}
+
+__device__ bool sc_rest_bounds(const ScAct& act, int32_t row, double a, double lj, double uj, double* rmin,
+                               double* rmax) {
+  if (act.invalid[row]) return false;
+  const double tmin = a >= 0.0 ? a * lj : a * uj;
+  const double tmax = a >= 0.0 ? a * uj : a * lj;
+  if (isnan(tmin) || isnan(tmax)) return false;
+  int32_t min_neg = act.cnt[4 * row + 0], min_pos = act.cnt[4 * row + 1];
+  int32_t max_neg = act.cnt[4 * row + 2], max_pos = act.cnt[4 * row + 3];
+  double vmin = act.fmin[row], vmax = act.fmax[row];
+  if (isinf(tmin)) {
+    if (tmin < 0.0) --min_neg;
+    else --min_pos;
+  } else {
+    vmin -= tmin;
+  }
+  if (isinf(tmax)) {
+    if (tmax < 0.0) --max_neg;
+    else --max_pos;
+  } else {
+    vmax -= tmax;
+  }
+  if (min_neg < 0 || min_pos < 0 || max_neg < 0 || max_pos < 0 || (min_neg > 0 && min_pos > 0) ||
+      (max_neg > 0 && max_pos > 0))
+    return false;
+  *rmin = min_neg > 0 ? -INFINITY : (min_pos > 0 ? INFINITY : vmin);
+  *rmax = max_neg > 0 ? -INFINITY : (max_pos > 0 ? INFINITY : vmax);
+  return !isnan(*rmin) && !isnan(*rmax);
+}
+
+__global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col,
+                             const uint8_t* smask, const int32_t* srow, const double* sval, const double* c,
+                             const double* l, const double* u, const double* AL, const double* AU, ScAct act,
+                             double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags) {
+  const int64_t j = gtid();
+  if (j >= n || !keep_col[j] || !smask[j]) return;
+  const int32_t row = srow[j];
+  if (row < 0 || row >= m || !keep_row[row]) return;
+  const double a = sval[j];
+  if (fabs(a) <= zt) return;
+  int32_t live = 0;
+  for (gp_offset q = A.row_ptr[row]; q < A.row_ptr[row + 1]; ++q)
+    if (keep_col[A.col_idx[q]] && ++live > 1) break;
+  if (live <= 1) return;
+  const double lhs = AL[row], rhs = AU[row], cj = c[j], lj = l[j], uj = u[j];
+  if ((cj > zt && a > zt && !isfinite(lhs) && !isfinite(lj)) ||
+      (cj > zt && a < -zt && !isfinite(rhs) && !isfinite(lj)) ||
+      (cj < -zt && a < -zt && !isfinite(lhs) && !isfinite(uj)) ||
+      (cj < -zt && a > zt && !isfinite(rhs) && !isfinite(uj))) {
+    atomicMax(&flags[1], 1);
+    return;
+  }
+  double rmin = 0.0, rmax = 0.0;
+  if (!sc_rest_bounds(act, row, a, lj, uj, &rmin, &rmax)) return;
+  uint8_t ac = SC_NONE;
+  double sd = 0.0;
+  if (isfinite(lhs) && isfinite(rhs) && fabs(lhs - rhs) <= tol) {
+    const double x1 = (rhs - rmin) / a, x2 = (rhs - rmax) / a;
+    const double ilb = fmin(x1, x2), iub = fmax(x1, x2);
+    const bool above = !isfinite(uj) || iub <= uj + tol;
+    const bool below = !isfinite(lj) || ilb >= lj - tol;
+    if (!(above || below)) return;
+    ac = above && below ? SC_ELIM_FREE : (above ? SC_ELIM_ABOVE : SC_ELIM_BELOW);
+    sd = rhs;
+  } else {
+    bool above = true, below = true;
+    if (isfinite(uj)) {
+      double iu = INFINITY;
+      if (a > 0.0 && isfinite(rhs)) iu = (rhs - rmin) / a;
+      else if (a < 0.0 && isfinite(lhs)) iu = (lhs - rmax) / a;
+      above = iu <= uj + tol;
+    }
+    if (isfinite(lj)) {
+      double il = -INFINITY;
+      if (a > 0.0 && isfinite(lhs)) il = (lhs - rmax) / a;
+      else if (a < 0.0 && isfinite(rhs)) il = (rhs - rmin) / a;
+      below = il >= lj - tol;
+    }
+    if (above && below) {
+      if ((cj > zt && a > 0.0) || (cj < -zt && a < 0.0)) sd = lhs;
+      else if ((cj > zt && a < 0.0) || (cj < -zt && a > 0.0)) sd = rhs;
+      else sd = isfinite(lhs) ? lhs : rhs;
+      ac = isfinite(sd) ? SC_SET_ROW_SIDE : SC_NONE;
+    } else if (((cj < -zt && a > 0.0 && above) || (cj > zt && a < 0.0 && below)) && isfinite(rhs)) {
+      sd = rhs;
+      ac = SC_TIGHTEN_LHS;
+    } else if (((cj > zt && a > 0.0 && below) || (cj < -zt && a < 0.0 && above)) && isfinite(lhs)) {
+      sd = lhs;
+      ac = SC_TIGHTEN_RHS;
+    }
+    if (ac == SC_NONE) return;
+  }
+  action[j] = ac;
+  side[j] = sd;
+  atomicMin(&owner[row], ac == SC_ELIM_FREE ? static_cast<int32_t>(j - INT_MAX) : static_cast<int32_t>(j));
+}
+
+__global__ void k_sc_stage(int32_t m, int32_t n, int32_t* owner, const uint8_t* action, const double* side,
+                           const double* sval, const double* c, const double* l, const double* u, const double* AL,
+                           const double* AU, double tol, uint8_t* row_delete, uint8_t* lower_change,
+                           uint8_t* upper_change, uint8_t* col_delete, double* AL_new, double* AU_new,
+                           double* contrib, int32_t* flags) {
+  const int64_t row = gtid();
+  if (row >= m) return;
+  const int32_t o = owner[row];
+  const int32_t j = o < 0 ? o + INT_MAX : o;
+  if (j < 0 || j >= n) return;
+  owner[row] = j;
+  const double lhs = AL[row], rhs = AU[row];
+  double nl = lhs, nu = rhs;
+  const uint8_t ac = action[j];
+  if (ac == SC_ELIM_FREE || ac == SC_ELIM_ABOVE || ac == SC_ELIM_BELOW) {
+    const double a = sval[j];
+    contrib[j] = c[j] * rhs / a;
+    if (ac == SC_ELIM_FREE) {
+      row_delete[row] = 1;
+    } else {
+      const bool above = ac == SC_ELIM_ABOVE;
+      const double bound = above ? l[j] : u[j];
+      const bool keep_lower = above ? a < 0.0 : a > 0.0;
+      const double shifted = nu - a * bound;
+      if (keep_lower) {
+        nl = shifted;
+        nu = INFINITY;
+      } else {
+        nl = -INFINITY;
+        nu = shifted;
+      }
+    }
+    col_delete[j] = 1;
+    AL_new[row] = nl;
+    AU_new[row] = nu;
+    atomicMax(&flags[2], 1);
+    return;
+  }
+  if (ac == SC_SET_ROW_SIDE) {
+    const double sd = side[j];
+    if (!isfinite(lhs) || fabs(lhs - sd) > tol) lower_change[row] = 1;
+    if (!isfinite(rhs) || fabs(rhs - sd) > tol) upper_change[row] = 1;
+    nl = sd;
+    nu = sd;
+  } else if (ac == SC_TIGHTEN_LHS) {
+    lower_change[row] = 1;
+    nl = rhs;
+  } else if (ac == SC_TIGHTEN_RHS) {
+    upper_change[row] = 1;
+    nu = lhs;
+  }
+  AL_new[row] = nl;
+  AU_new[row] = nu;
+  atomicMax(&flags[3], 1);
+}
+
+// _kernel_apply_singleton_objective_updates_deterministic: plain sequential sum
+// in AT order starting from c.
+__global__ void k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col_delete, const int32_t* owner,
+                          const double* sval, const double* c, double* c_new) {
+  const int64_t j = gtid();
+  if (j >= AT.rows || !keep_col[j] || col_delete[j]) return;
+  const int32_t n = AT.rows;
+  double updated = c[j];
+  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
+    const int32_t o = owner[AT.col_idx[q]];
+    if (o < 0 || o >= n || !col_delete[o]) continue;
+    updated += -(c[o] * AT.vals[q] / sval[o]);
+  }
+  c_new[j] = updated;
+}
+
+__global__ void k_sc_sub_counts(gp_csr A, int32_t n, const int32_t* owner, const uint8_t* col_delete,
+                                const uint8_t* keep_col, double zt, int64_t* r, int64_t* ni, int64_t* nv) {
+  const int64_t row = gtid();
+  if (row >= A.rows) return;
+  const int32_t j = owner[row];
+  int64_t k = 0;
+  bool sel = j >= 0 && j < n && col_delete[j];
+  if (sel) {
+    double pivot = 0.0;
+    for (gp_offset q = A.row_ptr[row]; q < A.row_ptr[row + 1]; ++q) {
+      const int32_t col = A.col_idx[q];
+      if (col == j) pivot = A.vals[q];
+      else if (keep_col[col]) ++k;
+    }
+    sel = fabs(pivot) > zt;
+  }
+  r[row] = sel ? 1 : 0;
+  ni[row] = sel ? 3 + k : 0;
+  nv[row] = sel ? 6 + k : 0;
+}
+
+// SingletonColumnSubstitution: idx [col, row, k, support cols], vals [pivot,
+// side, l, u, c, row_deleted, support coefficients].
+__global__ void k_sc_sub_pack(TapeView out, gp_csr A, const int64_t* r, const int64_t* ni, const int64_t* nv,
+                              const int32_t* owner, const uint8_t* keep_col, const uint8_t* row_delete,
+                              const double* side, const double* l, const double* u, const double* c) {
+  const int64_t row = gtid();
+  if (row >= A.rows || r[row + 1] == r[row]) return;
+  const int32_t j = owner[row];
+  const int64_t rec = out.rbase + r[row];
+  const int64_t i0 = out.ibase + ni[row], v0 = out.vbase + nv[row];
+  int64_t k = 0;
+  double pivot = 0.0;
+  for (gp_offset q = A.row_ptr[row]; q < A.row_ptr[row + 1]; ++q) {
+    const int32_t col = A.col_idx[q];
+    if (col == j) {
+      pivot = A.vals[q];
+    } else if (keep_col[col]) {
+      out.idx[i0 + 3 + k] = col;
+      out.vals[v0 + 6 + k] = A.vals[q];
+      ++k;
+    }
+  }
+  out.types[rec] = GP_RED_SINGLETON_COLUMN_SUBSTITUTION;
+  out.modes[rec] = GP_DUAL_MINIMAL;
+  out.idx[i0] = j;
+  out.idx[i0 + 1] = static_cast<int32_t>(row);
+  out.idx[i0 + 2] = static_cast<int32_t>(k);
+  out.vals[v0] = pivot;
+  out.vals[v0 + 1] = side[j];
+  out.vals[v0 + 2] = l[j];
+  out.vals[v0 + 3] = u[j];
+  out.vals[v0 + 4] = c[j];
+  out.vals[v0 + 5] = row_delete[row] ? 1.0 : 0.0;
+  out.istarts[rec + 1] = static_cast<gp_offset>(i0 + 3 + k);
+  out.vstarts[rec + 1] = static_cast<gp_offset>(v0 + 6 + k);
+}
+
+__global__ void k_rds_counts(int32_t m, const uint8_t* lo, const uint8_t* up, int64_t* r, int64_t* ni, int64_t* nv) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  const int64_t s = (lo[i] || up[i]) ? 1 : 0;
+  r[i] = s;
+  ni[i] = s;
+  nv[i] = s;
+}
+
+__global__ void k_rds_pack(TapeView out, int32_t m, const uint8_t* lo, const uint8_t* up, const int64_t* r) {
+  const int64_t i = gtid();
+  if (i >= m || !(lo[i] || up[i])) return;
+  const int64_t rec = out.rbase + r[i];
+  out.types[rec] = GP_RED_ROW_DUAL_SHIFT;
+  out.modes[rec] = GP_DUAL_MINIMAL;
+  out.idx[out.ibase + r[i]] = static_cast<int32_t>(i);
+  out.vals[out.vbase + r[i]] = 0.0;
+  out.istarts[rec + 1] = static_cast<gp_offset>(out.ibase + r[i] + 1);
+  out.vstarts[rec + 1] = static_cast<gp_offset>(out.vbase + r[i] + 1);
+}
+
+__global__ void k_sc_commit(int32_t m, int32_t n, const uint8_t* row_delete, const int32_t* owner,
+                            const double* AL_new, const double* AU_new, const uint8_t* col_delete,
+                            const double* c_new, int update_cost, uint8_t* keep_row, uint8_t* keep_col, double* AL,
+                            double* AU, double* c) {
+  const int64_t q = gtid();
+  if (q < m) {
+    if (row_delete[q]) keep_row[q] = 0;
+    if (owner[q] >= 0 && owner[q] < n) {
+      AL[q] = AL_new[q];
+      AU[q] = AU_new[q];
+    }
+  }
+  if (q < n) {
+    if (col_delete[q]) keep_col[q] = 0;
+    if (update_cost && keep_col[q]) c[q] = c_new[q];
+  }
+}
+
+// ============================================================== dual fix
+__global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_t* keep_row, const double* c,
+                                const double* l, const double* u, const double* AL, const double* AU, double zt,
+                                double bt, int32_t maxdeg, const int32_t* live_col_nnz, uint8_t* fixed, uint8_t* inf,
+                                uint8_t* ffix, double* val, double* contrib, int32_t* flags) {
+  const int64_t j = gtid();
+  if (j >= AT.rows) return;
+  fixed[j] = 0;
+  inf[j] = 0;
+  ffix[j] = 0;
+  val[j] = 0.0;
+  contrib[j] = 0.0;
+  if (!keep_col[j]) return;
+  const double lj = l[j], uj = u[j];
+  if (lj > uj + bt) {
+    atomicMax(&flags[0], 1);
+    return;
+  }
+  const gp_offset a0 = AT.row_ptr[j], a1 = AT.row_ptr[j + 1];
+  const int64_t degree = live_col_nnz ? static_cast<int64_t>(live_col_nnz[j]) : static_cast<int64_t>(a1 - a0);
+  if (fabs(c[j]) > zt && degree > maxdeg) return;
+  bool down = false, up = false;
+  for (gp_offset q = a0; q < a1; ++q) {
+    const int32_t r = AT.col_idx[q];
+    if (!keep_row[r]) continue;
+    const double a = AT.vals[q];
+    if (a > zt) {
+      down = down || isfinite(AL[r]);
+      up = up || isfinite(AU[r]);
+    } else if (a < -zt) {
+      down = down || isfinite(AU[r]);
+      up = up || isfinite(AL[r]);
+    }
+    if (down && up) break;
+  }
+  const double cj = c[j];
+  bool fix = false, infinite = false;
+  double v = 0.0;
+  if (cj > zt) {
+    if (!down) {
+      if (!isfinite(lj)) {
+        atomicMax(&flags[0], 2);
+        return;
+      }
+      v = lj;
+      fix = true;
+    }
+  } else if (cj < -zt) {
+    if (!up) {
+      if (!isfinite(uj)) {
+        atomicMax(&flags[0], 2);
+        return;
+      }
+      v = uj;
+      fix = true;
+    }
+  } else if (fabs(cj) <= zt) {
+    if (!down) {
+      fix = true;
+      if (isfinite(lj)) {
+        v = lj;
+      } else {
+        v = -INFINITY;
+        infinite = true;
+      }
+    } else if (!up) {
+      fix = true;
+      if (isfinite(uj)) {
+        v = uj;
+      } else {
+        v = INFINITY;
+        infinite = true;
+      }
+    }
+  }
+  if (fix) {
+    fixed[j] = 1;
+    inf[j] = infinite ? 1 : 0;
+    ffix[j] = infinite ? 0 : 1;
+    val[j] = v;
+    contrib[j] = infinite ? 0.0 : cj * v;
+    atomicMax(&flags[1], 1);
+    if (infinite) atomicMax(&flags[2], 1);
+  }
+}
+
+__global__ void k_df_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* ffix, const uint8_t* inf,
+                          const double* val, double* shift, uint8_t* row_remove, uint8_t* keep_after,
+                          int32_t* flags) {
+  const int64_t i = gtid();
+  if (i >= A.rows) return;
+  double s = 0.0;
+  uint8_t rem = 0;
+  if (keep_row[i]) {
+    RowAcc acc;
+    const gp_offset p0 = A.row_ptr[i];
+    for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+      acc.pos(q - p0);
+      const int32_t j = A.col_idx[q];
+      if (ffix[j]) acc.add(A.vals[q] * val[j]);
+      if (inf[j]) rem = 1;
+    }
+    s = acc.value();
+    if (rem) atomicMax(&flags[3], 1);
+  }
+  shift[i] = s;
+  row_remove[i] = rem;
+  keep_after[i] = (keep_row[i] && !rem) ? 1 : 0;
+}
+
+__device__ inline int64_t entry_len(const gp_csr& A, int32_t r, const uint8_t* entry) {
+  if (!entry) return A.row_ptr[r + 1] - A.row_ptr[r];
+  int64_t k = 0;
+  for (gp_offset e = A.row_ptr[r]; e < A.row_ptr[r + 1]; ++e) k += entry[A.col_idx[e]] != 0;
+  return k;
+}
+
+__global__ void k_df_inf_counts(gp_csr A, gp_csr AT, const uint8_t* fixed, const uint8_t* inf,
+                                const uint8_t* keep_row, const uint8_t* entry, int64_t* r, int64_t* ni,
+                                int64_t* nv) {
+  const int64_t j = gtid();
+  if (j >= AT.rows) return;
+  const bool sel = fixed[j] && inf[j];
+  int64_t size = 0;
+  if (sel)
+    for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
+      const int32_t row = AT.col_idx[q];
+      if (keep_row[row]) size += 1 + entry_len(A, row, entry);
+    }
+  r[j] = sel ? 1 : 0;
+  ni[j] = sel ? 3 + size : 0;
+  nv[j] = sel ? 1 + size : 0;
+}
+
+// InfiniteBoundDualFix: idx [+-1, col, active rows, (len, cols...)...],
+// vals [finite bound, (row side, coefficients...)...].
+__global__ void k_df_inf_pack(TapeView out, gp_csr A, gp_csr AT, const uint8_t* fixed, const uint8_t* inf,
+                              const uint8_t* keep_row, const uint8_t* entry, const double* val, const double* l,
+                              const double* u, const double* AL, const double* AU, const int64_t* r,
+                              const int64_t* ni, const int64_t* nv) {
+  const int64_t j = gtid();
+  if (j >= AT.rows || !(fixed[j] && inf[j])) return;
+  const int64_t rec = out.rbase + r[j];
+  int64_t ip = out.ibase + ni[j], vp = out.vbase + nv[j];
+  const bool positive = val[j] > 0.0;
+  out.types[rec] = GP_RED_INFINITE_BOUND_DUAL_FIX;
+  out.modes[rec] = GP_DUAL_EXACT;
+  out.idx[ip++] = positive ? 1 : -1;
+  out.idx[ip++] = static_cast<int32_t>(j);
+  const int64_t count_pos = ip++;
+  out.vals[vp++] = positive ? l[j] : u[j];
+  int32_t active = 0;
+  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
+    const int32_t row = AT.col_idx[q];
+    if (!keep_row[row]) continue;
+    ++active;
+    const int64_t header = ip++;
+    int32_t len = 0;
+    out.vals[vp++] = isfinite(AL[row]) ? AL[row] : AU[row];
+    for (gp_offset e = A.row_ptr[row]; e < A.row_ptr[row + 1]; ++e) {
+      if (entry && !entry[A.col_idx[e]]) continue;
+      ++len;
+      out.idx[ip++] = A.col_idx[e];
+      out.vals[vp++] = A.vals[e];
+    }
+    out.idx[header] = len;
+  }
+  out.idx[count_pos] = active;
+  out.istarts[rec + 1] = static_cast<gp_offset>(ip);
+  out.vstarts[rec + 1] = static_cast<gp_offset>(vp);
+}
+
+__global__ void k_df_apply(int32_t m, int32_t n, const uint8_t* fixed, const double* val, double* l, double* u,
+                           uint8_t* keep_col, const double* shift, const uint8_t* row_remove, double* AL, double* AU,
+                           uint8_t* keep_row) {
+  const int64_t k = gtid();
+  if (k < n && fixed[k]) {
+    l[k] = val[k];
+    u[k] = val[k];
+    keep_col[k] = 0;
+  }
+  if (k < m) {
+    AL[k] -= shift[k];
+    AU[k] -= shift[k];
+    if (row_remove[k]) keep_row[k] = 0;
+  }
+}
+
+// ============================================================== hashing
+constexpr uint64_t kFnvOffset = 0xcbf29ce484222325ULL;
+constexpr uint64_t kFnvPrime = 0x100000001b3ULL;
+__device__ inline uint64_t fnv_mix(uint64_t h, uint64_t x) { return (h ^ x) * kFnvPrime; }
+__device__ inline uint64_t dbits(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
+__device__ inline uint64_t as_u64(int64_t v) { return static_cast<uint64_t>(v); }
+
+// ============================================================== parallel rows
+__global__ void k_pr_hash(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col, double tol, int use_xor,
+                          uint64_t* hash, int32_t* ids) {
+  const int64_t i = gtid();
+  if (i >= A.rows) return;
+  ids[i] = static_cast<int32_t>(i);
+  if (!keep_row[i]) {
+    hash[i] = as_u64(i);
+    return;
+  }
+  const gp_offset p1 = A.row_ptr[i + 1];
+  gp_offset first = A.row_ptr[i];
+  while (keep_col && first < p1 && !keep_col[A.col_idx[first]]) ++first;
+  int64_t len = 0;
+  for (gp_offset q = first; q < p1; ++q)
+    if (!keep_col || keep_col[A.col_idx[q]]) ++len;
+  if (len <= 0) {
+    hash[i] = fnv_mix(kFnvOffset, as_u64(i));
+    return;
+  }
+  const double pivot = A.vals[first];
+  if (fabs(pivot) <= tol) {
+    hash[i] = fnv_mix(kFnvOffset, as_u64(i));
+    return;
+  }
+  uint64_t h = fnv_mix(kFnvOffset, as_u64(len)), agg = 0;
+  for (gp_offset q = first; q < p1; ++q) {
+    const int32_t col = A.col_idx[q];
+    if (keep_col && !keep_col[col]) continue;
+    const uint64_t nb = dbits(A.vals[q] / pivot);
+    if (use_xor) {
+      agg ^= fnv_mix(fnv_mix(kFnvOffset, as_u64(col)), nb);
+    } else {
+      h = fnv_mix(h, as_u64(col));
+      h = fnv_mix(h, nb);
+    }
+  }
+  hash[i] = use_xor ? fnv_mix(h, agg) : h;
+}
+
+__device__ bool pr_ratio(const gp_csr& A, int32_t i, int32_t k, const uint8_t* keep_col, double tol, double* out) {
+  gp_offset li = A.row_ptr[i], le = A.row_ptr[i + 1], ri = A.row_ptr[k], re = A.row_ptr[k + 1];
+  while (keep_col && li < le && !keep_col[A.col_idx[li]]) ++li;
+  while (keep_col && ri < re && !keep_col[A.col_idx[ri]]) ++ri;
+  if (li == le && ri == re) {
+    *out = 1.0;
+    return true;
+  }
+  double ratio = 0.0;
+  bool set = false;
+  while (li < le && ri < re) {
+    if (A.col_idx[li] != A.col_idx[ri]) return false;
+    const double ai = A.vals[li], ak = A.vals[ri];
+    if (!set) {
+      if (fabs(ak) <= tol) return false;
+      ratio = ai / ak;
+      set = true;
+    }
+    if (fabs(ai - ratio * ak) > tol) return false;
+    ++li;
+    while (keep_col && li < le && !keep_col[A.col_idx[li]]) ++li;
+    ++ri;
+    while (keep_col && ri < re && !keep_col[A.col_idx[ri]]) ++ri;
+  }
+  *out = ratio;
+  return set && li == le && ri == re;
+}
+
+__device__ inline void scaled_interval(double lo, double hi, double ratio, double* sl, double* su) {
+  const double rl = lo * ratio, ru = hi * ratio;
+  if (ratio >= 0.0) {
+    *sl = rl;
+    *su = ru;
+  } else {
+    *sl = ru;
+    *su = rl;
+  }
+}
+
+struct PrRecs {
+  int32_t* step;
+  int32_t* kept;
+  double* vals;  // 5 per deleted row: ratio, kept AL, kept AU, deleted AL, deleted AU
+};
+
+__global__ void k_pr_groups(int32_t m, const uint64_t* hash, const int32_t* ids, gp_csr A, const uint8_t* keep_row,
+                            const uint8_t* keep_col, double* AL, double* AU, double ctol, double itol,
+                            uint8_t* row_delete, PrRecs rec, int32_t* flags) {
+  const int64_t gs = gtid();
+  if (gs >= m) return;
+  if (gs > 0 && hash[gs - 1] == hash[gs]) return;
+  int64_t ge = gs;
+  while (ge + 1 < m && hash[ge + 1] == hash[gs]) ++ge;
+  if (ge == gs) return;
+  int32_t seed = -1;
+  bool seed_eq = false;
+  for (int64_t a = gs; a <= ge; ++a) {
+    const int32_t row = ids[a];
+    if (!keep_row[row]) continue;
+    const bool eq = isfinite(AL[row]) && isfinite(AU[row]) && fabs(AL[row] - AU[row]) <= itol;
+    if (seed < 0 || (eq && !seed_eq)) {
+      seed = row;
+      seed_eq = eq;
+    }
+  }
+  if (seed < 0) return;
+  int32_t rep = seed;
+  double rsl = AL[seed], rsu = AU[seed];
+  for (int64_t a = gs; a <= ge; ++a) {
+    const int32_t row = ids[a];
+    if (row == seed || !keep_row[row]) continue;
+    double r = 0.0;
+    if (!pr_ratio(A, seed, row, keep_col, ctol, &r)) continue;
+    double sl, su;
+    scaled_interval(AL[row], AU[row], r, &sl, &su);
+    if (rsl <= sl + itol && su <= rsu + itol) {
+      rep = row;
+      rsl = sl;
+      rsu = su;
+    }
+  }
+  double rl = AL[rep], ru = AU[rep];
+  bool found = false;
+  int32_t step = 0;
+  for (int64_t a = gs; a <= ge; ++a) {
+    const int32_t row = ids[a];
+    if (row == rep || !keep_row[row]) continue;
+    double r = 0.0;
+    if (!pr_ratio(A, rep, row, keep_col, ctol, &r)) continue;
+    found = true;
+    double sl, su;
+    scaled_interval(AL[row], AU[row], r, &sl, &su);
+    if (cmax(rl, sl) > cmin(ru, su) + itol) {
+      atomicMax(&flags[0], 1);
+      return;
+    }
+    ++step;
+    rec.step[row] = step;
+    rec.kept[row] = rep;
+    rec.vals[5 * static_cast<int64_t>(row) + 0] = r;
+    rec.vals[5 * static_cast<int64_t>(row) + 1] = rl;
+    rec.vals[5 * static_cast<int64_t>(row) + 2] = ru;
+    rec.vals[5 * static_cast<int64_t>(row) + 3] = AL[row];
+    rec.vals[5 * static_cast<int64_t>(row) + 4] = AU[row];
+    rl = cmax(rl, sl);
+    ru = cmin(ru, su);
+    row_delete[row] = 1;
+  }
+  if (found) {
+    AL[rep] = rl;
+    AU[rep] = ru;
+  }
+}
+
+__global__ void k_pr_apply(int32_t m, const uint8_t* row_delete, uint8_t* keep_row, int32_t* flags) {
+  const int64_t i = gtid();
+  if (i < m && row_delete[i]) {
+    keep_row[i] = 0;
+    atomicAdd(&flags[1], 1);
+  }
+}
+
+// Record order keys: (step, deleted row).
+__global__ void k_pr_keys(int32_t m, const uint8_t* row_delete, const int32_t* o2n, const int32_t* step,
+                          uint64_t* keys, int32_t* rows) {
+  const int64_t i = gtid();
+  if (i >= m || !row_delete[i]) return;
+  const int32_t t = o2n[i];
+  keys[t] = (static_cast<uint64_t>(static_cast<uint32_t>(step[i])) << 32) | static_cast<uint32_t>(i);
+  rows[t] = static_cast<int32_t>(i);
+}
+
+__global__ void k_pr_pack(TapeView out, int32_t count, const int32_t* rows, PrRecs rec) {
+  const int64_t t = gtid();
+  if (t >= count) return;
+  const int32_t row = rows[t];
+  const int64_t r = out.rbase + t;
+  out.types[r] = GP_RED_PARALLEL_ROW;
+  out.modes[r] = GP_DUAL_EXACT;
+  out.idx[out.ibase + 2 * t] = rec.kept[row];
+  out.idx[out.ibase + 2 * t + 1] = row;
+  for (int k = 0; k < 5; ++k) out.vals[out.vbase + 5 * t + k] = rec.vals[5 * static_cast<int64_t>(row) + k];
+  out.istarts[r + 1] = static_cast<gp_offset>(out.ibase + 2 * (t + 1));
+  out.vstarts[r + 1] = static_cast<gp_offset>(out.vbase + 5 * (t + 1));
+}
+
+// ============================================================== parallel columns
+__device__ inline bool pc_live(const gp_csr& AT, gp_offset q, const uint8_t* keep_row, double zt) {
+  return keep_row[AT.col_idx[q]] && fabs(AT.vals[q]) > zt;
+}
+__device__ inline gp_offset pc_next(const gp_csr& AT, gp_offset q, gp_offset stop, const uint8_t* keep_row, double zt) {
+  while (q < stop && !pc_live(AT, q, keep_row, zt)) ++q;
+  return q;
+}
+
+__global__ void k_pc_hash(gp_csr AT, const uint8_t* keep_col, const uint8_t* keep_row, double tol, uint64_t* hash,
+                          int32_t* ids) {
+  const int64_t j = gtid();
+  if (j >= AT.rows) return;
+  ids[j] = static_cast<int32_t>(j);
+  if (!keep_col[j]) {
+    hash[j] = as_u64(j);
+    return;
+  }
+  int32_t len = 0;
+  double pivot = 0.0;
+  bool set = false;
+  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
+    if (!pc_live(AT, q, keep_row, tol)) continue;
+    ++len;
+    if (!set) {
+      pivot = AT.vals[q];
+      set = true;
+    }
+  }
+  if (!set || len == 0) {
+    hash[j] = fnv_mix(kFnvOffset, as_u64(j));
+    return;
+  }
+  uint64_t h = fnv_mix(kFnvOffset, as_u64(len));
+  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
+    if (!pc_live(AT, q, keep_row, tol)) continue;
+    h = fnv_mix(h, as_u64(AT.col_idx[q]));
+    h = fnv_mix(h, dbits(AT.vals[q] / pivot));
+  }
+  hash[j] = h;
+}
+
+__device__ bool pc_ratio(const gp_csr& AT, int32_t j, int32_t k, const uint8_t* keep_row, double zt, double ctol,
+                         double* out) {
+  gp_offset pj = AT.row_ptr[j], pk = AT.row_ptr[k];
+  const gp_offset sj = AT.row_ptr[j + 1], sk = AT.row_ptr[k + 1];
+  double ratio = 0.0;
+  bool set = false;
+  for (;;) {
+    pj = pc_next(AT, pj, sj, keep_row, zt);
+    pk = pc_next(AT, pk, sk, keep_row, zt);
+    if (pj >= sj || pk >= sk) break;
+    if (AT.col_idx[pj] != AT.col_idx[pk]) return false;
+    const double aj = AT.vals[pj], ak = AT.vals[pk];
+    if (!set) {
+      if (fabs(aj) <= zt) return false;
+      ratio = ak / aj;
+      if (fabs(ratio) <= zt || !isfinite(ratio)) return false;
+      set = true;
+    }
+    const double expected = ratio * aj;
+    const double scale = fmax(fabs(ak), fabs(expected));
+    if (!isfinite(expected) || fabs(ak - expected) > ctol * scale) return false;
+    ++pj;
+    ++pk;
+  }
+  pj = pc_next(AT, pj, sj, keep_row, zt);
+  pk = pc_next(AT, pk, sk, keep_row, zt);
+  if (pj < sj || pk < sk || !set) return false;
+  *out = ratio;
+  return true;
+}
+
+__device__ inline bool out_of_guard(double ratio, double max_abs) {
+  const double ar = fabs(ratio);
+  return max_abs < 1.0 || ar > max_abs || ar * max_abs < 1.0;
+}
+
+__global__ void k_pc_guard(int32_t n, const uint64_t* hash, const int32_t* ids, gp_csr AT, const uint8_t* keep_col,
+                           const uint8_t* keep_row, const double* c, double zt, double ctol, double otol,
+                           double max_abs, int32_t max_group, int32_t* flags) {
+  const int64_t gs = gtid();
+  if (gs >= n) return;
+  if (gs > 0 && hash[gs - 1] == hash[gs]) return;
+  int64_t ge = gs;
+  while (ge + 1 < n && hash[ge + 1] == hash[gs]) ++ge;
+  const int64_t size = ge - gs + 1;
+  if (size < 2 || size > max_group) return;
+  for (int64_t a = gs; a < ge; ++a) {
+    const int32_t j = ids[a];
+    if (!keep_col[j]) continue;
+    for (int64_t b = a + 1; b <= ge; ++b) {
+      const int32_t k = ids[b];
+      if (!keep_col[k]) continue;
+      double r = 0.0;
+      if (!pc_ratio(AT, j, k, keep_row, zt, ctol, &r)) continue;
+      if (fabs(c[k] - r * c[j]) > otol) continue;
+      if (out_of_guard(r, max_abs)) {
+        atomicAdd(&flags[4], 1);
+        return;
+      }
+    }
+  }
+}
+
+__device__ inline double merged_lower(double tl, double sl, double su, double r) {
+  if (r > 0.0) {
+    if (isfinite(tl) && isfinite(sl)) return tl + r * sl;
+  } else if (isfinite(tl) && isfinite(su)) {
+    return tl + r * su;
+  }
+  return -INFINITY;
+}
+__device__ inline double merged_upper(double tu, double sl, double su, double r) {
+  if (r > 0.0) {
+    if (isfinite(tu) && isfinite(su)) return tu + r * su;
+  } else if (isfinite(tu) && isfinite(sl)) {
+    return tu + r * sl;
+  }
+  return INFINITY;
+}
+
+struct PcOut {
+  uint8_t* col_delete;
+  uint8_t* fixed;
+  double* fval;
+  int32_t* merge_to;
+  double* mrec;  // 5 per merged column: ratio, source l, u, target l, u
+};
+
+__global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids, gp_csr AT, const uint8_t* keep_col,
+                            const uint8_t* keep_row, const double* c, double* l, double* u, double zt, double ctol,
+                            double otol, double active_max, PcOut o, int32_t* flags) {
+  const int64_t gs = gtid();
+  if (gs >= n) return;
+  if (gs > 0 && hash[gs - 1] == hash[gs]) return;
+  int64_t ge = gs;
+  while (ge + 1 < n && hash[ge + 1] == hash[gs]) ++ge;
+  for (int64_t a = gs; a < ge; ++a) {
+    const int32_t j = ids[a];
+    if (!keep_col[j] || o.col_delete[j] || o.fixed[j]) continue;
+    for (int64_t b = a + 1; b <= ge; ++b) {
+      const int32_t k = ids[b];
+      if (!keep_col[k] || o.col_delete[k] || o.fixed[k]) continue;
+      if (isfinite(l[j]) && isfinite(u[j]) && isfinite(l[k]) && isfinite(u[k])) {
+        const gp_offset fj = pc_next(AT, AT.row_ptr[j], AT.row_ptr[j + 1], keep_row, zt);
+        const gp_offset fk = pc_next(AT, AT.row_ptr[k], AT.row_ptr[k + 1], keep_row, zt);
+        if (fj < AT.row_ptr[j + 1] && fk < AT.row_ptr[k + 1]) {
+          const double fr = AT.vals[fk] / AT.vals[fj];
+          if (fabs(c[k] - fr * c[j]) > otol) continue;
+        }
+      }
+      double r = 0.0;
+      if (!pc_ratio(AT, j, k, keep_row, zt, ctol, &r)) continue;
+      if (isfinite(active_max) && out_of_guard(r, active_max)) continue;
+      const double gap = c[k] - r * c[j];
+      const double tl = l[j], tu = u[j], sl = l[k], su = u[k];
+      if (fabs(gap) <= otol) {
+        o.merge_to[k] = j;
+        double* mr = o.mrec + 5 * static_cast<int64_t>(k);
+        mr[0] = r;
+        mr[1] = sl;
+        mr[2] = su;
+        mr[3] = tl;
+        mr[4] = tu;
+        l[j] = merged_lower(tl, sl, su, r);
+        u[j] = merged_upper(tu, sl, su, r);
+        o.col_delete[k] = 1;
+        atomicMax(&flags[1], 1);
+        continue;
+      }
+      bool xk_lo = false, xk_up = false, xj_lo = false, xj_up = false;
+      if (gap > otol) {
+        if (r > 0.0) {
+          xk_lo = !isfinite(tu);
+          xj_up = !isfinite(sl);
+        } else {
+          xk_lo = !isfinite(tl);
+          xj_lo = !isfinite(sl);
+        }
+      } else {
+        if (r > 0.0) {
+          xk_up = !isfinite(tl);
+          xj_lo = !isfinite(su);
+        } else {
+          xk_up = !isfinite(tu);
+          xj_up = !isfinite(su);
+        }
+      }
+      if (xk_lo || xk_up) {
+        const double v = xk_lo ? sl : su;
+        if (!isfinite(v)) {
+          atomicMax(&flags[0], 1);
+          return;
+        }
+        o.fixed[k] = 1;
+        o.fval[k] = v;
+        atomicMax(&flags[2], 1);
+        continue;
+      }
+      if (xj_lo || xj_up) {
+        const double v = xj_lo ? tl : tu;
+        if (!isfinite(v)) {
+          atomicMax(&flags[0], 1);
+          return;
+        }
+        o.fixed[j] = 1;
+        o.fval[j] = v;
+        atomicMax(&flags[2], 1);
+        break;
+      }
+    }
+  }
+}
+
+__global__ void k_pc_counts(int32_t n, const uint8_t* col_delete, int64_t* r, int64_t* ni, int64_t* nv) {
+  const int64_t j = gtid();
+  if (j >= n) return;
+  const int64_t s = col_delete[j] ? 1 : 0;
+  r[j] = s;
+  ni[j] = 2 * s;
+  nv[j] = 5 * s;
+}
+
+__global__ void k_pc_pack(TapeView out, int32_t n, const uint8_t* col_delete, const int32_t* merge_to,
+                          const double* mrec, const int64_t* r) {
+  const int64_t j = gtid();
+  if (j >= n || !col_delete[j]) return;
+  const int64_t t = r[j], rec = out.rbase + t;
+  out.types[rec] = GP_RED_PARALLEL_COLUMN;
+  out.modes[rec] = GP_DUAL_MINIMAL;
+  out.idx[out.ibase + 2 * t] = static_cast<int32_t>(j);
+  out.idx[out.ibase + 2 * t + 1] = merge_to[j];
+  for (int k = 0; k < 5; ++k) out.vals[out.vbase + 5 * t + k] = mrec[5 * j + k];
+  out.istarts[rec + 1] = static_cast<gp_offset>(out.ibase + 2 * (t + 1));
+  out.vstarts[rec + 1] = static_cast<gp_offset>(out.vbase + 5 * (t + 1));
+}
+
+__global__ void k_pc_apply(int32_t n, const uint8_t* col_delete, const uint8_t* fixed, const double* fval,
+                           uint8_t* keep_col, double* l, double* u) {
+  const int64_t j = gtid();
+  if (j >= n || !(col_delete[j] || fixed[j])) return;
+  keep_col[j] = 0;
+  if (fixed[j]) {
+    l[j] = fval[j];
+    u[j] = fval[j];
+  }
+}
+
+// ============================================================== doubleton equations
+constexpr double kDblMaxRatioPivot = 1.0e3;
+constexpr int kDblMatchingRounds = 8;
+
+__global__ void k_db_colnnz(gp_csr AT, const uint8_t* keep_row, const uint8_t* keep_col, double zt, int32_t* nnz) {
+  const int64_t j = gtid();
+  if (j >= AT.rows) return;
+  int32_t c = 0;
+  if (keep_col[j])
+    for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q)
+      c += (keep_row[AT.col_idx[q]] && fabs(AT.vals[q]) > zt) ? 1 : 0;
+  nnz[j] = c;
+}
+
+__device__ inline bool db_integral(double num, double den, double tol) {
+  if (fabs(den) <= tol) return false;
+  const double r = fabs(num / den);
+  if (!isfinite(r)) return false;
+  return fabs(r - rint(r)) <= tol;
+}
+
+__device__ int32_t db_fill_in(const gp_csr& AT, const uint8_t* keep_row, double zt, int32_t keep, int32_t elim) {
+  int32_t fill = -1;
+  gp_offset jj = AT.row_ptr[keep], kk = AT.row_ptr[elim];
+  const gp_offset jstop = AT.row_ptr[keep + 1], kstop = AT.row_ptr[elim + 1];
+  if (jstop <= jj || kstop <= kk) return fill;
+  auto live = [&](gp_offset q) { return keep_row[AT.col_idx[q]] && fabs(AT.vals[q]) > zt; };
+  while (jj < jstop && kk < kstop) {
+    gp_offset jl = jj, kl = kk;
+    while (jl < jstop && !live(jl)) ++jl;
+    while (kl < kstop && !live(kl)) ++kl;
+    if (jl >= jstop || kl >= kstop) break;
+    const int32_t rj = AT.col_idx[jl], rk = AT.col_idx[kl];
+    if (rj == rk) {
+      jj = jl + 1;
+      kk = kl + 1;
+    } else if (rk < rj) {
+      kk = kl + 1;
+      ++fill;
+    } else {
+      jj = jl + 1;
+    }
+  }
+  while (kk < kstop) {
+    gp_offset kl = kk;
+    while (kl < kstop && !live(kl)) ++kl;
+    if (kl >= kstop) break;
+    ++fill;
+    kk = kl + 1;
+  }
+  return fill;
+}
+
+struct DbCand {
+  int32_t* elim;
+  int32_t* keep;
+  double* elim_val;
+  double* keep_val;
+};
+
+__global__ void k_db_candidates(gp_csr A, gp_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                                const double* AL, const double* AU, const int32_t* col_nnz, double zt, double bt,
+                                double ratio_tol, int32_t maxfill, DbCand cand, uint8_t* active, int32_t* flags) {
+  const int64_t i = gtid();
+  if (i >= A.rows) return;
+  active[i] = 0;
+  if (!keep_row[i]) return;
+  const double lhs = AL[i], rhs = AU[i];
+  if (!isfinite(lhs) || !isfinite(rhs) || fabs(lhs - rhs) > bt) return;
+  int32_t live = 0, c1 = -1, c2 = -1;
+  double v1 = 0.0, v2 = 0.0;
+  for (gp_offset q = A.row_ptr[i]; q < A.row_ptr[i + 1]; ++q) {
+    const int32_t col = A.col_idx[q];
+    const double v = A.vals[q];
+    if (keep_col[col] && fabs(v) > zt) {
+      ++live;
+      if (live == 1) {
+        c1 = col;
+        v1 = v;
+      } else if (live == 2) {
+        c2 = col;
+        v2 = v;
+      } else {
+        break;
+      }
+    }
+  }
+  if (live != 2) return;
+  const bool i12 = db_integral(v1, v2, ratio_tol), i21 = db_integral(v2, v1, ratio_tol);
+  int32_t ec, kc;
+  double ev, kv;
+  if (col_nnz[c1] == 1 && col_nnz[c2] != 1) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else if (col_nnz[c1] != 1 && col_nnz[c2] == 1) {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  } else if (i12 && !i21) {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  } else if (i21 && !i12) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else if (col_nnz[c1] < col_nnz[c2]) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  }
+  if (fabs(ev) <= 0.0) return;
+  const double pr = fabs(kv / ev);
+  if (!(pr <= kDblMaxRatioPivot && pr >= 1.0 / kDblMaxRatioPivot)) return;
+  cand.elim[i] = ec;
+  cand.keep[i] = kc;
+  cand.elim_val[i] = ev;
+  cand.keep_val[i] = kv;
+  if (col_nnz[ec] > col_nnz[kc] + maxfill + 1) return;
+  if (db_fill_in(AT, keep_row, zt, kc, ec) <= maxfill) {
+    active[i] = 1;
+    atomicAdd(&flags[0], 1);
+  }
+}
+
+__global__ void k_db_claim(int32_t m, const uint8_t* active, const uint8_t* blocked, DbCand cand, int32_t* owner) {
+  const int64_t i = gtid();
+  if (i >= m || !active[i]) return;
+  const int32_t kc = cand.keep[i], ec = cand.elim[i];
+  if (!blocked[kc] && !blocked[ec]) {
+    atomicMin(&owner[kc], static_cast<int32_t>(i));
+    atomicMin(&owner[ec], static_cast<int32_t>(i));
+  }
+}
+
+// A row wins when it owns both columns; winners own disjoint columns, so the
+// blocked marks written here never change another row's outcome.
+__global__ void k_db_select(int32_t m, uint8_t* active, uint8_t* blocked, uint8_t* selected, DbCand cand,
+                            const int32_t* owner) {
+  const int64_t i = gtid();
+  if (i >= m || !active[i]) return;
+  const int32_t kc = cand.keep[i], ec = cand.elim[i];
+  if (owner[kc] == i && owner[ec] == i && !blocked[kc] && !blocked[ec]) {
+    selected[i] = 1;
+    active[i] = 0;
+    blocked[kc] = 1;
+    blocked[ec] = 1;
+  }
+}
+
+__global__ void k_db_rows(int32_t m, const uint8_t* selected, const int32_t* o2n, int32_t* rows) {
+  const int64_t i = gtid();
+  if (i < m && selected[i]) rows[o2n[i]] = static_cast<int32_t>(i);
+}
+
+struct DbPairs {
+  int32_t k;
+  const int32_t* rows;
+  double* rec;  // 6 per pair: AU(row), elim l, u, keep l, u, elim c
+  uint8_t* subst;
+  uint8_t* fixedp;
+  double* obj_s;
+  double* obj_f;
+};
+
+__global__ void k_db_apply(DbPairs P, DbCand cand, double* l, double* u, double* c, const double* AU,
+                           uint8_t* keep_row, uint8_t* keep_col, double bt, int32_t* flags) {
+  const int64_t t = gtid();
+  if (t >= P.k) return;
+  const int32_t row = P.rows[t], kc = cand.keep[row], ec = cand.elim[row];
+  double* rc = P.rec + 6 * t;
+  rc[0] = AU[row];
+  rc[1] = l[ec];
+  rc[2] = u[ec];
+  rc[3] = l[kc];
+  rc[4] = u[kc];
+  rc[5] = c[ec];
+  const double kv = cand.keep_val[row], ev = cand.elim_val[row];
+  const double alpha = -kv / ev, beta = AU[row] / ev;
+  double ml = -INFINITY, mu = INFINITY;
+  if (isfinite(l[ec])) {
+    const double b = (l[ec] - beta) / alpha;
+    if (alpha > 0.0) ml = b;
+    else mu = b;
+  }
+  if (isfinite(u[ec])) {
+    const double b = (u[ec] - beta) / alpha;
+    if (alpha > 0.0) mu = b;
+    else ml = b;
+  }
+  if (ml > mu) {
+    const double tmp = ml;
+    ml = mu;
+    mu = tmp;
+  }
+  const double kl = isfinite(ml) ? cmax(l[kc], ml) : l[kc];
+  const double ku = isfinite(mu) ? cmin(u[kc], mu) : u[kc];
+  P.subst[t] = 0;
+  P.fixedp[t] = 0;
+  P.obj_s[t] = 0.0;
+  P.obj_f[t] = 0.0;
+  if (kl > ku + bt) {
+    atomicMax(&flags[1], 1);
+    return;
+  }
+  if (isfinite(kl) && isfinite(ku) && ku <= kl + bt) {
+    const double at = 0.5 * (kl + ku);
+    l[kc] = at;
+    u[kc] = at;
+    keep_col[kc] = 0;
+    P.fixedp[t] = 1;
+    P.obj_f[t] = c[kc] * at;
+  } else {
+    l[kc] = kl;
+    u[kc] = ku;
+    c[kc] += c[ec] * alpha;
+    keep_row[row] = 0;
+    keep_col[ec] = 0;
+    P.subst[t] = 1;
+    P.obj_s[t] = c[ec] * beta;
+  }
+  atomicMax(&flags[2], 1);
+}
+
+__global__ void k_db_tape_counts(int32_t k, const uint8_t* subst, int64_t* r, int64_t* ni, int64_t* nv) {
+  const int64_t t = gtid();
+  if (t >= k) return;
+  const int64_t s = subst[t] ? 1 : 0;
+  r[t] = s;
+  ni[t] = 4 * s;
+  nv[t] = 12 * s;
+}
+
+// DoubletonEquation: idx {elim, keep, row, 0}, vals {elim a, keep a, AU(row),
+// elim l, u, keep l, u, new keep l, u, elim c, 0, 0}.
+__global__ void k_db_tape_pack(TapeView out, DbPairs P, DbCand cand, const double* l, const double* u,
+                               const int64_t* r) {
+  const int64_t t = gtid();
+  if (t >= P.k || !P.subst[t]) return;
+  const int32_t row = P.rows[t], kc = cand.keep[row], ec = cand.elim[row];
+  const double* rc = P.rec + 6 * t;
+  const int64_t s = r[t], rec = out.rbase + s;
+  out.types[rec] = GP_RED_DOUBLETON_EQUATION;
+  out.modes[rec] = GP_DUAL_MINIMAL;
+  int32_t* ix = out.idx + out.ibase + 4 * s;
+  ix[0] = ec;
+  ix[1] = kc;
+  ix[2] = row;
+  ix[3] = 0;
+  double* vx = out.vals + out.vbase + 12 * s;
+  vx[0] = cand.elim_val[row];
+  vx[1] = cand.keep_val[row];
+  vx[2] = rc[0];
+  vx[3] = rc[1];
+  vx[4] = rc[2];
+  vx[5] = rc[3];
+  vx[6] = rc[4];
+  vx[7] = l[kc];
+  vx[8] = u[kc];
+  vx[9] = rc[5];
+  vx[10] = 0.0;
+  vx[11] = 0.0;
+  out.istarts[rec + 1] = static_cast<gp_offset>(out.ibase + 4 * (s + 1));
+  out.vstarts[rec + 1] = static_cast<gp_offset>(out.vbase + 12 * (s + 1));
+}
+
+__global__ void k_db_pair_of(DbPairs P, DbCand cand, int32_t* pair_of) {
+  const int64_t t = gtid();
+  if (t >= P.k) return;
+  const int32_t row = P.rows[t];
+  if (P.subst[t]) pair_of[cand.elim[row]] = static_cast<int32_t>(t);
+  else if (P.fixedp[t]) pair_of[cand.keep[row]] = static_cast<int32_t>(t);
+}
+
+// Row shift of kept row i: a * beta for eliminated columns (other rows of the
+// pair) and a * fixed value for fixed keep columns, live entries, CSR order.
+__global__ void k_db_shift(gp_csr A, const uint8_t* keep_row, const int32_t* pair_of, DbPairs P, DbCand cand,
+                           const double* AU, const double* l, double zt, double* shift) {
+  const int64_t i = gtid();
+  if (i >= A.rows) return;
+  double s = 0.0;
+  if (keep_row[i]) {
+    RowAcc acc;
+    const gp_offset p0 = A.row_ptr[i];
+    for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+      acc.pos(q - p0);
+      const int32_t tt = pair_of[A.col_idx[q]];
+      const double a = A.vals[q];
+      if (tt < 0 || !(fabs(a) > zt)) continue;
+      const int32_t prow = P.rows[tt];
+      if (P.subst[tt]) {
+        if (prow == i) continue;
+        acc.add(a * (AU[prow] / cand.elim_val[prow]));
+      } else {
+        acc.add(a * l[cand.keep[prow]]);
+      }
+    }
+    s = acc.value();
+  }
+  shift[i] = s;
+}
+
+__global__ void k_db_entry_counts(DbPairs P, DbCand cand, gp_csr AT, const uint8_t* keep_row, double zt,
+                                  int64_t* cnt) {
+  const int64_t t = gtid();
+  if (t >= P.k) return;
+  int64_t c = 0;
+  if (P.subst[t]) {
+    const int32_t row = P.rows[t], ec = cand.elim[row];
+    for (gp_offset q = AT.row_ptr[ec]; q < AT.row_ptr[ec + 1]; ++q) {
+      const int32_t sr = AT.col_idx[q];
+      c += (sr != row && keep_row[sr] && fabs(AT.vals[q]) > zt) ? 1 : 0;
+    }
+  }
+  cnt[t] = c;
+}
+
+__global__ void k_db_delta(DbPairs P, DbCand cand, gp_csr AT, const uint8_t* keep_row, double zt, int32_t n,
+                           const int64_t* start, uint64_t* keys, double* vals) {
+  const int64_t t = gtid();
+  if (t >= P.k || !P.subst[t]) return;
+  const int32_t row = P.rows[t], ec = cand.elim[row], kc = cand.keep[row];
+  const double alpha = -cand.keep_val[row] / cand.elim_val[row];
+  int64_t w = start[t];
+  for (gp_offset q = AT.row_ptr[ec]; q < AT.row_ptr[ec + 1]; ++q) {
+    const int32_t sr = AT.col_idx[q];
+    const double a = AT.vals[q];
+    if (sr != row && keep_row[sr] && fabs(a) > zt) {
+      keys[w] = static_cast<uint64_t>(static_cast<int64_t>(sr) * n + kc);
+      vals[w] = alpha * a;
+      ++w;
+    }
+  }
+}
+
+__global__ void k_db_delta_rows(int64_t total, const uint64_t* keys, int32_t n, int32_t* counts) {
+  const int64_t e = gtid();
+  if (e < total) atomicAdd(&counts[static_cast<int64_t>(keys[e] / static_cast<uint64_t>(n))], 1);
+}
+
+// Merge of a source row with its sorted deltas (rule_doubleton_equations.cu:1242-1335):
+// equal columns are summed (source entries first), |sum| <= zero_tol dropped.
+template <bool kWrite>
+__global__ void k_db_merge(gp_csr A, const int32_t* drp, const uint64_t* keys, const double* dvals, int32_t n,
+                           double zt, int32_t* counts, gp_offset* out_ptr, int32_t* out_col, double* out_val) {
+  const int64_t i = gtid();
+  if (i >= A.rows) return;
+  gp_offset pp = A.row_ptr[i];
+  const gp_offset pe = A.row_ptr[i + 1];
+  int64_t qq = drp[i];
+  const int64_t qe = drp[i + 1];
+  int64_t w = kWrite ? out_ptr[i] : 0;
+  int32_t c = 0;
+  while (pp < pe || qq < qe) {
+    const int32_t sc = pp < pe ? A.col_idx[pp] : INT_MAX;
+    const int32_t dc = qq < qe ? static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) : INT_MAX;
+    const int32_t next = sc < dc ? sc : dc;
+    double merged = 0.0;
+    while (pp < pe && A.col_idx[pp] == next) merged += A.vals[pp++];
+    while (qq < qe && static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) == next) merged += dvals[qq++];
+    if (fabs(merged) > zt) {
+      if (kWrite) {
+        out_col[w] = next;
+        out_val[w] = merged;
+        ++w;
+      } else {
+        ++c;
+      }
+    }
+  }
+  if (!kWrite) counts[i] = c;
+}
+
+__global__ void k_i32_to_offset(const int32_t* src, int64_t n, gp_offset* dst) {
+  const int64_t i = gtid();
+  if (i < n) dst[i] = static_cast<gp_offset>(src[i]);
+}
+
+__global__ void k_db_shift_apply(int32_t m, const double* shift, double* AL, double* AU) {
+  const int64_t i = gtid();
+  if (i < m) {
+    AL[i] -= shift[i];
+    AU[i] -= shift[i];
+  }
+}
+}  // namespace
+
+// ---------------------------------------------------------------- singleton columns
+void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  if (plan.has_infeasible || plan.has_unbounded || !p.enable_singleton_columns) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  Flags flags;
+  DVec<uint8_t> needed(m);
+  fill_u8(needed.data(), 0, m);
+  hipLaunchKernelGGL(k_sc_count, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, plan.keep_col_mask, plan.keep_row_mask,
+                     s.singleton_column_mask, s.singleton_column_row, needed.data(), flags.d.data());
+  GP_LAUNCH_CHECK("singleton columns count");
+  if (flags.read()[0] == 0) return;
+  DVec<double> fmin(m), fmax(m), AL_new(m), AU_new(m), side(n), contrib(n), c_new(n);
+  DVec<int32_t> cnt(4 * static_cast<size_t>(m)), owner(m);
+  DVec<uint8_t> invalid(m), action(n), row_delete(m), lo(m), up(m), col_delete(n);
+  fill_u8(row_delete.data(), 0, m);
+  fill_u8(lo.data(), 0, m);
+  fill_u8(up.data(), 0, m);
+  fill_u8(col_delete.data(), 0, n);
+  GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
+  GP_LAUNCH_CHECK("singleton columns owner");
+  ScAct act{fmin.data(), fmax.data(), cnt.data(), invalid.data()};
+  hipLaunchKernelGGL(k_sc_activity, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, needed.data(), plan.keep_row_mask,
+                     plan.keep_col_mask, plan.new_l, plan.new_u, act);
+  GP_LAUNCH_CHECK("singleton columns activity");
+  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, lp.A, plan.keep_row_mask,
+                     plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val,
+                     plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, act, p.zero_tol, p.bound_tol,
+                     action.data(), side.data(), owner.data(), flags.d.data());
+  GP_LAUNCH_CHECK("singleton columns propose");
+  if (flags.read()[1]) {
+    plan.has_unbounded = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_sc_stage, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, n, owner.data(), action.data(), side.data(),
+                     s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.bound_tol,
+                     row_delete.data(), lo.data(), up.data(), col_delete.data(), AL_new.data(), AU_new.data(),
+                     contrib.data(), flags.d.data());
+  GP_LAUNCH_CHECK("singleton columns stage");
+  const int32_t* f = flags.read();
+  const bool status1 = f[2] != 0, status2 = f[3] != 0;
+  if (!status1 && !status2) return;
+  if (status1) {
+    hipLaunchKernelGGL(k_sc_cost, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
+                       col_delete.data(), owner.data(), s.singleton_column_val, plan.new_c, c_new.data());
+    GP_LAUNCH_CHECK("singleton columns cost");
+  }
+  if (status1 && tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    AppendCounts ac(static_cast<int64_t>(m) + 1);
+    hipLaunchKernelGGL(k_sc_sub_counts, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, n, owner.data(),
+                       col_delete.data(), plan.keep_col_mask, p.zero_tol, ac.r.data(), ac.i.data(), ac.v.data());
+    GP_LAUNCH_CHECK("singleton substitution counts");
+    GP_CHECK(hipMemsetAsync(ac.r.data() + m, 0, sizeof(int64_t), 0));
+    GP_CHECK(hipMemsetAsync(ac.i.data() + m, 0, sizeof(int64_t), 0));
+    GP_CHECK(hipMemsetAsync(ac.v.data() + m, 0, sizeof(int64_t), 0));
+    ac.scan(static_cast<int64_t>(m) + 1);
+    if (ac.R > 0) {
+      tape->reserve_more(ac.R, ac.I, ac.V);
+      hipLaunchKernelGGL(k_sc_sub_pack, dim3(grid_for(m)), dim3(kBlock), 0, 0, tape_view(*tape), lp.A, ac.r.data(),
+                         ac.i.data(), ac.v.data(), owner.data(), plan.keep_col_mask, row_delete.data(), side.data(),
+                         plan.new_l, plan.new_u, plan.new_c);
+      GP_LAUNCH_CHECK("singleton substitution pack");
+      tape->commit(ac.R, ac.I, ac.V);
+    }
+  }
+  if (status2 && tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    AppendCounts ac(m);
+    hipLaunchKernelGGL(k_rds_counts, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, lo.data(), up.data(), ac.r.data(),
+                       ac.i.data(), ac.v.data());
+    GP_LAUNCH_CHECK("row dual shift counts");
+    ac.scan(m);
+    if (ac.R > 0) {
+      tape->reserve_more(ac.R, ac.I, ac.V);
+      hipLaunchKernelGGL(k_rds_pack, dim3(grid_for(m)), dim3(kBlock), 0, 0, tape_view(*tape), m, lo.data(), up.data(),
+                         ac.r.data());
+      GP_LAUNCH_CHECK("row dual shift pack");
+      tape->commit(ac.R, ac.I, ac.V);
+    }
+  }
+  const int32_t mn = m > n ? m : n;
+  hipLaunchKernelGGL(k_sc_commit, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, row_delete.data(), owner.data(),
+                     AL_new.data(), AU_new.data(), col_delete.data(), c_new.data(), status1 ? 1 : 0,
+                     plan.keep_row_mask, plan.keep_col_mask, plan.new_AL, plan.new_AU, plan.new_c);
+  GP_LAUNCH_CHECK("singleton columns commit");
+  plan.obj_constant_delta += det_sum(contrib.data(), n);
+  plan.has_row_action = 1;
+  plan.has_col_action = plan.has_col_action || status1;
+  plan.has_change = 1;
+}
+
+// ---------------------------------------------------------------- dual fix
+void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols, m = lp.A.rows;
+  if (n == 0) return;
+  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  Flags flags;
+  DVec<uint8_t> fixed(n), inf(n), ffix(n);
+  DVec<double> val(n), contrib(n);
+  hipLaunchKernelGGL(k_df_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
+                     plan.keep_row_mask, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
+                     p.bound_tol, p.dual_fix_max_nonzero_cost_col_degree, s.live_masks ? s.col_nnz : nullptr,
+                     fixed.data(), inf.data(), ffix.data(), val.data(), contrib.data(), flags.d.data());
+  GP_LAUNCH_CHECK("dual fix candidates");
+  const int32_t* f = flags.read();
+  if (f[0] == 1) {
+    plan.has_infeasible = 1;
+    return;
+  }
+  if (f[0] == 2) {
+    plan.has_unbounded = 1;
+    return;
+  }
+  if (!f[1]) return;
+  const bool any_inf = f[2] != 0;
+  DVec<double> shift(m ? m : 1);
+  DVec<uint8_t> row_remove(m ? m : 1), keep_after(m ? m : 1);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_df_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, ffix.data(),
+                       inf.data(), val.data(), shift.data(), row_remove.data(), keep_after.data(), flags.d.data());
+    GP_LAUNCH_CHECK("dual fix rows");
+  }
+  const bool row_changed = flags.read()[3] != 0;
+  if (tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    if (any_inf) {
+      AppendCounts ac(n);
+      hipLaunchKernelGGL(k_df_inf_counts, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.A, lp.AT, fixed.data(), inf.data(),
+                         plan.keep_row_mask, plan.rule_entry_keep_col, ac.r.data(), ac.i.data(), ac.v.data());
+      GP_LAUNCH_CHECK("infinite dual fix counts");
+      ac.scan(n);
+      if (ac.R > 0) {
+        tape->reserve_more(ac.R, ac.I, ac.V);
+        hipLaunchKernelGGL(k_df_inf_pack, dim3(grid_for(n)), dim3(kBlock), 0, 0, tape_view(*tape), lp.A, lp.AT,
+                           fixed.data(), inf.data(), plan.keep_row_mask, plan.rule_entry_keep_col, val.data(),
+                           plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, ac.r.data(), ac.i.data(), ac.v.data());
+        GP_LAUNCH_CHECK("infinite dual fix pack");
+        tape->commit(ac.R, ac.I, ac.V);
+      }
+    }
+    if (p.record_postsolve_tape_cpu)
+      fixed_variable_records(tape, ffix.data(), val.data(), keep_after.data(), plan.new_c, lp.AT, n, true);
+  }
+  const int32_t mn = m > n ? m : n;
+  hipLaunchKernelGGL(k_df_apply, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, fixed.data(), val.data(), plan.new_l,
+                     plan.new_u, plan.keep_col_mask, shift.data(), row_remove.data(), plan.new_AL, plan.new_AU,
+                     plan.keep_row_mask);
+  GP_LAUNCH_CHECK("dual fix apply");
+  plan.obj_constant_delta += det_sum(contrib.data(), n);
+  plan.has_row_action = plan.has_row_action || row_changed;
+  plan.has_col_action = 1;
+  plan.has_change = 1;
+}
+
+// ---------------------------------------------------------------- parallel rows
+void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows;
+  if (m <= 1) return;
+  const uint8_t* keep_col = s.live_masks ? plan.keep_col_mask : nullptr;
+  const int use_xor = static_cast<int64_t>(lp.A.nnz) > 32 * static_cast<int64_t>(m) ? 1 : 0;
+  DVec<uint64_t> hash(m);
+  DVec<int32_t> ids(m);
+  {
+    TimedScope ts(GP_KT_HASH_SORT, 0.0);
+    hipLaunchKernelGGL(k_pr_hash, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, keep_col,
+                       p.zero_tol, use_xor, hash.data(), ids.data());
+    GP_LAUNCH_CHECK("parallel rows hash");
+    sort_pairs_u64_i32(hash.data(), ids.data(), m);
+  }
+  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  Flags flags;
+  DVec<uint8_t> row_delete(m);
+  fill_u8(row_delete.data(), 0, m);
+  DVec<int32_t> step(m), kept(m);
+  DVec<double> rvals(5 * static_cast<size_t>(m));
+  PrRecs rec{step.data(), kept.data(), rvals.data()};
+  hipLaunchKernelGGL(k_pr_groups, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, hash.data(), ids.data(), lp.A,
+                     plan.keep_row_mask, keep_col, plan.new_AL, plan.new_AU, p.zero_tol, p.feasibility_tol,
+                     row_delete.data(), rec, flags.d.data());
+  GP_LAUNCH_CHECK("parallel rows groups");
+  if (flags.read()[0]) {
+    plan.has_infeasible = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_pr_apply, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, row_delete.data(), plan.keep_row_mask,
+                     flags.d.data());
+  GP_LAUNCH_CHECK("parallel rows apply");
+  const int32_t changed = flags.read()[1];
+  if (!changed) return;
+  if (tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    DVec<int32_t> o2n(m), rows(changed);
+    DVec<uint64_t> keys(changed);
+    build_old_to_new(row_delete.data(), m, o2n.data());
+    hipLaunchKernelGGL(k_pr_keys, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, row_delete.data(), o2n.data(), step.data(),
+                       keys.data(), rows.data());
+    GP_LAUNCH_CHECK("parallel rows keys");
+    sort_pairs_u64_i32(keys.data(), rows.data(), changed);
+    tape->reserve_more(changed, 2 * static_cast<int64_t>(changed), 5 * static_cast<int64_t>(changed));
+    hipLaunchKernelGGL(k_pr_pack, dim3(grid_for(changed)), dim3(kBlock), 0, 0, tape_view(*tape), changed, rows.data(),
+                       rec);
+    GP_LAUNCH_CHECK("parallel rows pack");
+    tape->commit(changed, 2 * static_cast<int64_t>(changed), 5 * static_cast<int64_t>(changed));
+  }
+  plan.has_row_action = 1;
+  plan.has_change = 1;
+}
+
+// ---------------------------------------------------------------- parallel columns
+void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  (void)s;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (n <= 1) return;
+  const double zt = p.zero_tol;
+  const double ctol = fmax(p.zero_tol, p.bound_tol), otol = ctol;
+  DVec<uint64_t> hash(n);
+  DVec<int32_t> ids(n);
+  {
+    TimedScope ts(GP_KT_HASH_SORT, 0.0);
+    hipLaunchKernelGGL(k_pc_hash, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask, plan.keep_row_mask,
+                       zt, hash.data(), ids.data());
+    GP_LAUNCH_CHECK("parallel columns hash");
+    sort_pairs_u64_i32(hash.data(), ids.data(), n);
+  }
+  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  Flags flags;
+  double active_max = p.parallel_columns_max_abs_ratio;
+  if (std::isfinite(p.parallel_columns_max_abs_ratio) && p.parallel_columns_scale_guard_min_groups > 1) {
+    hipLaunchKernelGGL(k_pc_guard, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, hash.data(), ids.data(), lp.AT,
+                       plan.keep_col_mask, plan.keep_row_mask, plan.new_c, zt, ctol, otol,
+                       p.parallel_columns_max_abs_ratio, p.parallel_columns_scale_guard_max_group_size,
+                       flags.d.data());
+    GP_LAUNCH_CHECK("parallel columns scale guard");
+    if (flags.read()[4] < p.parallel_columns_scale_guard_min_groups) active_max = INFINITY;
+  }
+  DVec<uint8_t> col_delete(n), fixed(n);
+  fill_u8(col_delete.data(), 0, n);
+  fill_u8(fixed.data(), 0, n);
+  DVec<double> fval(n), mrec(5 * static_cast<size_t>(n));
+  DVec<int32_t> merge_to(n);
+  PcOut o{col_delete.data(), fixed.data(), fval.data(), merge_to.data(), mrec.data()};
+  hipLaunchKernelGGL(k_pc_groups, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, hash.data(), ids.data(), lp.AT,
+                     plan.keep_col_mask, plan.keep_row_mask, plan.new_c, plan.new_l, plan.new_u, zt, ctol, otol,
+                     active_max, o, flags.d.data());
+  GP_LAUNCH_CHECK("parallel columns groups");
+  const int32_t* f = flags.read();
+  if (f[0]) {
+    plan.has_unbounded = 1;
+    return;
+  }
+  const bool status1 = f[1] != 0, status2 = f[2] != 0;
+  if (!status1 && !status2) return;
+  if (status1 && tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    AppendCounts ac(n);
+    hipLaunchKernelGGL(k_pc_counts, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, col_delete.data(), ac.r.data(),
+                       ac.i.data(), ac.v.data());
+    GP_LAUNCH_CHECK("parallel columns counts");
+    ac.scan(n);
+    if (ac.R > 0) {
+      tape->reserve_more(ac.R, ac.I, ac.V);
+      hipLaunchKernelGGL(k_pc_pack, dim3(grid_for(n)), dim3(kBlock), 0, 0, tape_view(*tape), n, col_delete.data(),
+                         merge_to.data(), mrec.data(), ac.r.data());
+      GP_LAUNCH_CHECK("parallel columns pack");
+      tape->commit(ac.R, ac.I, ac.V);
+    }
+  }
+  double obj = 0.0;
+  if (status2) {
+    obj = fixed_variable_records(tape_on(plan, p) ? plan_tape(plan) : nullptr, fixed.data(), fval.data(),
+                                 plan.keep_row_mask, plan.new_c, lp.AT, n, true);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_shift_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, fixed.data(),
+                         fval.data(), plan.new_AL, plan.new_AU);
+      GP_LAUNCH_CHECK("parallel columns row shift");
+    }
+    plan.has_row_action = 1;
+  }
+  hipLaunchKernelGGL(k_pc_apply, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, col_delete.data(), fixed.data(),
+                     fval.data(), plan.keep_col_mask, plan.new_l, plan.new_u);
+  GP_LAUNCH_CHECK("parallel columns apply");
+  plan.obj_constant_delta += obj;
+  plan.has_col_action = 1;
+  plan.has_change = 1;
+}
+
+// ---------------------------------------------------------------- doubleton equations
+void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  (void)s;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  const gp_csr A = plan.has_new_A ? plan.new_A : lp.A;
+  const gp_csr& AT = lp.AT;
+  const double zt = p.zero_tol, bt = p.bound_tol;
+  const double ratio_tol = fmax(1.0e-9, bt);
+  Flags flags;
+  DVec<int32_t> col_nnz(n), owner(n), ce(m), ck(m);
+  DVec<double> cev(m), ckv(m);
+  DVec<uint8_t> active(m), selected(m), blocked(n);
+  fill_u8(selected.data(), 0, m);
+  fill_u8(blocked.data(), 0, n);
+  DbCand cand{ce.data(), ck.data(), cev.data(), ckv.data()};
+  hipLaunchKernelGGL(k_db_colnnz, dim3(grid_for(n)), dim3(kBlock), 0, 0, AT, plan.keep_row_mask, plan.keep_col_mask,
+                     zt, col_nnz.data());
+  GP_LAUNCH_CHECK("doubleton column counts");
+  hipLaunchKernelGGL(k_db_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, AT, plan.keep_row_mask,
+                     plan.keep_col_mask, plan.new_AL, plan.new_AU, col_nnz.data(), zt, bt, ratio_tol,
+                     p.doubleton_equations_max_fill_in_proxy, cand, active.data(), flags.d.data());
+  GP_LAUNCH_CHECK("doubleton candidates");
+  const int32_t acceptable = flags.read()[0];
+  if (acceptable == 0) return;
+  for (int round = 0; round < kDblMatchingRounds; ++round) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, n);
+    hipLaunchKernelGGL(k_db_claim, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, active.data(), blocked.data(), cand,
+                       owner.data());
+    hipLaunchKernelGGL(k_db_select, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, active.data(), blocked.data(),
+                       selected.data(), cand, owner.data());
+    GP_LAUNCH_CHECK("doubleton matching");
+  }
+  DVec<int32_t> o2n(m);
+  const int32_t k = build_old_to_new(selected.data(), m, o2n.data());
+  const double ratio = static_cast<double>(k) / static_cast<double>(acceptable);
+  if (k == 0 || (p.doubleton_equations_scan && (k < p.doubleton_equations_min_selected_per_batch ||
+                                                ratio < p.doubleton_equations_min_selected_ratio)))
+    return;
+  DVec<int32_t> rows(k);
+  DVec<double> rec(6 * static_cast<size_t>(k)), obj_s(k), obj_f(k);
+  DVec<uint8_t> subst(k), fixedp(k);
+  hipLaunchKernelGGL(k_db_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, selected.data(), o2n.data(), rows.data());
+  GP_LAUNCH_CHECK("doubleton rows");
+  DbPairs P{k, rows.data(), rec.data(), subst.data(), fixedp.data(), obj_s.data(), obj_f.data()};
+  hipLaunchKernelGGL(k_db_apply, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, plan.new_l, plan.new_u, plan.new_c,
+                     plan.new_AU, plan.keep_row_mask, plan.keep_col_mask, bt, flags.d.data());
+  GP_LAUNCH_CHECK("doubleton apply");
+  const int32_t* f = flags.read();
+  if (f[1]) {
+    plan.has_infeasible = 1;
+    return;
+  }
+  if (!f[2]) return;
+  const int32_t subst_count = count_u8(subst.data(), k);
+  if (tape_on(plan, p) && subst_count > 0) {
+    DeviceTape* tape = plan_tape(plan);
+    AppendCounts ac(k);
+    hipLaunchKernelGGL(k_db_tape_counts, dim3(grid_for(k)), dim3(kBlock), 0, 0, k, subst.data(), ac.r.data(),
+                       ac.i.data(), ac.v.data());
+    GP_LAUNCH_CHECK("doubleton tape counts");
+    ac.scan(k);
+    tape->reserve_more(ac.R, ac.I, ac.V);
+    hipLaunchKernelGGL(k_db_tape_pack, dim3(grid_for(k)), dim3(kBlock), 0, 0, tape_view(*tape), P, cand, plan.new_l,
+                       plan.new_u, ac.r.data());
+    GP_LAUNCH_CHECK("doubleton tape pack");
+    tape->commit(ac.R, ac.I, ac.V);
+  }
+  DVec<int32_t> pair_of(n);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, 0, pair_of.data(), -1, n);
+  hipLaunchKernelGGL(k_db_pair_of, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, pair_of.data());
+  GP_LAUNCH_CHECK("doubleton pair map");
+  DVec<double> shift(m);
+  hipLaunchKernelGGL(k_db_shift, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, plan.keep_row_mask, pair_of.data(), P, cand,
+                     plan.new_AU, plan.new_l, zt, shift.data());
+  GP_LAUNCH_CHECK("doubleton row shift");
+  if (subst_count > 0) {
+    DVec<int64_t> start(static_cast<size_t>(k) + 1);
+    hipLaunchKernelGGL(k_db_entry_counts, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, AT, plan.keep_row_mask, zt,
+                       start.data());
+    GP_LAUNCH_CHECK("doubleton entry counts");
+    int64_t total = 0;
+    exclusive_scan_i64(start.data(), k, &total);
+    if (total > 0) {
+      DVec<uint64_t> keys(static_cast<size_t>(total));
+      DVec<double> dvals(static_cast<size_t>(total));
+      hipLaunchKernelGGL(k_db_delta, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, AT, plan.keep_row_mask, zt, n,
+                         start.data(), keys.data(), dvals.data());
+      GP_LAUNCH_CHECK("doubleton deltas");
+      sort_pairs_u64_f64(keys.data(), dvals.data(), total);
+      DVec<int32_t> drp(static_cast<size_t>(m) + 1), cnt(static_cast<size_t>(m) + 1);
+      GP_CHECK(hipMemsetAsync(drp.data(), 0, sizeof(int32_t) * (static_cast<size_t>(m) + 1), 0));
+      hipLaunchKernelGGL(k_db_delta_rows, dim3(grid_for(total)), dim3(kBlock), 0, 0, total, keys.data(), n, drp.data());
+      GP_LAUNCH_CHECK("doubleton delta rows");
+      exclusive_scan_i32_total(drp.data(), static_cast<int64_t>(m) + 1);
+      hipLaunchKernelGGL(k_db_merge<false>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, drp.data(), keys.data(),
+                         dvals.data(), n, zt, cnt.data(), nullptr, nullptr, nullptr);
+      GP_LAUNCH_CHECK("doubleton merge counts");
+      GP_CHECK(hipMemsetAsync(cnt.data() + m, 0, sizeof(int32_t), 0));
+      const int64_t nnz = exclusive_scan_i32_total(cnt.data(), static_cast<int64_t>(m) + 1);
+      if (nnz > INT32_MAX) throw std::length_error("doubleton substitution exceeds int32 offsets");
+      gp_csr out;
+      out.rows = m;
+      out.cols = n;
+      out.nnz = static_cast<gp_offset>(nnz);
+      out.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(m) + 1);
+      out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(nnz));
+      out.vals = pool_alloc<double>(static_cast<size_t>(nnz));
+      hipLaunchKernelGGL(k_i32_to_offset, dim3(grid_for(m + 1)), dim3(kBlock), 0, 0, cnt.data(),
+                         static_cast<int64_t>(m) + 1, out.row_ptr);
+      hipLaunchKernelGGL(k_db_merge<true>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, drp.data(), keys.data(),
+                         dvals.data(), n, zt, nullptr, out.row_ptr, out.col_idx, out.vals);
+      GP_LAUNCH_CHECK("doubleton merge rows");
+      if (plan.has_new_A) {
+        GP_CHECK(hipStreamSynchronize(0));
+        csr_pool_free(plan.new_A);
+      }
+      plan.new_A = out;
+      plan.has_new_A = 1;
+    }
+  }
+  hipLaunchKernelGGL(k_db_shift_apply, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, shift.data(), plan.new_AL,
+                     plan.new_AU);
+  GP_LAUNCH_CHECK("doubleton shift apply");
+  plan.has_row_action = subst_count > 0;
+  plan.has_col_action = 1;
+  plan.has_change = 1;
+  plan.obj_constant_delta += det_sum(obj_s.data(), k);
+  plan.obj_constant_delta += det_sum(obj_f.data(), k);
+}
 
 }  // namespace gp

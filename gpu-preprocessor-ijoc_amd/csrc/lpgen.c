@@ -419,9 +419,11 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
     for (int32_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p) aty[col_idx[p]] += vals[p] * y0[i];
   for (int32_t j = 0; j < n; ++j)
     if (is_free[j]) c[j] = aty[j];
-  /* duplicate columns keep the exact cost of their source */
+  /* bounded duplicate columns keep the exact cost of their source; a free
+   * duplicate keeps c = (A'y0)_j, since planted rows may have made it differ
+   * from its source and any other cost would leave the LP dual infeasible */
   for (int32_t j = 0; j < n; ++j)
-    if (ctype[j] == COL_DUPLICATE) c[j] = c[dup_src[j]];
+    if (ctype[j] == COL_DUPLICATE && !is_free[j]) c[j] = c[dup_src[j]];
   free(aty);
 
   out->m = m;

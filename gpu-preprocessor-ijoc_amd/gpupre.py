@@ -134,7 +134,7 @@ class GpStats(C.Structure):
 
 
 PLAN_FLAGS = ["has_change", "has_row_action", "has_col_action", "has_redundant_row_deletion",
-              "has_row_deletion", "has_infeasible", "has_unbounded"]
+              "has_row_deletion", "has_infeasible", "has_unbounded", "has_new_A"]
 
 
 # ---------------------------------------------------------------- host LP container

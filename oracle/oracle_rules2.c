@@ -194,7 +194,7 @@ int or_rule_singleton_columns(gp_plan* plan, const gp_device_lp* lp, const gp_st
   for (int32_t j = 0; j < n; ++j) {
     if (!keep_col[j] || !s->singleton_column_mask[j]) continue;
     const int32_t r = s->singleton_column_row[j];
-    if (r >= 0 && keep_row[r]) ++count;
+    if (r >= 0 && r < m && keep_row[r]) ++count;
   }
   if (count == 0) return 0;
 

@@ -90,6 +90,15 @@ class OracleBackend:
     def set_live(self, v):
         self.p.stats.live_masks = v
 
+    def new_A(self):
+        """Replacement matrix of the plan (row_ptr, col_idx, vals) or None."""
+        A = self.p.plan.new_A
+        if not self.p.plan.has_new_A:
+            return None
+        from oracle import _arr
+        return (_arr(A.row_ptr, A.rows + 1, G.NP_OFFSET), _arr(A.col_idx, A.nnz, np.int32),
+                _arr(A.vals, A.nnz, np.float64))
+
 
 class GpuBackend:
     name = "gpu"
@@ -142,6 +151,13 @@ class GpuBackend:
 
     def set_live(self, v):
         self.stats.live_masks = v
+
+    def new_A(self):
+        A = self.plan.new_A
+        if not self.plan.has_new_A:
+            return None
+        return (G.to_host(A.row_ptr, A.rows + 1, G.NP_OFFSET), G.to_host(A.col_idx, A.nnz, np.int32),
+                G.to_host(A.vals, A.nnz, np.float64))
 
     def tape(self):
         if not self.plan.tape:

@@ -15,3 +15,13 @@ def test_gpu_reference_kat(case):
         check_kat(case, b, states)
     finally:
         b.free()
+
+
+from kat_programs import PROGRAMS  # noqa: E402
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,program", PROGRAMS, ids=[p[0] for p in PROGRAMS])
+def test_gpu_reference_program(name, program):
+    print(f"[kat program] {name}", flush=True)
+    program("gpu")

@@ -15,7 +15,8 @@ from backends import GpuBackend, OracleBackend
 pytestmark = pytest.mark.gpu
 
 RULES_READY = ["fixed_variables", "empty_columns", "empty_rows", "singleton_rows", "redundant_rows",
-               "bound_strengthening"]
+               "bound_strengthening", "singleton_columns", "dual_fix", "doubleton_equations", "parallel_rows",
+               "parallel_columns"]
 
 
 def bits(a):
@@ -125,7 +126,9 @@ def test_rule_parity_random_state(rule, seed):
     print(f"[rule parity] {rule} seed={seed}", flush=True)
     lp = small_lp(seed)
     kr, kc = random_state(lp, 100 + seed)
-    params = G.parity_params(record_postsolve_tape_cpu=1)
+    # batch-size gate of the doubleton rule off, so small LPs exercise it
+    params = G.parity_params(record_postsolve_tape_cpu=1, doubleton_equations_min_selected_per_batch=1,
+                             doubleton_equations_min_selected_ratio=0.0)
     ob, gb = OracleBackend(lp), GpuBackend(lp)
     try:
         for b in (ob, gb):
@@ -143,6 +146,11 @@ def test_rule_parity_random_state(rule, seed):
         tg, to = gb.tape(), ob.tape()
         for k in to:
             assert_same(tg[k], to[k], f"{rule}: tape {k}")
+        ag, ao = gb.new_A(), ob.new_A()
+        assert (ag is None) == (ao is None)
+        if ao is not None:
+            for x, y, k in zip(ag, ao, ("row_ptr", "col_idx", "vals")):
+                assert_same(x, y, f"{rule}: new_A {k}")
     finally:
         gb.free()
 
