@@ -67,7 +67,8 @@ def main():
     for k in range(args.steps):
         s = G.preprocess(dlp, params)
         core.append(s.elapsed_seconds)
-        reduced = (s.reduced_rows, s.reduced_cols, s.reduced_nnz, s.iterations)
+        reduced = (s.reduced_rows, s.reduced_cols, s.reduced_nnz, s.iterations,
+                   "infeasible" if s.has_infeasible else "unbounded" if s.has_unbounded else "ok")
         s.free()
     G.check(G.load().gp_synchronize())
     wall = time.perf_counter() - t0
@@ -120,7 +121,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload} synthetic LP presolve", "rows": lp.m, "cols": lp.n,
                    "nnz": lp.nnz, "seed": args.seed, "folding": False, "structure_specific_rules": False,
-                   "reduced": {"rows": reduced[0], "cols": reduced[1], "nnz": reduced[2], "iterations": reduced[3]}},
+                   "reduced": {"rows": reduced[0], "cols": reduced[1], "nnz": reduced[2], "iterations": reduced[3],
+                               "status": reduced[4]}},
         "presolve_core_ms_p50": round(float(np.median(core)) * 1e3, 3),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu,
     }

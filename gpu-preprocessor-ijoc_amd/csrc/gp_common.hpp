@@ -177,6 +177,13 @@ inline unsigned grid_for(int64_t n, int block = kBlock) {
   if (g < 1) g = 1;
   return static_cast<unsigned>(g);
 }
+// Grid for grid-stride kernels that publish one atomic per block: enough
+// blocks to fill 256 CUs several times over, few enough to keep the
+// same-address atomics cheap.
+inline unsigned grid_capped(int64_t n, int block = kBlock, int64_t cap = 4096) {
+  const int64_t g = grid_for(n, block);
+  return static_cast<unsigned>(g < cap ? g : cap);
+}
 
 // ---- kernel-family timing ---------------------------------------------------------
 // When enabled, every launch family is bracketed by HIP events on the null
@@ -268,6 +275,44 @@ __device__ inline void atomic_total_min(double* slot, double v) {
     if (__atomic_compare_exchange_n(addr, &old, want, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
       break;
   }
+}
+
+// ---- packed keep masks ----------------------------------------------------------------
+// Bit-packed copy of a 0/1 byte mask for the gather-heavy kernels: an 8M-entry
+// mask becomes 1 MB and stays resident in every XCD's L2, so the random
+// keep[col] lookups of a CSR sweep stop costing one HBM line each.  A null
+// view means "everything live".
+struct Bits {
+  const uint32_t* w;
+  __device__ inline bool operator[](int64_t i) const { return !w || ((w[i >> 5] >> (i & 31)) & 1u); }
+  __host__ __device__ explicit operator bool() const { return w != nullptr; }
+};
+
+struct PackedMask {
+  DVec<uint32_t> words;
+  bool null = true;
+  PackedMask() = default;
+  PackedMask(const uint8_t* mask, int64_t n) { pack(mask, n); }
+  void pack(const uint8_t* mask, int64_t n);  // gp_primitives.hip
+  Bits view() const { return Bits{null ? nullptr : words.data()}; }
+};
+
+// ---- wave-aggregated flag / counter updates ------------------------------------------
+// One atomic per wavefront instead of one per lane: the lowest active lane
+// publishes for every lane that reached the call.
+__device__ inline bool wave_leader() {
+  const unsigned long long active = __ballot(1);
+  return __lane_id() == static_cast<unsigned>(__ffsll(static_cast<unsigned long long>(active)) - 1);
+}
+// flag = max(flag, v); every lane reaching one call site passes the same v.
+// A plain read first: once set, later waves skip the atomic entirely.
+__device__ inline void wave_flag(int32_t* flag, int32_t v) {
+  if (wave_leader() && __atomic_load_n(flag, __ATOMIC_RELAXED) < v) atomicMax(flag, v);
+}
+// counter += number of lanes reaching the call.
+__device__ inline void wave_count(int32_t* counter) {
+  const unsigned long long active = __ballot(1);
+  if (__lane_id() == static_cast<unsigned>(__ffsll(active) - 1)) atomicAdd(counter, static_cast<int32_t>(__popcll(active)));
 }
 
 // ---- primitives (gp_primitives.hip) ---------------------------------------------

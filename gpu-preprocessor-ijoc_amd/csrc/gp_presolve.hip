@@ -188,7 +188,13 @@ void apply_plan(Working& cur, gp_plan& plan, Record& r, bool captured) {
   const gp_csr& src = plan.has_new_A ? plan.new_A : cur.lp.A;
   gp_device_lp next{};
   compact_csr(src, plan.keep_row_mask, plan.keep_col_mask, row_o2n.data(), col_o2n.data(), m_new, n_new, next.A);
-  transpose_csr(next.A, next.AT);
+  // Without a rewritten matrix the transpose of the compacted A is the compacted
+  // AT: dropping dead entries keeps each column's increasing row order.
+  if (plan.has_new_A)
+    transpose_csr(next.A, next.AT);
+  else
+    compact_csr(cur.lp.AT, plan.keep_col_mask, plan.keep_row_mask, col_o2n.data(), row_o2n.data(), n_new, m_new,
+                next.AT);
   {
     TimedScope ts(GP_KT_COMPACT, 2.0 * (24.0 * n + 16.0 * m) / 2.0 + 24.0 * n_new + 16.0 * m_new + 4.0 * (m + n));
     next.c = pool_alloc<double>(n_new);

@@ -422,7 +422,7 @@ void csr_pool_free(gp_csr& A) {
 // ---- compaction ------------------------------------------------------------------------------
 namespace {
 __global__ void k_compact_count(const gp_offset* rp, const int32_t* col, int32_t m, const uint8_t* keep_row,
-                                const uint8_t* keep_col, int64_t* cnt) {
+                                Bits keep_col, int64_t* cnt) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   int64_t c = 0;
@@ -463,8 +463,9 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   {
     TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
     if (m > 0) {
+      const PackedMask cols(keep_col, A.cols);
       hipLaunchKernelGGL(k_compact_count, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m,
-                         keep_row, keep_col, cnt.data());
+                         keep_row, cols.view(), cnt.data());
       GP_LAUNCH_CHECK("compact count");
     }
     exclusive_scan_i64(cnt.data(), m, &total);
@@ -488,39 +489,99 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   GP_LAUNCH_CHECK("compact last");
 }
 
-// ---- live statistics (one thread per row; integer results are order free) ----------
+// ---- packed masks ----------------------------------------------------------------------------
 namespace {
-__global__ void k_live_stats(gp_csr M, const uint8_t* keep_outer, const uint8_t* keep_inner, int32_t* degree,
-                             uint8_t* empty, uint8_t* singleton, int32_t* support, double* coefficient,
-                             unsigned long long* total) {
-  __shared__ unsigned long long lds[kBlock];
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int32_t count = 0;
-  if (r < M.rows) {
-    const bool active = !keep_outer || keep_outer[r];
+__global__ void k_pack_bits(const uint8_t* mask, int64_t n, uint32_t* words) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t base = w * 32;
+  if (base >= n) return;
+  uint32_t bits = 0;
+  const int64_t lim = n - base < 32 ? n - base : 32;
+  for (int64_t k = 0; k < lim; ++k) bits |= (mask[base + k] ? 1u : 0u) << k;
+  words[w] = bits;
+}
+}  // namespace
+
+void PackedMask::pack(const uint8_t* mask, int64_t n) {
+  null = mask == nullptr;
+  if (null) return;
+  const int64_t nw = (n + 31) / 32;
+  words.resize(static_cast<size_t>(nw ? nw : 1));
+  if (nw > 0) {
+    hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(nw)), dim3(kBlock), 0, 0, mask, n, words.data());
+    GP_LAUNCH_CHECK("pack bits");
+  }
+}
+
+// ---- live statistics ---------------------------------------------------------------------
+// A group of G lanes per row (G = 4..64 from the mean row length): lanes stride
+// the row so each load instruction reads G consecutive entries of 64/G rows.
+// Counts are integers and the singleton entry is the largest live position,
+// so every result is independent of the lane schedule.
+namespace {
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_live_stats(gp_csr M, const uint8_t* keep_outer,
+                                                       Bits keep_inner, int32_t* degree, uint8_t* empty,
+                                                       uint8_t* singleton, int32_t* support, double* coefficient,
+                                                       unsigned long long* total) {
+  __shared__ unsigned long long lds[kBlock / G];
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  unsigned long long acc = 0;
+  for (int64_t r = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; r < M.rows; r += groups) {
+    int32_t count = 0;
     gp_offset position = -1;
+    const bool active = !keep_outer || keep_outer[r];
     if (active) {
-      for (gp_offset p = M.row_ptr[r]; p < M.row_ptr[r + 1]; ++p) {
-        if (keep_inner && !keep_inner[M.col_idx[p]]) continue;
+      const gp_offset p1 = M.row_ptr[r + 1];
+      for (gp_offset p = M.row_ptr[r] + lane; p < p1; p += G) {
+        if (!keep_inner[M.col_idx[p]]) continue;
         ++count;
         position = p;
       }
     }
-    degree[r] = count;
-    empty[r] = active && count == 0;
-    singleton[r] = active && count == 1;
-    support[r] = count == 1 ? M.col_idx[position] : -1;
-    coefficient[r] = count == 1 ? M.vals[position] : 0.0;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      count += __shfl_xor(count, o, G);
+      const gp_offset other = __shfl_xor(position, o, G);
+      position = other > position ? other : position;
+    }
+    if (lane == 0) {
+      degree[r] = count;
+      empty[r] = active && count == 0;
+      singleton[r] = active && count == 1;
+      support[r] = count == 1 ? M.col_idx[position] : -1;
+      coefficient[r] = count == 1 ? M.vals[position] : 0.0;
+      acc += static_cast<unsigned long long>(count);
+    }
   }
   if (total) {
-    lds[threadIdx.x] = static_cast<unsigned long long>(count);
+    if (lane == 0) lds[threadIdx.x / G] = acc;
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int k = 0; k < kBlock / G; ++k) t += lds[k];
+      if (t) atomicAdd(total, t);
     }
-    if (threadIdx.x == 0 && lds[0]) atomicAdd(total, lds[0]);
   }
+}
+
+template <int G>
+void launch_live_stats(const gp_csr& M, const uint8_t* outer, Bits inner, int32_t* degree, uint8_t* empty,
+                       uint8_t* singleton, int32_t* support, double* coefficient, unsigned long long* total) {
+  const int64_t threads = static_cast<int64_t>(M.rows) * G;
+  hipLaunchKernelGGL(k_live_stats<G>, dim3(grid_capped(threads, kBlock, 8192)), dim3(kBlock), 0, 0, M, outer, inner,
+                     degree, empty, singleton, support, coefficient, total);
+}
+
+void live_stats_side(const gp_csr& M, const uint8_t* outer, Bits inner, int32_t* degree, uint8_t* empty,
+                     uint8_t* singleton, int32_t* support, double* coefficient, unsigned long long* total) {
+  const double mean = M.rows ? static_cast<double>(M.nnz) / M.rows : 0.0;
+  if (mean <= 3.0) launch_live_stats<2>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
+  else if (mean <= 10.0) launch_live_stats<4>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
+  else if (mean <= 24.0) launch_live_stats<8>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
+  else if (mean <= 64.0) launch_live_stats<16>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
+  else launch_live_stats<64>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
 }
 }  // namespace
 
@@ -530,18 +591,15 @@ void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan
   TimedScope ts(GP_KT_LIVE_STATS, bytes);
   unsigned long long* total = reinterpret_cast<unsigned long long*>(device_total);
   if (total) GP_CHECK(hipMemsetAsync(total, 0, sizeof(unsigned long long), 0));
+  const PackedMask rows(plan.keep_row_mask, lp.A.rows), cols(plan.keep_col_mask, lp.A.cols);
   if (lp.A.rows > 0) {
-    hipLaunchKernelGGL(k_live_stats, dim3(grid_for(lp.A.rows)), dim3(kBlock), 0, 0, lp.A,
-                       static_cast<const uint8_t*>(plan.keep_row_mask), static_cast<const uint8_t*>(plan.keep_col_mask),
-                       s.row_nnz, s.empty_row_mask, s.singleton_row_mask, s.singleton_row_col,
-                       s.singleton_row_val, total);
+    live_stats_side(lp.A, plan.keep_row_mask, cols.view(), s.row_nnz, s.empty_row_mask, s.singleton_row_mask,
+                    s.singleton_row_col, s.singleton_row_val, total);
     GP_LAUNCH_CHECK("live stats rows");
   }
   if (lp.AT.rows > 0) {
-    hipLaunchKernelGGL(k_live_stats, dim3(grid_for(lp.AT.rows)), dim3(kBlock), 0, 0, lp.AT,
-                       static_cast<const uint8_t*>(plan.keep_col_mask), static_cast<const uint8_t*>(plan.keep_row_mask),
-                       s.col_nnz, s.empty_col_mask, s.singleton_column_mask, s.singleton_column_row,
-                       s.singleton_column_val, static_cast<unsigned long long*>(nullptr));
+    live_stats_side(lp.AT, plan.keep_col_mask, rows.view(), s.col_nnz, s.empty_col_mask,
+                    s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val, nullptr);
     GP_LAUNCH_CHECK("live stats cols");
   }
   s.live_masks = 1;
@@ -550,7 +608,7 @@ void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan
 // ---- row activity summary ------------------------------------------------------------------
 namespace {
 __global__ void k_activity(gp_csr A, const double* l, const double* u, double zero_tol, const uint8_t* keep_row,
-                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
+                           Bits keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                            int32_t* max_pinf) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= A.rows) return;
@@ -588,7 +646,8 @@ void row_activity_summary(const gp_csr& A, const double* l, const double* u, dou
   if (A.rows <= 0) return;
   const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (double)(A.rows + A.cols) + 24.0 * A.rows;
   TimedScope ts(GP_KT_ACTIVITY, bytes);
-  hipLaunchKernelGGL(k_activity, dim3(grid_for(A.rows)), dim3(kBlock), 0, 0, A, l, u, zero_tol, keep_row, keep_col,
+  const PackedMask cols(keep_col, A.cols);
+  hipLaunchKernelGGL(k_activity, dim3(grid_for(A.rows)), dim3(kBlock), 0, 0, A, l, u, zero_tol, keep_row, cols.view(),
                      min_fin, max_fin, min_ninf, max_pinf);
   GP_LAUNCH_CHECK("activity summary");
 }

@@ -27,7 +27,7 @@ struct Flags {
 // ---------------------------------------------------------------- fixed variables
 __global__ void k_fixed_candidates(int32_t n, const uint8_t* keep_col, const double* l, const double* u,
                                    const double* c, double bound_tol, uint8_t* fixed, double* val,
-                                   double* radius) {
+                                   double* radius, int32_t* flags) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint8_t f = 0;
@@ -43,6 +43,8 @@ __global__ void k_fixed_candidates(int32_t n, const uint8_t* keep_col, const dou
   fixed[j] = f;
   val[j] = v;
   radius[j] = r;
+  if (f) wave_flag(flags + 1, 1);
+  if (r > 0.0) wave_flag(flags + 2, 1);
 }
 
 __global__ void k_row_error(gp_csr A, const uint8_t* keep_row, const uint8_t* fixed, const double* radius,
@@ -74,7 +76,7 @@ __global__ void k_fixed_filter(gp_csr AT, const uint8_t* keep_row, const double*
       }
     }
   }
-  atomicAdd(count, 1);
+  wave_count(count);
 }
 
 // row_shift[i] = sum over entries with fixed[col] of a * val[col] (CSR order)
@@ -111,19 +113,25 @@ void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params
   TimedScope ts(GP_KT_RULES_OTHER, 0.0);
   DVec<uint8_t> fixed(n);
   DVec<double> val(n), radius(n), row_error(m ? m : 1);
-  hipLaunchKernelGGL(k_fixed_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask, plan.new_l,
-                     plan.new_u, plan.new_c, p.bound_tol, fixed.data(), val.data(), radius.data());
-  GP_LAUNCH_CHECK("fixed candidates");
-  if (m > 0) {
-    hipLaunchKernelGGL(k_row_error, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, fixed.data(),
-                       radius.data(), row_error.data());
-    GP_LAUNCH_CHECK("fixed row error");
-  }
   Flags flags;
-  hipLaunchKernelGGL(k_fixed_filter, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_row_mask, plan.new_l,
-                     plan.new_u, row_error.data(), 0.1 * p.feasibility_tol, fixed.data(), flags.d.data());
-  GP_LAUNCH_CHECK("fixed filter");
-  if (flags.read()[0] == 0) return;
+  hipLaunchKernelGGL(k_fixed_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask, plan.new_l,
+                     plan.new_u, plan.new_c, p.bound_tol, fixed.data(), val.data(), radius.data(), flags.d.data());
+  GP_LAUNCH_CHECK("fixed candidates");
+  const int32_t* cf = flags.read();
+  if (!cf[1]) return;  // no candidate
+  // With every radius zero the error budget of every row is zero and the
+  // filter keeps all candidates; the budget pass is skipped.
+  if (cf[2]) {
+    if (m > 0) {
+      hipLaunchKernelGGL(k_row_error, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, fixed.data(),
+                         radius.data(), row_error.data());
+      GP_LAUNCH_CHECK("fixed row error");
+    }
+    hipLaunchKernelGGL(k_fixed_filter, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_row_mask, plan.new_l,
+                       plan.new_u, row_error.data(), 0.1 * p.feasibility_tol, fixed.data(), flags.d.data());
+    GP_LAUNCH_CHECK("fixed filter");
+    if (flags.read()[0] == 0) return;
+  }
   const double obj = fixed_variable_records(tape_on(plan, p) && p.record_postsolve_tape_cpu ? plan_tape(plan) : nullptr,
                                             fixed.data(), val.data(), plan.keep_row_mask, plan.new_c, lp.AT, n, true);
   if (m > 0) {
@@ -151,11 +159,11 @@ __global__ void k_empty_cols(int32_t n, const uint8_t* keep_col, const uint8_t* 
   if (!(keep_col[j] && empty[j])) return;
   const double cj = c[j], lj = l[j], uj = u[j];
   if (lj > uj + bound_tol) {
-    atomicMax(flags + 0, 1);
+    wave_flag(flags + 0, 1);
     return;
   }
   if ((cj > zero_tol && !isfinite(lj)) || (cj < -zero_tol && !isfinite(uj))) {
-    atomicMax(flags + 1, 1);
+    wave_flag(flags + 1, 1);
     return;
   }
   double v = 0.0;
@@ -165,7 +173,7 @@ __global__ void k_empty_cols(int32_t n, const uint8_t* keep_col, const uint8_t* 
   else if (isfinite(uj)) v = uj;
   fixed[j] = 1;
   val[j] = v;
-  atomicMax(flags + 2, 1);
+  wave_flag(flags + 2, 1);
 }
 }  // namespace
 
@@ -205,8 +213,8 @@ __global__ void k_empty_rows(int32_t m, const uint8_t* keep_row, const double* A
                              const int32_t* row_nnz, double tol, int32_t* flags) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m || !(keep_row[i] && row_nnz[i] == 0)) return;
-  if (AL[i] <= tol && AU[i] >= -tol) atomicMax(flags + 1, 1);
-  else atomicMax(flags + 0, 1);
+  if (AL[i] <= tol && AU[i] >= -tol) wave_flag(flags + 1, 1);
+  else wave_flag(flags + 0, 1);
 }
 __global__ void k_empty_rows_apply(int32_t m, uint8_t* keep_row, const double* AL, const double* AU,
                                    const int32_t* row_nnz, double tol) {
@@ -263,10 +271,10 @@ __global__ void k_singleton_check(int32_t m, int32_t n, const double* cl, const 
                                   const double* u, const uint8_t* remove, double tol, int32_t* flags) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < n) {
-    if (cl[j] > cu[j] + tol) atomicMax(flags + 0, 1);
-    if (cl[j] != l[j] || cu[j] != u[j]) atomicMax(flags + 2, 1);
+    if (cl[j] > cu[j] + tol) wave_flag(flags + 0, 1);
+    if (cl[j] != l[j] || cu[j] != u[j]) wave_flag(flags + 2, 1);
   }
-  if (j < m && remove[j]) atomicMax(flags + 1, 1);
+  if (j < m && remove[j]) wave_flag(flags + 1, 1);
 }
 __global__ void k_singleton_commit(int32_t m, int32_t n, const double* cl, const double* cu, const uint8_t* remove,
                                    uint8_t* keep_row, double* l, double* u) {
@@ -373,7 +381,7 @@ struct Act {
   }
 };
 
-__global__ void k_redundant_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col, const double* AL,
+__global__ void k_redundant_rows(gp_csr A, const uint8_t* keep_row, Bits keep_col, const double* AL,
                                  const double* AU, const double* l, const double* u, const int32_t* row_nnz,
                                  double tol, uint8_t* full, uint8_t* dl, uint8_t* du, int32_t* flags) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -397,20 +405,20 @@ __global__ void k_redundant_rows(gp_csr A, const uint8_t* keep_row, const uint8_
   }
   const double rmin = lo.value(), rmax = hi.value();
   if ((lf && rmax < al - tol) || (uf && rmin > au + tol)) {
-    atomicMax(flags + 0, 1);
+    wave_flag(flags + 0, 1);
     return;
   }
   const bool li = !lf || rmin >= al - tol;
   const bool ui = !uf || rmax <= au + tol;
   if (li && ui) {
     full[i] = 1;
-    atomicMax(flags + 1, 1);
+    wave_flag(flags + 1, 1);
   } else if (lf && li) {
     dl[i] = 1;
-    atomicMax(flags + 2, 1);
+    wave_flag(flags + 2, 1);
   } else if (uf && ui) {
     du[i] = 1;
-    atomicMax(flags + 3, 1);
+    wave_flag(flags + 3, 1);
   }
 }
 __global__ void k_redundant_apply(int32_t m, const uint8_t* full, const uint8_t* dl, const uint8_t* du,
@@ -469,8 +477,9 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   TimedScope ts(GP_KT_RULES_OTHER, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + 16.0 * m);
   DVec<uint8_t> full(m), dl(m), du(m);
   Flags flags;
+  const PackedMask live_cols(s.live_masks ? static_cast<const uint8_t*>(plan.keep_col_mask) : nullptr, lp.A.cols);
   hipLaunchKernelGGL(k_redundant_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask,
-                     s.live_masks ? static_cast<const uint8_t*>(plan.keep_col_mask) : nullptr, plan.new_AL,
+                     live_cols.view(), plan.new_AL,
                      plan.new_AU, plan.new_l, plan.new_u, s.row_nnz, p.bound_tol, full.data(), dl.data(), du.data(),
                      flags.d.data());
   GP_LAUNCH_CHECK("redundant rows");
@@ -535,7 +544,7 @@ __device__ inline void implied_bounds(double a, double lj, double uj, double al,
   }
 }
 
-__global__ void k_bs_candidates(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_nnz,
+__global__ void k_bs_candidates(gp_csr A, const uint8_t* keep_row, Bits keep_col, const int32_t* row_nnz,
                                 const double* AL, const double* AU, const double* l, const double* u,
                                 const double* min_fin, const double* max_fin, const int32_t* min_ninf,
                                 const int32_t* max_pinf, double zero_tol, double* cl, double* cu) {
@@ -559,7 +568,7 @@ __global__ void k_bs_candidates(gp_csr A, const uint8_t* keep_row, const uint8_t
   }
 }
 
-__global__ void k_col_max_abs(gp_csr AT, const uint8_t* keep_row, double* vmax) {
+__global__ void k_col_max_abs(gp_csr AT, Bits keep_row, double* vmax) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= AT.rows) return;
   double v = 0.0;
@@ -593,7 +602,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   if (isfinite(cand_l)) el = cmax(el, cand_l);
   if (isfinite(cand_u)) eu = cmin(eu, cand_u);
   if (el > eu + ft) {
-    atomicMax(flags + 0, 1);
+    wave_flag(flags + 0, 1);
     return;
   }
   const bool was_free = !isfinite(ref_l[j]) && !isfinite(ref_u[j]);
@@ -606,7 +615,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   }
   if (isfinite(cand_l) && cand_l > lj && isfinite(uj)) {
     if (cand_l >= uj + ft) {
-      atomicMax(flags + 0, 1);
+      wave_flag(flags + 0, 1);
       return;
     }
     if (cand_l >= uj || (uj - cand_l) * vmax[j] <= ft) {
@@ -626,7 +635,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   }
   if (!fx && isfinite(cand_u) && cand_u < uj && isfinite(new_l)) {
     if (cand_u <= new_l - ft) {
-      atomicMax(flags + 0, 1);
+      wave_flag(flags + 0, 1);
       return;
     }
     if (cand_u <= new_l || (cand_u - new_l) * vmax[j] <= ft) {
@@ -645,9 +654,9 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   }
   if (fx) {
     lc = uc = 0;
-    atomicMax(flags + 2, 1);
+    wave_flag(flags + 2, 1);
   }
-  if (lc || uc) atomicMax(flags + 1, 1);
+  if (lc || uc) wave_flag(flags + 1, 1);
   fl[j] = new_l;
   fu[j] = new_u;
   lch[j] = lc;
@@ -656,7 +665,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   fval[j] = fat;
 }
 
-__global__ void k_bs_support(gp_csr AT, const uint8_t* keep_row, const int32_t* row_nnz, const double* AL,
+__global__ void k_bs_support(gp_csr AT, Bits keep_row, const int32_t* row_nnz, const double* AL,
                              const double* AU, const double* l, const double* u, const double* min_fin,
                              const double* max_fin, const int32_t* min_ninf, const int32_t* max_pinf,
                              const uint8_t* lch, const uint8_t* uch, const double* fl, const double* fu,
@@ -751,7 +760,7 @@ __global__ void k_bs_fixed_rows(gp_csr A, const uint8_t* fixed, const double* fv
   AU[i] = au;
   if (live == 0) {
     if (al <= ft && au >= -ft) keep_row[i] = 0;
-    else atomicMax(flags + 3, 1);
+    else wave_flag(flags + 3, 1);
   }
 }
 }  // namespace
@@ -772,16 +781,19 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   {
     TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 40.0 * m + (double)(m + n) + 4.0 * m +
                                               16.0 * n + 16.0 * n);
-    hipLaunchKernelGGL(k_bs_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, keep_col,
+    const PackedMask live_cols(keep_col, n);
+    hipLaunchKernelGGL(k_bs_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask,
+                       live_cols.view(),
                        s.row_nnz, plan.new_AL, plan.new_AU, plan.new_l, plan.new_u, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), p.zero_tol, cl.data(), cu.data());
     GP_LAUNCH_CHECK("bound candidates");
   }
   Flags flags;
+  const PackedMask rows_all(plan.keep_row_mask, m);
   {
     TimedScope ts(GP_KT_RULES_OTHER, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + (double)m + 8.0 * n);
-    hipLaunchKernelGGL(k_col_max_abs, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT,
-                       s.live_masks ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, vmax.data());
+    const PackedMask live_rows(s.live_masks ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
+    hipLaunchKernelGGL(k_col_max_abs, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, live_rows.view(), vmax.data());
     GP_LAUNCH_CHECK("col max abs");
     const double* ref_l = plan.propagation_reference_l ? plan.propagation_reference_l : (lp.l ? lp.l : plan.new_l);
     const double* ref_u = plan.propagation_reference_u ? plan.propagation_reference_u : (lp.u ? lp.u : plan.new_u);
@@ -804,7 +816,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     const double support_tol = cmax(10.0 * p.bound_tol, 1.0e-10);
     DVec<int32_t> sl(n), su(n);
     AppendCounts cnt(n);
-    hipLaunchKernelGGL(k_bs_support, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_row_mask, s.row_nnz,
+    hipLaunchKernelGGL(k_bs_support, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, rows_all.view(), s.row_nnz,
                        plan.new_AL, plan.new_AU, plan.new_l, plan.new_u, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), lch.data(), uch.data(), fl.data(), fu.data(), p.zero_tol,
                        p.bound_tol, support_tol, sl.data(), su.data(), cnt.r.data(), cnt.i.data(), cnt.v.data());

@@ -56,43 +56,36 @@ __global__ void k_shift_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* s
 enum : uint8_t { SC_NONE = 0, SC_SET_ROW_SIDE = 1, SC_TIGHTEN_LHS = 2, SC_TIGHTEN_RHS = 3, SC_ELIM_FREE = 4,
                  SC_ELIM_ABOVE = 5, SC_ELIM_BELOW = 6 };
 
-struct ScAct {
-  double* fmin;
-  double* fmax;
-  int32_t* cnt;  // 4 per row: min_neg, min_pos, max_neg, max_pos
-  uint8_t* invalid;
-};
-
-__global__ void k_sc_count(int32_t n, int32_t m, const uint8_t* keep_col, const uint8_t* keep_row, const uint8_t* smask,
-                           const int32_t* srow, uint8_t* needed, int32_t* flags) {
-  const int64_t j = gtid();
-  if (j >= n || !keep_col[j] || !smask[j]) return;
-  const int32_t r = srow[j];
-  if (r >= 0 && r < m && keep_row[r]) {
-    needed[r] = 1;
-    atomicAdd(&flags[0], 1);
-  }
+__global__ void k_sc_count(int32_t n, const uint8_t* keep_col, const uint8_t* smask, int32_t* flags) {
+  // existence test only: any kept singleton column
+  for (int64_t j = gtid(); j < n; j += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    if (keep_col[j] && smask[j]) {
+      wave_flag(&flags[0], 1);
+      break;
+    }
 }
 
-// _kernel_singleton_row_activity_cache: live entries, no zero filter.
-__global__ void k_sc_activity(gp_csr A, const uint8_t* needed, const uint8_t* keep_row, const uint8_t* keep_col,
-                              const double* l, const double* u, ScAct act) {
-  const int64_t i = gtid();
-  if (i >= A.rows || !needed[i] || !keep_row[i]) return;
+// Activity of the support row over live entries, no zero filter
+// (_kernel_singleton_row_activity_cache), then the bounds of the rest of the
+// row without column j (_singleton_column_activity_bounds_from_cache).  Every
+// singleton column of a row computes the same row activity.
+__device__ bool sc_rest_bounds(const gp_csr& A, int32_t row, Bits keep_col, const double* l, const double* u,
+                               double a, double lj, double uj, int32_t* live_out, double* rmin, double* rmax) {
   RowAcc mn, mx;
-  int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-  uint8_t bad = 0;
-  const gp_offset p0 = A.row_ptr[i];
-  for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+  int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, live = 0;
+  bool bad = false;
+  const gp_offset p0 = A.row_ptr[row];
+  for (gp_offset q = p0; q < A.row_ptr[row + 1]; ++q) {
     mn.pos(q - p0);
     mx.pos(q - p0);
-    const int32_t j = A.col_idx[q];
-    if (!keep_col[j]) continue;
-    const double a = A.vals[q];
-    const double tmin = a >= 0.0 ? a * l[j] : a * u[j];
-    const double tmax = a >= 0.0 ? a * u[j] : a * l[j];
+    const int32_t k = A.col_idx[q];
+    if (!keep_col[k]) continue;
+    ++live;
+    const double ak = A.vals[q];
+    const double tmin = ak >= 0.0 ? ak * l[k] : ak * u[k];
+    const double tmax = ak >= 0.0 ? ak * u[k] : ak * l[k];
     if (isnan(tmin) || isnan(tmax)) {
-      bad = 1;
+      bad = true;
       continue;
     }
     if (isinf(tmin)) {
@@ -108,47 +101,33 @@ __global__ void k_sc_activity(gp_csr A, const uint8_t* needed, const uint8_t* ke
       mx.add(tmax);
     }
   }
-  act.fmin[i] = mn.value();
-  act.fmax[i] = mx.value();
-  act.cnt[4 * i + 0] = c0;
-  act.cnt[4 * i + 1] = c1;
-  act.cnt[4 * i + 2] = c2;
-  act.cnt[4 * i + 3] = c3;
-  act.invalid[i] = bad;
-}
-
-__device__ bool sc_rest_bounds(const ScAct& act, int32_t row, double a, double lj, double uj, double* rmin,
-                               double* rmax) {
-  if (act.invalid[row]) return false;
+  *live_out = live;
+  if (bad) return false;
   const double tmin = a >= 0.0 ? a * lj : a * uj;
   const double tmax = a >= 0.0 ? a * uj : a * lj;
   if (isnan(tmin) || isnan(tmax)) return false;
-  int32_t min_neg = act.cnt[4 * row + 0], min_pos = act.cnt[4 * row + 1];
-  int32_t max_neg = act.cnt[4 * row + 2], max_pos = act.cnt[4 * row + 3];
-  double vmin = act.fmin[row], vmax = act.fmax[row];
+  double vmin = mn.value(), vmax = mx.value();
   if (isinf(tmin)) {
-    if (tmin < 0.0) --min_neg;
-    else --min_pos;
+    if (tmin < 0.0) --c0;
+    else --c1;
   } else {
     vmin -= tmin;
   }
   if (isinf(tmax)) {
-    if (tmax < 0.0) --max_neg;
-    else --max_pos;
+    if (tmax < 0.0) --c2;
+    else --c3;
   } else {
     vmax -= tmax;
   }
-  if (min_neg < 0 || min_pos < 0 || max_neg < 0 || max_pos < 0 || (min_neg > 0 && min_pos > 0) ||
-      (max_neg > 0 && max_pos > 0))
-    return false;
-  *rmin = min_neg > 0 ? -INFINITY : (min_pos > 0 ? INFINITY : vmin);
-  *rmax = max_neg > 0 ? -INFINITY : (max_pos > 0 ? INFINITY : vmax);
+  if (c0 < 0 || c1 < 0 || c2 < 0 || c3 < 0 || (c0 > 0 && c1 > 0) || (c2 > 0 && c3 > 0)) return false;
+  *rmin = c0 > 0 ? -INFINITY : (c1 > 0 ? INFINITY : vmin);
+  *rmax = c2 > 0 ? -INFINITY : (c3 > 0 ? INFINITY : vmax);
   return !isnan(*rmin) && !isnan(*rmax);
 }
 
 __global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col,
-                             const uint8_t* smask, const int32_t* srow, const double* sval, const double* c,
-                             const double* l, const double* u, const double* AL, const double* AU, ScAct act,
+                             Bits live_col, const uint8_t* smask, const int32_t* srow, const double* sval,
+                             const double* c, const double* l, const double* u, const double* AL, const double* AU,
                              double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags) {
   const int64_t j = gtid();
   if (j >= n || !keep_col[j] || !smask[j]) return;
@@ -156,20 +135,19 @@ __global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep
   if (row < 0 || row >= m || !keep_row[row]) return;
   const double a = sval[j];
   if (fabs(a) <= zt) return;
-  int32_t live = 0;
-  for (gp_offset q = A.row_ptr[row]; q < A.row_ptr[row + 1]; ++q)
-    if (keep_col[A.col_idx[q]] && ++live > 1) break;
-  if (live <= 1) return;
   const double lhs = AL[row], rhs = AU[row], cj = c[j], lj = l[j], uj = u[j];
+  int32_t live = 0;
+  double rmin = 0.0, rmax = 0.0;
+  const bool rest_ok = sc_rest_bounds(A, row, live_col, l, u, a, lj, uj, &live, &rmin, &rmax);
+  if (live <= 1) return;
   if ((cj > zt && a > zt && !isfinite(lhs) && !isfinite(lj)) ||
       (cj > zt && a < -zt && !isfinite(rhs) && !isfinite(lj)) ||
       (cj < -zt && a < -zt && !isfinite(lhs) && !isfinite(uj)) ||
       (cj < -zt && a > zt && !isfinite(rhs) && !isfinite(uj))) {
-    atomicMax(&flags[1], 1);
+    wave_flag(&flags[1], 1);
     return;
   }
-  double rmin = 0.0, rmax = 0.0;
-  if (!sc_rest_bounds(act, row, a, lj, uj, &rmin, &rmax)) return;
+  if (!rest_ok) return;
   uint8_t ac = SC_NONE;
   double sd = 0.0;
   if (isfinite(lhs) && isfinite(rhs) && fabs(lhs - rhs) <= tol) {
@@ -211,6 +189,7 @@ __global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep
   action[j] = ac;
   side[j] = sd;
   atomicMin(&owner[row], ac == SC_ELIM_FREE ? static_cast<int32_t>(j - INT_MAX) : static_cast<int32_t>(j));
+  wave_flag(&flags[4], 1);
 }
 
 __global__ void k_sc_stage(int32_t m, int32_t n, int32_t* owner, const uint8_t* action, const double* side,
@@ -248,7 +227,7 @@ __global__ void k_sc_stage(int32_t m, int32_t n, int32_t* owner, const uint8_t* 
     col_delete[j] = 1;
     AL_new[row] = nl;
     AU_new[row] = nu;
-    atomicMax(&flags[2], 1);
+    wave_flag(&flags[2], 1);
     return;
   }
   if (ac == SC_SET_ROW_SIDE) {
@@ -266,15 +245,25 @@ __global__ void k_sc_stage(int32_t m, int32_t n, int32_t* owner, const uint8_t* 
   }
   AL_new[row] = nl;
   AU_new[row] = nu;
-  atomicMax(&flags[3], 1);
+  wave_flag(&flags[3], 1);
+}
+
+// Columns met by an eliminated row (owner column deleted); only their cost
+// changes.
+__global__ void k_sc_touch(gp_csr A, int32_t n, const int32_t* owner, const uint8_t* col_delete, uint8_t* touched) {
+  const int64_t row = gtid();
+  if (row >= A.rows) return;
+  const int32_t o = owner[row];
+  if (o < 0 || o >= n || !col_delete[o]) return;
+  for (gp_offset q = A.row_ptr[row]; q < A.row_ptr[row + 1]; ++q) touched[A.col_idx[q]] = 1;
 }
 
 // _kernel_apply_singleton_objective_updates_deterministic: plain sequential sum
-// in AT order starting from c.
-__global__ void k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col_delete, const int32_t* owner,
-                          const double* sval, const double* c, double* c_new) {
+// in AT order starting from c (untouched columns keep c exactly).
+__global__ void k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col_delete, const uint8_t* touched,
+                          const int32_t* owner, const double* sval, double* c) {
   const int64_t j = gtid();
-  if (j >= AT.rows || !keep_col[j] || col_delete[j]) return;
+  if (j >= AT.rows || !touched[j] || !keep_col[j] || col_delete[j]) return;
   const int32_t n = AT.rows;
   double updated = c[j];
   for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
@@ -282,7 +271,7 @@ __global__ void k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col
     if (o < 0 || o >= n || !col_delete[o]) continue;
     updated += -(c[o] * AT.vals[q] / sval[o]);
   }
-  c_new[j] = updated;
+  c[j] = updated;  // reads only deleted owner columns, which never change here
 }
 
 __global__ void k_sc_sub_counts(gp_csr A, int32_t n, const int32_t* owner, const uint8_t* col_delete,
@@ -365,9 +354,8 @@ __global__ void k_rds_pack(TapeView out, int32_t m, const uint8_t* lo, const uin
 }
 
 __global__ void k_sc_commit(int32_t m, int32_t n, const uint8_t* row_delete, const int32_t* owner,
-                            const double* AL_new, const double* AU_new, const uint8_t* col_delete,
-                            const double* c_new, int update_cost, uint8_t* keep_row, uint8_t* keep_col, double* AL,
-                            double* AU, double* c) {
+                            const double* AL_new, const double* AU_new, const uint8_t* col_delete, uint8_t* keep_row,
+                            uint8_t* keep_col, double* AL, double* AU) {
   const int64_t q = gtid();
   if (q < m) {
     if (row_delete[q]) keep_row[q] = 0;
@@ -376,10 +364,7 @@ __global__ void k_sc_commit(int32_t m, int32_t n, const uint8_t* row_delete, con
       AU[q] = AU_new[q];
     }
   }
-  if (q < n) {
-    if (col_delete[q]) keep_col[q] = 0;
-    if (update_cost && keep_col[q]) c[q] = c_new[q];
-  }
+  if (q < n && col_delete[q]) keep_col[q] = 0;
 }
 
 // ============================================================== dual fix
@@ -397,7 +382,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
   if (!keep_col[j]) return;
   const double lj = l[j], uj = u[j];
   if (lj > uj + bt) {
-    atomicMax(&flags[0], 1);
+    wave_flag(&flags[0], 1);
     return;
   }
   const gp_offset a0 = AT.row_ptr[j], a1 = AT.row_ptr[j + 1];
@@ -423,7 +408,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
   if (cj > zt) {
     if (!down) {
       if (!isfinite(lj)) {
-        atomicMax(&flags[0], 2);
+        wave_flag(&flags[0], 2);
         return;
       }
       v = lj;
@@ -432,7 +417,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
   } else if (cj < -zt) {
     if (!up) {
       if (!isfinite(uj)) {
-        atomicMax(&flags[0], 2);
+        wave_flag(&flags[0], 2);
         return;
       }
       v = uj;
@@ -463,8 +448,8 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
     ffix[j] = infinite ? 0 : 1;
     val[j] = v;
     contrib[j] = infinite ? 0.0 : cj * v;
-    atomicMax(&flags[1], 1);
-    if (infinite) atomicMax(&flags[2], 1);
+    wave_flag(&flags[1], 1);
+    if (infinite) wave_flag(&flags[2], 1);
   }
 }
 
@@ -485,7 +470,7 @@ __global__ void k_df_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* ffix
       if (inf[j]) rem = 1;
     }
     s = acc.value();
-    if (rem) atomicMax(&flags[3], 1);
+    if (rem) wave_flag(&flags[3], 1);
   }
   shift[i] = s;
   row_remove[i] = rem;
@@ -710,7 +695,7 @@ __global__ void k_pr_groups(int32_t m, const uint64_t* hash, const int32_t* ids,
     double sl, su;
     scaled_interval(AL[row], AU[row], r, &sl, &su);
     if (cmax(rl, sl) > cmin(ru, su) + itol) {
-      atomicMax(&flags[0], 1);
+      wave_flag(&flags[0], 1);
       return;
     }
     ++step;
@@ -735,7 +720,7 @@ __global__ void k_pr_apply(int32_t m, const uint8_t* row_delete, uint8_t* keep_r
   const int64_t i = gtid();
   if (i < m && row_delete[i]) {
     keep_row[i] = 0;
-    atomicAdd(&flags[1], 1);
+    wave_count(&flags[1]);
   }
 }
 
@@ -861,7 +846,7 @@ __global__ void k_pc_guard(int32_t n, const uint64_t* hash, const int32_t* ids, 
       if (!pc_ratio(AT, j, k, keep_row, zt, ctol, &r)) continue;
       if (fabs(c[k] - r * c[j]) > otol) continue;
       if (out_of_guard(r, max_abs)) {
-        atomicAdd(&flags[4], 1);
+        wave_count(&flags[4]);
         return;
       }
     }
@@ -931,7 +916,7 @@ __global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids,
         l[j] = merged_lower(tl, sl, su, r);
         u[j] = merged_upper(tu, sl, su, r);
         o.col_delete[k] = 1;
-        atomicMax(&flags[1], 1);
+        wave_flag(&flags[1], 1);
         continue;
       }
       bool xk_lo = false, xk_up = false, xj_lo = false, xj_up = false;
@@ -955,23 +940,23 @@ __global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids,
       if (xk_lo || xk_up) {
         const double v = xk_lo ? sl : su;
         if (!isfinite(v)) {
-          atomicMax(&flags[0], 1);
+          wave_flag(&flags[0], 1);
           return;
         }
         o.fixed[k] = 1;
         o.fval[k] = v;
-        atomicMax(&flags[2], 1);
+        wave_flag(&flags[2], 1);
         continue;
       }
       if (xj_lo || xj_up) {
         const double v = xj_lo ? tl : tu;
         if (!isfinite(v)) {
-          atomicMax(&flags[0], 1);
+          wave_flag(&flags[0], 1);
           return;
         }
         o.fixed[j] = 1;
         o.fval[j] = v;
-        atomicMax(&flags[2], 1);
+        wave_flag(&flags[2], 1);
         break;
       }
     }
@@ -1126,7 +1111,7 @@ __global__ void k_db_candidates(gp_csr A, gp_csr AT, const uint8_t* keep_row, co
   if (col_nnz[ec] > col_nnz[kc] + maxfill + 1) return;
   if (db_fill_in(AT, keep_row, zt, kc, ec) <= maxfill) {
     active[i] = 1;
-    atomicAdd(&flags[0], 1);
+    wave_count(&flags[0]);
   }
 }
 
@@ -1207,7 +1192,7 @@ __global__ void k_db_apply(DbPairs P, DbCand cand, double* l, double* u, double*
   P.obj_s[t] = 0.0;
   P.obj_f[t] = 0.0;
   if (kl > ku + bt) {
-    atomicMax(&flags[1], 1);
+    wave_flag(&flags[1], 1);
     return;
   }
   if (isfinite(kl) && isfinite(ku) && ku <= kl + bt) {
@@ -1226,7 +1211,7 @@ __global__ void k_db_apply(DbPairs P, DbCand cand, double* l, double* u, double*
     P.subst[t] = 1;
     P.obj_s[t] = c[ec] * beta;
   }
-  atomicMax(&flags[2], 1);
+  wave_flag(&flags[2], 1);
 }
 
 __global__ void k_db_tape_counts(int32_t k, const uint8_t* subst, int64_t* r, int64_t* ni, int64_t* nv) {
@@ -1399,15 +1384,13 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   if (m == 0 || n == 0) return;
   TimedScope ts(GP_KT_RULES_OTHER, 0.0);
   Flags flags;
-  DVec<uint8_t> needed(m);
-  fill_u8(needed.data(), 0, m);
-  hipLaunchKernelGGL(k_sc_count, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, plan.keep_col_mask, plan.keep_row_mask,
-                     s.singleton_column_mask, s.singleton_column_row, needed.data(), flags.d.data());
+  hipLaunchKernelGGL(k_sc_count, dim3(grid_capped(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask,
+                     s.singleton_column_mask, flags.d.data());
   GP_LAUNCH_CHECK("singleton columns count");
   if (flags.read()[0] == 0) return;
-  DVec<double> fmin(m), fmax(m), AL_new(m), AU_new(m), side(n), contrib(n), c_new(n);
-  DVec<int32_t> cnt(4 * static_cast<size_t>(m)), owner(m);
-  DVec<uint8_t> invalid(m), action(n), row_delete(m), lo(m), up(m), col_delete(n);
+  DVec<double> AL_new(m), AU_new(m), side(n), contrib(n);
+  DVec<int32_t> owner(m);
+  DVec<uint8_t> action(n), row_delete(m), lo(m), up(m), col_delete(n);
   fill_u8(row_delete.data(), 0, m);
   fill_u8(lo.data(), 0, m);
   fill_u8(up.data(), 0, m);
@@ -1415,19 +1398,18 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
   GP_LAUNCH_CHECK("singleton columns owner");
-  ScAct act{fmin.data(), fmax.data(), cnt.data(), invalid.data()};
-  hipLaunchKernelGGL(k_sc_activity, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, needed.data(), plan.keep_row_mask,
-                     plan.keep_col_mask, plan.new_l, plan.new_u, act);
-  GP_LAUNCH_CHECK("singleton columns activity");
+  const PackedMask live_col(plan.keep_col_mask, n);
   hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, lp.A, plan.keep_row_mask,
-                     plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val,
-                     plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, act, p.zero_tol, p.bound_tol,
-                     action.data(), side.data(), owner.data(), flags.d.data());
+                     plan.keep_col_mask, live_col.view(), s.singleton_column_mask, s.singleton_column_row,
+                     s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
+                     p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data());
   GP_LAUNCH_CHECK("singleton columns propose");
-  if (flags.read()[1]) {
+  const int32_t* pf = flags.read();
+  if (pf[1]) {
     plan.has_unbounded = 1;
     return;
   }
+  if (!pf[4]) return;  // no proposal
   hipLaunchKernelGGL(k_sc_stage, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, n, owner.data(), action.data(), side.data(),
                      s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.bound_tol,
                      row_delete.data(), lo.data(), up.data(), col_delete.data(), AL_new.data(), AU_new.data(),
@@ -1436,11 +1418,6 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   const int32_t* f = flags.read();
   const bool status1 = f[2] != 0, status2 = f[3] != 0;
   if (!status1 && !status2) return;
-  if (status1) {
-    hipLaunchKernelGGL(k_sc_cost, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
-                       col_delete.data(), owner.data(), s.singleton_column_val, plan.new_c, c_new.data());
-    GP_LAUNCH_CHECK("singleton columns cost");
-  }
   if (status1 && tape_on(plan, p)) {
     DeviceTape* tape = plan_tape(plan);
     AppendCounts ac(static_cast<int64_t>(m) + 1);
@@ -1475,10 +1452,21 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
       tape->commit(ac.R, ac.I, ac.V);
     }
   }
+  if (status1) {
+    // cost update after the tape, which records the pre-update costs
+    DVec<uint8_t> touched(n);
+    fill_u8(touched.data(), 0, n);
+    hipLaunchKernelGGL(k_sc_touch, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, n, owner.data(), col_delete.data(),
+                       touched.data());
+    GP_LAUNCH_CHECK("singleton columns touch");
+    hipLaunchKernelGGL(k_sc_cost, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask, col_delete.data(),
+                       touched.data(), owner.data(), s.singleton_column_val, plan.new_c);
+    GP_LAUNCH_CHECK("singleton columns cost");
+  }
   const int32_t mn = m > n ? m : n;
   hipLaunchKernelGGL(k_sc_commit, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, row_delete.data(), owner.data(),
-                     AL_new.data(), AU_new.data(), col_delete.data(), c_new.data(), status1 ? 1 : 0,
-                     plan.keep_row_mask, plan.keep_col_mask, plan.new_AL, plan.new_AU, plan.new_c);
+                     AL_new.data(), AU_new.data(), col_delete.data(), plan.keep_row_mask, plan.keep_col_mask,
+                     plan.new_AL, plan.new_AU);
   GP_LAUNCH_CHECK("singleton columns commit");
   plan.obj_constant_delta += det_sum(contrib.data(), n);
   plan.has_row_action = 1;

@@ -44,12 +44,25 @@ static double normal(rng* r) {
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
-static double coeff(rng* r) {
+static double coeff(rng* r, int integral) {
+  if (integral) {
+    const double mag = (double)(below(r, 4) + 1);
+    return unif(r) < 0.5 ? -mag : mag;
+  }
   double a;
   do {
     a = normal(r);
   } while (fabs(a) < 1e-3);
   return a;
+}
+
+/* uniform on [a, b], or a uniform integer in [ceil(a), floor(b)] */
+static double draw(rng* r, double a, double b, int integral) {
+  if (!integral) return a + (b - a) * unif(r);
+  const double lo = ceil(a), hi = floor(b);
+  if (hi <= lo) return lo;
+  double v = lo + floor((hi - lo + 1.0) * unif(r));
+  return v > hi ? hi : v;
 }
 
 static void* xmalloc(size_t bytes) {
@@ -185,21 +198,21 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
                : t < spec->frac_empty_cols + spec->frac_duplicate_cols ? COL_DUPLICATE
                                                                          : COL_REGULAR;
     const double b = unif(&r);
-    const double ub = unif_ab(&r, 1.0, 100.0);
+    const double ub = draw(&r, 1.0, 100.0, spec->integral);
     c[j] = unif_ab(&r, -1.0, 1.0);
     is_free[j] = 0;
     if (b < spec->frac_fixed_cols) {
-      x0[j] = unif_ab(&r, 0.0, ub);
+      x0[j] = draw(&r, 0.0, ub, spec->integral);
       l[j] = u[j] = x0[j];
     } else if (b < spec->frac_fixed_cols + spec->frac_free) {
       l[j] = -INFINITY;
       u[j] = INFINITY;
-      x0[j] = unif_ab(&r, -10.0, 10.0);
+      x0[j] = draw(&r, -10.0, 10.0, spec->integral);
       is_free[j] = 1;
     } else {
       l[j] = 0.0;
       u[j] = ub;
-      x0[j] = unif_ab(&r, 0.0, ub);
+      x0[j] = draw(&r, 0.0, ub, spec->integral);
     }
   }
   /* a regular source for every duplicate column */
@@ -266,7 +279,7 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
         }
       if (dup) continue;
       picked[got++] = i;
-      coo_push(&e, i, j, coeff(&r));
+      coo_push(&e, i, j, coeff(&r, spec->integral));
     }
     col_count[j] = d;
   }
@@ -280,7 +293,8 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
     l[j] = l[s];
     u[j] = u[s];
     is_free[j] = is_free[s];
-    x0[j] = isfinite(l[j]) ? (l[j] == u[j] ? l[j] : unif_ab(&r, l[j], u[j])) : unif_ab(&r, -10.0, 10.0);
+    x0[j] = isfinite(l[j]) ? (l[j] == u[j] ? l[j] : draw(&r, l[j], u[j], spec->integral))
+                           : draw(&r, -10.0, 10.0, spec->integral);
   }
   /* non-empty columns for planted rows */
   int32_t n_live = 0;
@@ -289,7 +303,7 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
     if (ctype[j] != COL_EMPTY) live_cols[n_live++] = j;
   for (int32_t i = 0; i < m; ++i) {
     if (rtype[i] == ROW_SINGLETON) {
-      coo_push(&e, i, live_cols[below(&r, (uint32_t)n_live)], coeff(&r));
+      coo_push(&e, i, live_cols[below(&r, (uint32_t)n_live)], coeff(&r, spec->integral));
     } else if (rtype[i] == ROW_DOUBLETON) {
       int32_t a = live_cols[below(&r, (uint32_t)n_live)], b;
       do {
@@ -300,8 +314,8 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
         a = b;
         b = t;
       }
-      coo_push(&e, i, a, coeff(&r));
-      if (b != a) coo_push(&e, i, b, coeff(&r));
+      coo_push(&e, i, a, coeff(&r, spec->integral));
+      if (b != a) coo_push(&e, i, b, coeff(&r, spec->integral));
     }
   }
 
@@ -342,7 +356,8 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
       }
     }
     par_src[i] = s;
-    par_f[i] = unif_ab(&r, 0.5, 4.0);
+    par_f[i] = spec->integral ? (double)(below(&r, 3) + 1) * (unif(&r) < 0.25 ? -1.0 : 1.0)
+                              : unif_ab(&r, 0.5, 4.0);
     if (s >= 0) extra += rp[s + 1] - rp[s];
   }
   const int64_t nnz = nnz0 + extra;
@@ -398,7 +413,7 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
     double act = 0.0;
     for (int32_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p) act += vals[p] * x0[col_idx[p]];
     const double t = unif(&r);
-    const double slack = unif_ab(&r, 0.0, 10.0);
+    const double slack = draw(&r, 0.0, 10.0, spec->integral);
     const double yv = unif_ab(&r, 0.0, 1.0);
     if (rtype[i] == ROW_DOUBLETON || t < spec->frac_equality) {
       AL[i] = AU[i] = act;

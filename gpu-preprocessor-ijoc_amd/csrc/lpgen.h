@@ -25,6 +25,9 @@ typedef struct lpgen_spec {
   double frac_duplicate_cols;
   double frac_fixed_cols;
   double frac_empty_cols;
+  /* 1: integer coefficients, planted point, bounds, slacks and parallel-row
+   * factors, so every activity and row shift of the LP is exact in binary64 */
+  int32_t integral;
 } lpgen_spec;
 
 typedef struct lpgen_lp {

@@ -446,7 +446,7 @@ class LpgenSpec(C.Structure):
                 ("frac_equality", C.c_double), ("frac_free", C.c_double),
                 ("frac_singleton_rows", C.c_double), ("frac_doubleton_rows", C.c_double),
                 ("frac_parallel_rows", C.c_double), ("frac_duplicate_cols", C.c_double),
-                ("frac_fixed_cols", C.c_double), ("frac_empty_cols", C.c_double)]
+                ("frac_fixed_cols", C.c_double), ("frac_empty_cols", C.c_double), ("integral", C.c_int32)]
 
 
 class LpgenLp(C.Structure):
@@ -478,9 +478,13 @@ WORKLOADS = {
     "small": dict(m=2000, n=3000, degree_mode=0, mean_degree=6.0),
     # 200,000 x 300,000, ~2M nnz, Zipf(1.5) column degree capped at 500, 1% empty columns
     "medium": dict(m=200_000, n=300_000, degree_mode=1, mean_degree=2_000_000 / 300_000, frac_empty_cols=0.01),
-    # 5M x 8M, ~60M nnz, same recipe
+    # 5M x 8M, ~60M nnz, same recipe with integer data (exact arithmetic: the
+    # presolve reaches its fixpoint instead of stopping on a tolerance artefact)
     "large": dict(m=5_000_000, n=8_000_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
-                  frac_empty_cols=0.01),
+                  frac_empty_cols=0.01, integral=1),
+    # the large recipe at 1/10 scale (parity cases for the bench workload)
+    "large10": dict(m=500_000, n=800_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
+                    frac_empty_cols=0.01, integral=1),
 }
 
 

@@ -12,6 +12,7 @@
 #include "oracle_internal.h"
 
 #include <limits.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -250,7 +251,17 @@ int or_rule_singleton_rows(gp_plan* plan, const gp_device_lp* lp, const gp_stats
   }
   int infeasible = 0, changed_bounds = 0, removed = 0;
   for (int32_t j = 0; j < n; ++j) {
-    if (cl[j] > cu[j] + p->bound_tol) infeasible = 1;
+    if (cl[j] > cu[j] + p->bound_tol) {
+      infeasible = 1;
+      if (getenv("GP_ORACLE_TRACE_RULES")) {
+        fprintf(stderr, "[oracle] singleton rows: col %d implied [%.17g, %.17g] bounds [%.17g, %.17g]\n", j, cl[j],
+                cu[j], plan->new_l[j], plan->new_u[j]);
+        for (int32_t i = 0; i < m; ++i)
+          if (remove[i] && s->singleton_row_col[i] == j)
+            fprintf(stderr, "[oracle]   row %d a %.17g AL %.17g AU %.17g\n", i, s->singleton_row_val[i],
+                    plan->new_AL[i], plan->new_AU[i]);
+      }
+    }
     if (cl[j] != plan->new_l[j] || cu[j] != plan->new_u[j]) changed_bounds = 1;
   }
   for (int32_t i = 0; i < m; ++i) removed |= remove[i];

@@ -5,6 +5,8 @@
  */
 #include "oracle_internal.h"
 
+#include <stdio.h>
+
 #include <limits.h>
 #include <stdlib.h>
 #include <string.h>
@@ -207,6 +209,9 @@ static int deferred_rule(or_sched* s, int enabled, int32_t rule) {
   plan->has_redundant_row_deletion = plan->has_row_deletion = 0;
   plan->obj_constant_delta = 0.0;
   or_rule_apply(rule, plan, &s->cur.lp, &s->d.stats, s->p);
+  if (getenv("GP_ORACLE_TRACE_RULES") && (plan->has_infeasible || plan->has_unbounded))
+    fprintf(stderr, "[oracle] rule %d: infeasible=%d unbounded=%d\n", rule, plan->has_infeasible,
+            plan->has_unbounded);
   s->r->has_infeasible = plan->has_infeasible;
   s->r->has_unbounded = plan->has_unbounded;
   s->terminal = plan->has_infeasible || plan->has_unbounded;

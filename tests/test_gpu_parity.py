@@ -203,6 +203,13 @@ def test_pipeline_parity_medium():
     s.free()
 
 
+def test_pipeline_parity_large10():
+    """The bench recipe (integer data, Zipf columns) at 1/10 scale: 500k x 800k."""
+    lp = G.generate_workload("large10", seed=0)
+    s, ores, rec = compare_pipeline(lp, G.parity_params(), "large10 seed=0")
+    s.free()
+
+
 def test_postsolve_parity_small():
     lp = small_lp(5)
     params = G.parity_params(record_postsolve_tape_cpu=1)
