@@ -482,6 +482,10 @@ WORKLOADS = {
     # presolve reaches its fixpoint instead of stopping on a tolerance artefact)
     "large": dict(m=5_000_000, n=8_000_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
                   frac_empty_cols=0.01, integral=1),
+    # the large recipe with FP64 N(0,1) values as in the config text; the
+    # reference's tolerance policy ends it as infeasible after two rounds
+    "large_fp64": dict(m=5_000_000, n=8_000_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
+                       frac_empty_cols=0.01),
     # the large recipe at 1/10 scale (parity cases for the bench workload)
     "large10": dict(m=500_000, n=800_000, degree_mode=1, mean_degree=60_000_000 / 8_000_000,
                     frac_empty_cols=0.01, integral=1),
