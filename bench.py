@@ -77,6 +77,18 @@ def main():
     ms_per_step = wall / max(1, args.steps) * 1e3
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, core p50 {np.median(core)*1e3:.2f} ms")
 
+    # postsolve of a reduced solution (zeros: the replay cost does not depend on
+    # the values) through the public postprocess, original model given
+    s = G.preprocess(dlp, params)
+    rec_count = s.record().types.size
+    zx, zy = np.zeros(s.reduced_cols), np.zeros(s.reduced_rows)
+    G.check(G.load().gp_synchronize())
+    t = time.perf_counter()
+    G.postprocess(s, zx, zy, zx, dlp)
+    post_ms = (time.perf_counter() - t) * 1e3
+    s.free()
+    log(f"postsolve: {post_ms:.1f} ms for {rec_count} tape records")
+
     roofline = None
     families = {}
     if not args.no_kernel_timing:
@@ -144,6 +156,7 @@ def main():
                    "reduced": {"rows": reduced[0], "cols": reduced[1], "nnz": reduced[2], "iterations": reduced[3],
                                "status": reduced[4]}},
         "presolve_core_ms_p50": round(float(np.median(core)) * 1e3, 3),
+        "postsolve_ms": round(post_ms, 2), "postsolve_records": int(rec_count),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
     }
     print(json.dumps(out), flush=True)
