@@ -327,6 +327,7 @@ double det_sum(const double* x, int64_t n);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n);
 // Stable ascending sort of (key, value) pairs on bits [0, end_bit) of the key,
 // in place (gp_sort.hip).
+void sort_keys_u64(uint64_t* keys, int64_t n, int end_bit = 64);
 void sort_pairs_u64_i32(uint64_t* keys, int32_t* vals, int64_t n, int end_bit = 64);
 void sort_pairs_u64_f64(uint64_t* keys, double* vals, int64_t n, int end_bit = 64);
 void gather_compact(double* dst, const double* src, const int32_t* old_to_new, int32_t n_old);

@@ -399,6 +399,255 @@ __global__ void k_replay_serial(PsState s, PsTape t, int64_t count, int primal, 
   for (int64_t k = count - 1; k >= 0; --k) replay_record(s, t, k, primal != 0, dual != 0);
 }
 
+// ---- component-parallel replay ------------------------------------------------
+// Two records of one pass conflict when one writes a value (x / z / retrieved
+// of a column, y of a row) that the other reads or writes.  Records are joined
+// into components by a lock-free union-find over "accessor of an entity that
+// has a writer"; components touch disjoint state, so replaying each one in
+// reverse tape order on its own lane gives exactly the serial result (the
+// reference's component mode, parallel_replay.cuh, makes the same argument).
+// A RowImpliedBoundChange that writes a whole row of z is pruned before the
+// dual pass when its own test on the final x and its tape values already
+// makes it a no-op; the reference sends every tape holding one of them to the
+// serial replay instead.
+
+constexpr int32_t kNoOwner = INT32_MAX;
+
+// Calls f(entity, write) for every state entry record k touches in this pass;
+// entity j < n0 is column j, entity n0 + i is row i.
+template <class F>
+__device__ void record_accesses(const PsState& s, const PsTape& t, int64_t k, bool primal, int32_t n0, F& f) {
+  const int32_t* ix = t.idx;
+  const int64_t i0 = t.is[k], i1 = t.is[k + 1] - 1;
+  const int64_t v0 = t.vs[k], v1 = t.vs[k + 1] - 1;
+  switch (t.types[k]) {
+    case GP_RED_FIXED_VARIABLE:
+      f(ix[i0], true);
+      if (!primal)
+        for (int64_t p = i0 + 1; p <= i1; ++p) f(n0 + ix[p], false);
+      break;
+    case GP_RED_INFINITE_BOUND_DUAL_FIX: {
+      const int32_t col = ix[i0 + 1];
+      f(col, true);
+      if (primal) {
+        const int32_t rows = ix[i0 + 2];
+        int64_t ip = i0 + 3;
+        for (int32_t r = 0; r < rows; ++r) {
+          const int64_t len = ix[ip];
+          for (int64_t q = 0; q < len; ++q)
+            if (ix[ip + 1 + q] != col) f(ix[ip + 1 + q], false);
+          ip += len + 1;
+        }
+      }
+      break;
+    }
+    case GP_RED_SINGLETON_COLUMN_SUBSTITUTION: {
+      f(ix[i0], true);
+      const int32_t row = ix[i0 + 1], support = ix[i0 + 2];
+      if (primal) {
+        for (int32_t q = 0; q < support; ++q) f(ix[i0 + 3 + q], false);
+      } else if (row >= 0) {
+        f(n0 + row, true);
+      }
+      break;
+    }
+    case GP_RED_PARALLEL_COLUMN:
+      f(ix[i0], true);
+      f(ix[i0 + 1], primal);
+      break;
+    case GP_RED_DOUBLETON_EQUATION: {
+      const int32_t ec = ix[i0], kc = ix[i0 + 1], row = ix[i0 + 2];
+      f(ec, true);
+      if (primal) {
+        f(kc, false);
+        break;
+      }
+      if (row >= 0) f(n0 + row, true);
+      if (!s.has_model) break;
+      f(kc, true);
+      if (i1 >= i0 + 3 && v1 >= v0 + 11) {
+        const int32_t sc = ix[i0 + 3];
+        for (int32_t q = 0; q < sc; ++q) f(n0 + ix[i0 + 4 + q], false);
+      } else {
+        const gp_csr& AT = s.model.AT;
+        for (gp_offset p = AT.row_ptr[ec]; p < AT.row_ptr[ec + 1]; ++p) f(n0 + AT.col_idx[p], false);
+      }
+      break;
+    }
+    case GP_RED_ROW_DUAL_SHIFT:
+      if (!primal) f(n0 + ix[i0], true);
+      break;
+    case GP_RED_PARALLEL_ROW:
+      if (!primal && i1 >= i0 + 1 && v1 >= v0 + 4) {
+        f(n0 + ix[i0], true);
+        f(n0 + ix[i0 + 1], true);
+      }
+      break;
+    case GP_RED_DELETED_ROW:
+      if (!primal && s.has_model && i1 >= i0 + 1 && v1 >= v0 + 2) {
+        const int32_t row = ix[i0], col = ix[i0 + 1];
+        f(col, true);
+        f(n0 + row, true);
+        const gp_csr& AT = s.model.AT;
+        for (gp_offset p = AT.row_ptr[col]; p < AT.row_ptr[col + 1]; ++p) f(n0 + AT.col_idx[p], false);
+      }
+      break;
+    case GP_RED_ROW_IMPLIED_BOUND_CHANGE:
+      if (!primal && s.has_model && i1 >= i0 + 1) {
+        const int32_t col = ix[i0], row = ix[i0 + 1];
+        // the record's early exits that depend on tape values and final x only
+        const double ol = t.vals[v0], ou = t.vals[v0 + 1], nl = t.vals[v0 + 2], nu = t.vals[v0 + 3];
+        const bool lch = nl > ol + s.tol, uch = nu < ou - s.tol;
+        if (lch == uch) break;
+        if (fabs(s.x[col] - (lch ? nl : nu)) > s.tol) break;
+        f(col, true);
+        f(n0 + row, true);
+        const gp_csr& A = s.model.A;
+        for (gp_offset p = A.row_ptr[row]; p < A.row_ptr[row + 1]; ++p) f(A.col_idx[p], true);
+      }
+      break;
+    default: break;
+  }
+}
+
+__device__ inline int32_t uf_load(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of v with path halving.  Links always point to a smaller index, and a
+// halving store only replaces a non-root's parent by one of its ancestors.
+__device__ int32_t uf_find(int32_t* parent, int32_t v) {
+  for (;;) {
+    const int32_t p = uf_load(parent + v);
+    if (p == v) return v;
+    const int32_t g = uf_load(parent + p);
+    if (g == p) return p;
+    __hip_atomic_store(parent + v, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = g;
+  }
+}
+
+__device__ void uf_unite(int32_t* parent, int32_t a, int32_t b) {
+  for (;;) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a < b) {
+      const int32_t t = a;
+      a = b;
+      b = t;
+    }
+    int32_t expected = a;
+    if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+struct OwnerVisit {
+  int32_t* owner;
+  int32_t k;
+  int32_t limit;
+  bool any = false;
+  __device__ void operator()(int32_t e, bool write) {
+    if (e < 0 || e >= limit) return;
+    any = true;
+    if (write) atomicMin(owner + e, k);
+  }
+};
+
+struct UniteVisit {
+  const int32_t* owner;
+  int32_t* parent;
+  int32_t k;
+  int32_t limit;
+  __device__ void operator()(int32_t e, bool) {
+    if (e < 0 || e >= limit) return;
+    const int32_t o = owner[e];
+    if (o != kNoOwner && o != k) uf_unite(parent, k, o);
+  }
+};
+
+__global__ void k_cc_init(int32_t* owner, int32_t entities, int32_t* parent, int32_t count) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < entities) owner[i] = kNoOwner;
+  if (i < count) parent[i] = static_cast<int32_t>(i);
+}
+
+// Every write access claims its entity for the smallest writing record.
+__global__ void k_cc_owner(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
+                           int32_t* owner, uint8_t* active) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  OwnerVisit f{owner, static_cast<int32_t>(k), entities};
+  record_accesses(s, t, k, primal != 0, n0, f);
+  active[k] = f.any;
+}
+
+// Every access of a written entity joins the record to that entity's owner.
+__global__ void k_cc_unite(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
+                           const int32_t* owner, int32_t* parent, const uint8_t* active) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count || !active[k]) return;
+  UniteVisit f{owner, parent, static_cast<int32_t>(k), entities};
+  record_accesses(s, t, k, primal != 0, n0, f);
+}
+
+// (component root << 32 | record); records without accesses sort last.
+__global__ void k_cc_keys(int32_t* parent, const uint8_t* active, int32_t count, uint64_t* keys) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  keys[k] = active[k] ? (static_cast<uint64_t>(uf_find(parent, static_cast<int32_t>(k))) << 32) |
+                            static_cast<uint64_t>(k)
+                      : ~uint64_t{0};
+}
+
+// The lane holding a component's last (largest) record replays the
+// component from there down, i.e. in reverse tape order.
+__global__ void k_replay_components(PsState s, PsTape t, const uint64_t* keys, int32_t count, int primal,
+                                    int dual) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const uint64_t key = keys[p];
+  if (key == ~uint64_t{0}) return;
+  const uint32_t comp = static_cast<uint32_t>(key >> 32);
+  if (p + 1 < count && static_cast<uint32_t>(keys[p + 1] >> 32) == comp) return;
+  for (int64_t q = p; q >= 0 && static_cast<uint32_t>(keys[q] >> 32) == comp; --q)
+    replay_record(s, t, static_cast<int64_t>(static_cast<uint32_t>(keys[q])), primal != 0, dual != 0);
+}
+
+// One pass (primal or dual) of the component-parallel replay; stream-ordered.
+void replay_components(const PsState& s, const PsTape& t, int32_t count, int32_t m0, int32_t n0, bool primal) {
+  const int32_t entities = n0 + m0;
+  DVec<int32_t> owner(static_cast<size_t>(entities > 0 ? entities : 1));
+  DVec<int32_t> parent(static_cast<size_t>(count));
+  DVec<uint8_t> active(static_cast<size_t>(count));
+  DVec<uint64_t> keys(static_cast<size_t>(count));
+  const int64_t most = entities > count ? entities : count;
+  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(most)), dim3(kBlock), 0, 0, owner.data(), entities, parent.data(),
+                     count);
+  hipLaunchKernelGGL(k_cc_owner, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
+                     entities, owner.data(), active.data());
+  hipLaunchKernelGGL(k_cc_unite, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
+                     entities, owner.data(), parent.data(), active.data());
+  hipLaunchKernelGGL(k_cc_keys, dim3(grid_for(count)), dim3(kBlock), 0, 0, parent.data(), active.data(), count,
+                     keys.data());
+  GP_LAUNCH_CHECK("postsolve components");
+  // roots and records are < 2^31; the all-ones key of inactive records needs all 64 bits
+  sort_keys_u64(keys.data(), count, 64);
+  hipLaunchKernelGGL(k_replay_components, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, keys.data(), count,
+                     primal ? 1 : 0, primal ? 0 : 1);
+  GP_LAUNCH_CHECK("postsolve component replay");
+}
+
+// Tapes shorter than this replay serially (the component analysis costs a
+// few launches and a sort); GP_POSTSOLVE_PARALLEL_MIN overrides it.
+int64_t parallel_min_records() {
+  const char* e = getenv("GP_POSTSOLVE_PARALLEL_MIN");
+  if (e && *e) return strtoll(e, nullptr, 10);
+  return 1024;
+}
+
 __global__ void k_scatter(double* dst, const double* src, const int32_t* map, int32_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[map[i]] = src[i];
@@ -512,7 +761,11 @@ extern "C" int gp_postprocess(const double* x, const double* y, const double* z,
     if (original) s.model = *original;
     s.tol = 1e-7;
     PsTape pt{t.types.data(), t.istarts.data(), t.vstarts.data(), t.idx.data(), t.vals.data()};
-    if (t.count > 0) {
+    if (t.count > 0 && t.count >= parallel_min_records() && t.count < INT32_MAX) {
+      // the dual pass is analysed after the primal replay: its pruning reads the final x
+      replay_components(s, pt, static_cast<int32_t>(t.count), m0, n0, true);
+      replay_components(s, pt, static_cast<int32_t>(t.count), m0, n0, false);
+    } else if (t.count > 0) {
       hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 1, 0);
       hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 0, 1);
       GP_LAUNCH_CHECK("postsolve replay");

@@ -28,6 +28,19 @@ RULES = [
     "parallel_rows", "parallel_columns",
 ]
 RULE = {name: i for i, name in enumerate(RULES)}
+# postsolve record types (the GP_RED_* enum of gpupre.h)
+RED_FIXED_VARIABLE = 0
+RED_INFINITE_BOUND_DUAL_FIX = 1
+RED_SINGLETON_COLUMN_SUBSTITUTION = 2
+RED_PARALLEL_COLUMN = 3
+RED_PARALLEL_ROW = 4
+RED_DELETED_ROW = 5
+RED_ROW_LOWER_BOUND_CHANGE = 8
+RED_ROW_UPPER_BOUND_CHANGE = 9
+RED_ROW_DUAL_SHIFT = 10
+RED_VARIABLE_BOUND_CHANGE = 11
+RED_ROW_IMPLIED_BOUND_CHANGE = 12
+RED_DOUBLETON_EQUATION = 13
 PHASES = ["cleanup", "propagation", "aggregation", "parallel", "end"]
 KERNEL_FAMILIES = ["live_stats", "activity", "bound_candidates", "compact", "transpose",
                    "rules_other", "hash_sort"]
