@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the presolve engine has no "
@@ -45,6 +48,18 @@ def main():
 
     import numpy as np
     import gpupre as G
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads n*8 bytes, writes n*8 bytes
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {n * 8} bytes read and written")
+        return
 
     if G.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)

@@ -344,6 +344,13 @@ void csr_pool_free(gp_csr& A);
 // Statistics.
 void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
                         int64_t* device_total);
+// Bring live statistics from the masks (old_rows, old_cols) they were last
+// computed for to the plan's current masks, visiting only deleted lines.
+// `scratch` holds 4 int32.  Returns false (stats then need a full refresh)
+// when a mask went from 0 to 1 or a count went negative.
+bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, const uint8_t* old_rows,
+                       const uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
+                       int32_t* scratch);
 void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
                           const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
                           double* max_fin, int32_t* min_ninf, int32_t* max_pinf);

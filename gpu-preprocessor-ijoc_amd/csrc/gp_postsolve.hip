@@ -399,22 +399,23 @@ __global__ void k_replay_serial(PsState s, PsTape t, int64_t count, int primal, 
   for (int64_t k = count - 1; k >= 0; --k) replay_record(s, t, k, primal != 0, dual != 0);
 }
 
-// ---- component-parallel replay ------------------------------------------------
-// Two records of one pass conflict when one writes a value (x / z / retrieved
-// of a column, y of a row) that the other reads or writes.  Records are joined
-// into components by a lock-free union-find over "accessor of an entity that
-// has a writer"; components touch disjoint state, so replaying each one in
-// reverse tape order on its own lane gives exactly the serial result (the
-// reference's component mode, parallel_replay.cuh, makes the same argument).
+// ---- dependency-ordered parallel replay ----------------------------------------
+// Within one pass (primal or dual) a record reads and writes state entries:
+// x / z / retrieved of a column, y of a row.  Listing, per entry, its
+// accesses in replay order (reverse tape order) gives the only orderings the
+// serial replay's result depends on: a write must follow every earlier access
+// of the entry, a read must follow the entry's previous write.  The replay
+// runs in rounds: a round replays, in parallel, every pending record whose
+// accesses are all unblocked, then counts them as done.  Records replayed in
+// one round touch no common entry that either of them writes, so the result
+// is the serial replay's bit for bit, and the number of rounds is the depth
+// of the dependency graph (a handful on presolve tapes), not the size of its
+// connected pieces.  The reference instead joins conflicting records into
+// components and replays each component serially (parallel_replay.cuh).
 // A RowImpliedBoundChange that writes a whole row of z is pruned before the
 // dual pass when its own test on the final x and its tape values already
-// makes it a no-op; the reference sends every tape holding one of them to the
-// serial replay instead.
+// makes it a no-op.
 
-constexpr int32_t kNoOwner = INT32_MAX;
-
-// Calls f(entity, write) for every state entry record k touches in this pass;
-// entity j < n0 is column j, entity n0 + i is row i.
 template <class F>
 __device__ void record_accesses(const PsState& s, const PsTape& t, int64_t k, bool primal, int32_t n0, F& f) {
   const int32_t* ix = t.idx;
@@ -510,134 +511,226 @@ __device__ void record_accesses(const PsState& s, const PsTape& t, int64_t k, bo
   }
 }
 
-__device__ inline int32_t uf_load(const int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Root of v with path halving.  Links always point to a smaller index, and a
-// halving store only replaces a non-root's parent by one of its ancestors.
-__device__ int32_t uf_find(int32_t* parent, int32_t v) {
-  for (;;) {
-    const int32_t p = uf_load(parent + v);
-    if (p == v) return v;
-    const int32_t g = uf_load(parent + p);
-    if (g == p) return p;
-    __hip_atomic_store(parent + v, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v = g;
-  }
-}
-
-__device__ void uf_unite(int32_t* parent, int32_t a, int32_t b) {
-  for (;;) {
-    a = uf_find(parent, a);
-    b = uf_find(parent, b);
-    if (a == b) return;
-    if (a < b) {
-      const int32_t t = a;
-      a = b;
-      b = t;
-    }
-    int32_t expected = a;
-    if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return;
-  }
-}
-
-struct OwnerVisit {
-  int32_t* owner;
-  int32_t k;
+// Access enumeration into flat arrays.
+struct CountVisit {
   int32_t limit;
-  bool any = false;
+  int32_t n = 0;
+  __device__ void operator()(int32_t e, bool) {
+    if (e >= 0 && e < limit) ++n;
+  }
+};
+
+struct FillVisit {
+  uint64_t* keys;
+  int32_t* ids;
+  uint8_t* writes;
+  int64_t a;
+  uint32_t rank;
+  int32_t limit;
   __device__ void operator()(int32_t e, bool write) {
     if (e < 0 || e >= limit) return;
-    any = true;
-    if (write) atomicMin(owner + e, k);
+    keys[a] = (static_cast<uint64_t>(e) << 32) | rank;
+    ids[a] = static_cast<int32_t>(a);
+    writes[a] = write ? 1 : 0;
+    ++a;
   }
 };
 
-struct UniteVisit {
-  const int32_t* owner;
-  int32_t* parent;
-  int32_t k;
-  int32_t limit;
-  __device__ void operator()(int32_t e, bool) {
-    if (e < 0 || e >= limit) return;
-    const int32_t o = owner[e];
-    if (o != kNoOwner && o != k) uf_unite(parent, k, o);
+__global__ void k_acc_count(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
+                            int64_t* acc_start) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  CountVisit f{entities};
+  record_accesses(s, t, k, primal != 0, n0, f);
+  acc_start[k] = f.n;
+}
+
+// Sort key of an access: (entry << 32 | replay rank), replay rank = count-1-k.
+__global__ void k_acc_fill(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
+                           const int64_t* acc_start, uint64_t* keys, int32_t* ids, uint8_t* writes) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  FillVisit f{keys, ids, writes, acc_start[k], static_cast<uint32_t>(count - 1 - k), entities};
+  record_accesses(s, t, k, primal != 0, n0, f);
+}
+
+// Per sorted access: first of its (entry, record) run?  A record that lists
+// one entry twice counts once, as a write if any of the copies writes.
+__global__ void k_acc_runs(const uint64_t* keys, const int32_t* ids, const uint8_t* writes, int32_t total,
+                           int32_t* first, int32_t* first_write) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const bool head = q == 0 || keys[q - 1] != keys[q];
+  int32_t w = 0;
+  if (head)
+    for (int64_t r = q; r < total && keys[r] == keys[q]; ++r) w |= writes[ids[r]];
+  first[q] = head ? 1 : 0;
+  first_write[q] = head ? w : 0;
+}
+
+// Entry-relative positions: ent_base[e] = scan values at the entry's first access.
+__global__ void k_acc_entry_base(const uint64_t* keys, const int32_t* pos, const int32_t* wpos, int32_t total,
+                                 int32_t* ent_pos, int32_t* ent_wpos) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const uint32_t e = static_cast<uint32_t>(keys[q] >> 32);
+  if (q == 0 || static_cast<uint32_t>(keys[q - 1] >> 32) != e) {
+    ent_pos[e] = pos[q];
+    ent_wpos[e] = wpos[q];
   }
+}
+
+// Per access (in record order): position among the entry's accesses, number
+// of writes before it, write flag; position -1 marks a repeated access.
+struct AccInfo {
+  int32_t entry;
+  int32_t position;
+  int32_t writes_before;
+  int32_t write;
 };
 
-__global__ void k_cc_init(int32_t* owner, int32_t entities, int32_t* parent, int32_t count) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < entities) owner[i] = kNoOwner;
-  if (i < count) parent[i] = static_cast<int32_t>(i);
+__global__ void k_acc_info(const uint64_t* keys, const int32_t* ids, const int32_t* first, const int32_t* first_write,
+                           const int32_t* pos, const int32_t* wpos, const int32_t* ent_pos, const int32_t* ent_wpos,
+                           int32_t total, AccInfo* info) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int32_t e = static_cast<int32_t>(keys[q] >> 32);
+  AccInfo a{e, -1, 0, 0};
+  if (first[q]) {
+    a.position = pos[q] - ent_pos[e];
+    a.writes_before = wpos[q] - ent_wpos[e];
+    a.write = first_write[q];
+  }
+  info[ids[q]] = a;
 }
 
-// Every write access claims its entity for the smallest writing record.
-__global__ void k_cc_owner(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
-                           int32_t* owner, uint8_t* active) {
+// state[k]: 0 pending, -1 no accesses in this pass, r > 0 replayed in round r.
+__global__ void k_acc_state(const int64_t* acc_start, int32_t count, int32_t* state) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  OwnerVisit f{owner, static_cast<int32_t>(k), entities};
-  record_accesses(s, t, k, primal != 0, n0, f);
-  active[k] = f.any;
+  if (k < count) state[k] = acc_start[k + 1] > acc_start[k] ? 0 : -1;
 }
 
-// Every access of a written entity joins the record to that entity's owner.
-__global__ void k_cc_unite(PsState s, PsTape t, int32_t count, int primal, int32_t n0, int32_t entities,
-                           const int32_t* owner, int32_t* parent, const uint8_t* active) {
+__global__ void k_round_replay(PsState s, PsTape t, int32_t count, int primal, int dual, const int64_t* acc_start,
+                               const AccInfo* info, const int32_t* done, const int32_t* writes_done, int32_t* state,
+                               int32_t round) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= count || !active[k]) return;
-  UniteVisit f{owner, parent, static_cast<int32_t>(k), entities};
-  record_accesses(s, t, k, primal != 0, n0, f);
+  if (k >= count || state[k] != 0) return;
+  for (int64_t a = acc_start[k]; a < acc_start[k + 1]; ++a) {
+    const AccInfo f = info[a];
+    if (f.position < 0) continue;
+    if (f.write ? done[f.entry] != f.position : writes_done[f.entry] < f.writes_before) return;
+  }
+  replay_record(s, t, k, primal != 0, dual != 0);
+  state[k] = round;
 }
 
-// (component root << 32 | record); records without accesses sort last.
-__global__ void k_cc_keys(int32_t* parent, const uint8_t* active, int32_t count, uint64_t* keys) {
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  keys[k] = active[k] ? (static_cast<uint64_t>(uf_find(parent, static_cast<int32_t>(k))) << 32) |
-                            static_cast<uint64_t>(k)
-                      : ~uint64_t{0};
+__global__ void __launch_bounds__(kBlock) k_round_commit(int32_t count, const int64_t* acc_start,
+                                                          const AccInfo* info, int32_t* done, int32_t* writes_done,
+                                                          const int32_t* state, int32_t round,
+                                                          unsigned long long* pending) {
+  __shared__ unsigned long long lds[kBlock / 64];
+  unsigned long long left = 0;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t st = state[k];
+    if (st == 0) ++left;
+    if (st != round) continue;
+    for (int64_t a = acc_start[k]; a < acc_start[k + 1]; ++a) {
+      const AccInfo f = info[a];
+      if (f.position < 0) continue;
+      atomicAdd(done + f.entry, 1);
+      if (f.write) atomicAdd(writes_done + f.entry, 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) left += __shfl_xor(left, o, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = left;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tsum = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tsum += lds[w];
+    if (tsum) atomicAdd(pending, tsum);
+  }
 }
 
-// The lane holding a component's last (largest) record replays the
-// component from there down, i.e. in reverse tape order.
-__global__ void k_replay_components(PsState s, PsTape t, const uint64_t* keys, int32_t count, int primal,
-                                    int dual) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  const uint64_t key = keys[p];
-  if (key == ~uint64_t{0}) return;
-  const uint32_t comp = static_cast<uint32_t>(key >> 32);
-  if (p + 1 < count && static_cast<uint32_t>(keys[p + 1] >> 32) == comp) return;
-  for (int64_t q = p; q >= 0 && static_cast<uint32_t>(keys[q] >> 32) == comp; --q)
-    replay_record(s, t, static_cast<int64_t>(static_cast<uint32_t>(keys[q])), primal != 0, dual != 0);
+// Finishes a pass whose dependency chain is long and thin: the pending
+// records, serially in replay order, after every record already replayed.
+__global__ void k_replay_pending(PsState s, PsTape t, int32_t count, int primal, int dual, const int32_t* state) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int64_t k = count - 1; k >= 0; --k)
+    if (state[k] == 0) replay_record(s, t, k, primal != 0, dual != 0);
 }
 
-// One pass (primal or dual) of the component-parallel replay; stream-ordered.
-void replay_components(const PsState& s, const PsTape& t, int32_t count, int32_t m0, int32_t n0, bool primal) {
+// One pass of the dependency-ordered replay; returns false (nothing replayed)
+// when the access lists exceed the sort's int32 item limit.
+bool replay_rounds(const PsState& s, const PsTape& t, int32_t count, int32_t m0, int32_t n0, bool primal) {
   const int32_t entities = n0 + m0;
-  DVec<int32_t> owner(static_cast<size_t>(entities > 0 ? entities : 1));
-  DVec<int32_t> parent(static_cast<size_t>(count));
-  DVec<uint8_t> active(static_cast<size_t>(count));
-  DVec<uint64_t> keys(static_cast<size_t>(count));
-  const int64_t most = entities > count ? entities : count;
-  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(most)), dim3(kBlock), 0, 0, owner.data(), entities, parent.data(),
-                     count);
-  hipLaunchKernelGGL(k_cc_owner, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
-                     entities, owner.data(), active.data());
-  hipLaunchKernelGGL(k_cc_unite, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
-                     entities, owner.data(), parent.data(), active.data());
-  hipLaunchKernelGGL(k_cc_keys, dim3(grid_for(count)), dim3(kBlock), 0, 0, parent.data(), active.data(), count,
-                     keys.data());
-  GP_LAUNCH_CHECK("postsolve components");
-  // roots and records are < 2^31; the all-ones key of inactive records needs all 64 bits
-  sort_keys_u64(keys.data(), count, 64);
-  hipLaunchKernelGGL(k_replay_components, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, keys.data(), count,
-                     primal ? 1 : 0, primal ? 0 : 1);
-  GP_LAUNCH_CHECK("postsolve component replay");
+  DVec<int64_t> acc_start(static_cast<size_t>(count) + 1);
+  hipLaunchKernelGGL(k_acc_count, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
+                     entities, acc_start.data());
+  GP_LAUNCH_CHECK("postsolve access count");
+  int64_t total64 = 0;
+  exclusive_scan_i64(acc_start.data(), count, &total64);
+  h2d(acc_start.data() + count, &total64, 1);
+  if (total64 >= INT32_MAX) return false;
+  const int32_t total = static_cast<int32_t>(total64);
+  DVec<int32_t> state(static_cast<size_t>(count));
+  hipLaunchKernelGGL(k_acc_state, dim3(grid_for(count)), dim3(kBlock), 0, 0, acc_start.data(), count, state.data());
+  GP_LAUNCH_CHECK("postsolve access state");
+  if (total == 0) return true;
+  DVec<AccInfo> info(static_cast<size_t>(total));
+  {
+    DVec<uint64_t> keys(static_cast<size_t>(total));
+    DVec<int32_t> ids(static_cast<size_t>(total));
+    DVec<uint8_t> writes(static_cast<size_t>(total));
+    hipLaunchKernelGGL(k_acc_fill, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0, n0,
+                       entities, acc_start.data(), keys.data(), ids.data(), writes.data());
+    GP_LAUNCH_CHECK("postsolve access fill");
+    sort_pairs_u64_i32(keys.data(), ids.data(), total, 64);
+    DVec<int32_t> pos(static_cast<size_t>(total)), wpos(static_cast<size_t>(total));
+    DVec<int32_t> first_write(static_cast<size_t>(total));
+    hipLaunchKernelGGL(k_acc_runs, dim3(grid_for(total)), dim3(kBlock), 0, 0, keys.data(), ids.data(), writes.data(),
+                       total, pos.data(), first_write.data());
+    GP_LAUNCH_CHECK("postsolve access runs");
+    DVec<int32_t> first(static_cast<size_t>(total));
+    d2d(first.data(), pos.data(), static_cast<size_t>(total));
+    d2d(wpos.data(), first_write.data(), static_cast<size_t>(total));
+    exclusive_scan_i32_total(pos.data(), total);
+    exclusive_scan_i32_total(wpos.data(), total);
+    DVec<int32_t> ent_pos(static_cast<size_t>(entities)), ent_wpos(static_cast<size_t>(entities));
+    hipLaunchKernelGGL(k_acc_entry_base, dim3(grid_for(total)), dim3(kBlock), 0, 0, keys.data(), pos.data(),
+                       wpos.data(), total, ent_pos.data(), ent_wpos.data());
+    hipLaunchKernelGGL(k_acc_info, dim3(grid_for(total)), dim3(kBlock), 0, 0, keys.data(), ids.data(), first.data(),
+                       first_write.data(), pos.data(), wpos.data(), ent_pos.data(), ent_wpos.data(), total,
+                       info.data());
+    GP_LAUNCH_CHECK("postsolve access info");
+  }
+  DVec<int32_t> done(static_cast<size_t>(entities)), writes_done(static_cast<size_t>(entities));
+  DVec<unsigned long long> pending(1);
+  GP_CHECK(hipMemsetAsync(done.data(), 0, sizeof(int32_t) * static_cast<size_t>(entities), 0));
+  GP_CHECK(hipMemsetAsync(writes_done.data(), 0, sizeof(int32_t) * static_cast<size_t>(entities), 0));
+  unsigned long long before = ~0ull;
+  for (int32_t round = 1;; ++round) {
+    GP_CHECK(hipMemsetAsync(pending.data(), 0, sizeof(unsigned long long), 0));
+    hipLaunchKernelGGL(k_round_replay, dim3(grid_for(count)), dim3(kBlock), 0, 0, s, t, count, primal ? 1 : 0,
+                       primal ? 0 : 1, acc_start.data(), info.data(), done.data(), writes_done.data(), state.data(),
+                       round);
+    hipLaunchKernelGGL(k_round_commit, dim3(grid_capped(count, kBlock, 2048)), dim3(kBlock), 0, 0, count,
+                       acc_start.data(), info.data(), done.data(), writes_done.data(), state.data(), round,
+                       pending.data());
+    GP_LAUNCH_CHECK("postsolve replay round");
+    const unsigned long long left = read1(pending.data());
+    if (left == 0) break;
+    // every round replays at least the earliest pending record in replay order
+    if (left >= before) throw Error(GP_ERR_RUNTIME, "postsolve replay made no progress");
+    if (round >= 16 && before - left < 64) {
+      hipLaunchKernelGGL(k_replay_pending, dim3(1), dim3(64), 0, 0, s, t, count, primal ? 1 : 0, primal ? 0 : 1,
+                         state.data());
+      GP_LAUNCH_CHECK("postsolve replay tail");
+      break;
+    }
+    before = left;
+  }
+  return true;
 }
 
 // Tapes shorter than this replay serially (the component analysis costs a
@@ -761,15 +854,13 @@ extern "C" int gp_postprocess(const double* x, const double* y, const double* z,
     if (original) s.model = *original;
     s.tol = 1e-7;
     PsTape pt{t.types.data(), t.istarts.data(), t.vstarts.data(), t.idx.data(), t.vals.data()};
-    if (t.count > 0 && t.count >= parallel_min_records() && t.count < INT32_MAX) {
-      // the dual pass is analysed after the primal replay: its pruning reads the final x
-      replay_components(s, pt, static_cast<int32_t>(t.count), m0, n0, true);
-      replay_components(s, pt, static_cast<int32_t>(t.count), m0, n0, false);
-    } else if (t.count > 0) {
+    const bool rounds = t.count > 0 && t.count >= parallel_min_records() && t.count < INT32_MAX;
+    // the dual pass is analysed after the primal replay: its pruning reads the final x
+    if (t.count > 0 && !(rounds && replay_rounds(s, pt, static_cast<int32_t>(t.count), m0, n0, true)))
       hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 1, 0);
+    if (t.count > 0 && !(rounds && replay_rounds(s, pt, static_cast<int32_t>(t.count), m0, n0, false)))
       hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 0, 1);
-      GP_LAUNCH_CHECK("postsolve replay");
-    }
+    GP_LAUNCH_CHECK("postsolve replay");
     if (original && n0 > 0) {
       DVec<uint8_t> protect(n0);
       GP_CHECK(hipMemsetAsync(protect.data(), 0, n0, 0));

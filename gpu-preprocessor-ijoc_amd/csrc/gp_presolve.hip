@@ -231,7 +231,10 @@ struct Deferred {
   DVec<uint8_t> entry_cols;
   DVec<double> reference_bounds;
   DVec<int64_t> live_total;
-  bool active = false, pending = false, stats_dirty = false, nnz_dirty = false;
+  // masks the statistics were last computed for, and incremental-update scratch
+  DVec<uint8_t> snap_rows, snap_cols, touched_rows, touched_cols;
+  DVec<int32_t> inc_scratch;
+  bool active = false, pending = false, stats_dirty = false, nnz_dirty = false, have_snapshot = false;
   void reset() {
     if (active) {
       plan_destroy(plan);
@@ -239,8 +242,13 @@ struct Deferred {
       entry_cols.reset();
       reference_bounds.reset();
       live_total.reset();
+      snap_rows.reset();
+      snap_cols.reset();
+      touched_rows.reset();
+      touched_cols.reset();
+      inc_scratch.reset();
     }
-    active = pending = stats_dirty = nnz_dirty = false;
+    active = pending = stats_dirty = nnz_dirty = have_snapshot = false;
   }
 };
 
@@ -273,11 +281,30 @@ struct Scheduler {
     d.active = d.stats_dirty = d.nnz_dirty = true;
   }
 
+  // Live statistics of the plan's masks: a full sweep for a new plan, then
+  // updates that visit only the rows and columns deleted since the last one.
   void refresh_stats() {
-    if (d.stats_dirty) {
+    if (!d.stats_dirty) return;
+    const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
+    bool done = false;
+    if (d.have_snapshot && p->incremental_live_stats)
+      done = update_live_stats(d.stats, cur.lp, d.plan, d.snap_rows.data(), d.snap_cols.data(),
+                               d.touched_rows.data(), d.touched_cols.data(), d.live_total.data(),
+                               d.inc_scratch.data());
+    if (!done) {
       refresh_live_stats(d.stats, cur.lp, d.plan, d.live_total.data());
-      d.stats_dirty = false;
+      if (!d.have_snapshot) {
+        d.snap_rows.resize(m ? m : 1);
+        d.snap_cols.resize(n ? n : 1);
+        d.touched_rows.resize(m ? m : 1);
+        d.touched_cols.resize(n ? n : 1);
+        d.inc_scratch.resize(4);
+        d.have_snapshot = true;
+      }
     }
+    d2d(d.snap_rows.data(), d.plan.keep_row_mask, static_cast<size_t>(m));
+    d2d(d.snap_cols.data(), d.plan.keep_col_mask, static_cast<size_t>(n));
+    d.stats_dirty = false;
   }
 
   int64_t live_nnz() {
@@ -565,6 +592,7 @@ void gp_default_params(gp_params* p) {
   p->enable_structure_specific_rules = 1;
   p->enable_startup = 1;
   p->bound_strengthening_bound_only_nnz_round_budget = 8000000;
+  p->incremental_live_stats = 1;
 }
 
 int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary* out) {

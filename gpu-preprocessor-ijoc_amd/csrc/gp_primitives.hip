@@ -605,6 +605,151 @@ void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan
   s.live_masks = 1;
 }
 
+// ---- incremental live statistics ----------------------------------------------------------
+// Keep masks only ever go from 1 to 0 inside one plan, so the statistics of
+// the current masks follow from those of an earlier snapshot by visiting the
+// entries of the rows and columns deleted since then
+// (update_live_stats_from_deletions, gpu_presolve_kernels.cu:520-682).  Every
+// quantity is an integer count or the single live entry of a row / column,
+// so the result equals a full refresh bit for bit.
+namespace {
+__device__ inline void block_add_u64(unsigned long long v, unsigned long long* total) {
+  __shared__ unsigned long long lds[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kBlock / 64; ++k) t += lds[k];
+    if (t) atomicAdd(total, t);
+  }
+}
+
+// Rows deleted since the snapshot: their formerly live entries leave the
+// live total; columns still live lose one entry each.
+__global__ void __launch_bounds__(kBlock) k_inc_rows(gp_csr A, const uint8_t* old_rows, const uint8_t* old_cols,
+                                                      const uint8_t* keep_rows, const uint8_t* keep_cols,
+                                                      int32_t* col_nnz, uint8_t* touched_cols,
+                                                      unsigned long long* removed, int32_t* invalid) {
+  unsigned long long count = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < A.rows;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (!old_rows[i]) {
+      if (keep_rows[i]) atomicExch(invalid, 1);
+      continue;
+    }
+    if (keep_rows[i]) continue;
+    for (gp_offset p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) {
+      const int32_t j = A.col_idx[p];
+      if (!old_cols[j]) continue;
+      ++count;
+      if (keep_cols[j]) {
+        atomicSub(col_nnz + j, 1);
+        touched_cols[j] = 1;
+      }
+    }
+  }
+  block_add_u64(count, removed);
+}
+
+// Columns deleted since the snapshot: rows still live lose one entry each.
+__global__ void __launch_bounds__(kBlock) k_inc_cols(gp_csr AT, const uint8_t* old_cols, const uint8_t* keep_rows,
+                                                      const uint8_t* keep_cols, int32_t* row_nnz,
+                                                      uint8_t* touched_rows, unsigned long long* removed,
+                                                      int32_t* invalid) {
+  unsigned long long count = 0;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < AT.rows;
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (!old_cols[j]) {
+      if (keep_cols[j]) atomicExch(invalid, 1);
+      continue;
+    }
+    if (keep_cols[j]) continue;
+    for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) {
+      const int32_t i = AT.col_idx[p];
+      if (!keep_rows[i]) continue;
+      ++count;
+      atomicSub(row_nnz + i, 1);
+      touched_rows[i] = 1;
+    }
+  }
+  block_add_u64(count, removed);
+}
+
+// Flags and the singleton entry of every row (column) deleted or touched
+// since the snapshot, as the full refresh writes them.
+__device__ inline void finalize_line(const gp_csr& M, int64_t r, bool active, const uint8_t* keep_inner,
+                                     int32_t* degree, uint8_t* empty, uint8_t* singleton, int32_t* support,
+                                     double* coefficient, int32_t* invalid) {
+  const int32_t d = active ? degree[r] : 0;
+  if (d < 0) atomicExch(invalid, 1);
+  degree[r] = d;
+  empty[r] = active && d == 0;
+  singleton[r] = active && d == 1;
+  int32_t s = -1;
+  double v = 0.0;
+  if (active && d == 1) {
+    gp_offset found = -1;
+    for (gp_offset p = M.row_ptr[r]; p < M.row_ptr[r + 1]; ++p)
+      if (keep_inner[M.col_idx[p]]) found = p;
+    if (found < 0) {
+      atomicExch(invalid, 1);
+    } else {
+      s = M.col_idx[found];
+      v = M.vals[found];
+    }
+  }
+  support[r] = s;
+  coefficient[r] = v;
+}
+
+__global__ void k_inc_finalize(gp_csr A, gp_csr AT, const uint8_t* old_rows, const uint8_t* old_cols,
+                               const uint8_t* keep_rows, const uint8_t* keep_cols, const uint8_t* touched_rows,
+                               const uint8_t* touched_cols, gp_stats s, int32_t* invalid) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < A.rows && ((old_rows[k] && !keep_rows[k]) || touched_rows[k]))
+    finalize_line(A, k, keep_rows[k] != 0, keep_cols, s.row_nnz, s.empty_row_mask, s.singleton_row_mask,
+                  s.singleton_row_col, s.singleton_row_val, invalid);
+  if (k < AT.rows && ((old_cols[k] && !keep_cols[k]) || touched_cols[k]))
+    finalize_line(AT, k, keep_cols[k] != 0, keep_rows, s.col_nnz, s.empty_col_mask, s.singleton_column_mask,
+                  s.singleton_column_row, s.singleton_column_val, invalid);
+}
+
+__global__ void k_inc_total(int64_t* total, const unsigned long long* removed, int32_t* invalid) {
+  if (static_cast<unsigned long long>(*total) < *removed) {
+    *invalid = 1;
+    return;
+  }
+  *total -= static_cast<int64_t>(*removed);
+}
+}  // namespace
+
+bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, const uint8_t* old_rows,
+                       const uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
+                       int32_t* scratch) {
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (!plan.keep_row_mask || !plan.keep_col_mask || !device_total) return false;
+  TimedScope ts(GP_KT_LIVE_STATS, 2.0 * (m + n) + 2.0 * (m + n) + 4.0 * (m + n));
+  if (m) GP_CHECK(hipMemsetAsync(touched_rows, 0, static_cast<size_t>(m), 0));
+  if (n) GP_CHECK(hipMemsetAsync(touched_cols, 0, static_cast<size_t>(n), 0));
+  GP_CHECK(hipMemsetAsync(scratch, 0, 4 * sizeof(int32_t), 0));
+  unsigned long long* removed = reinterpret_cast<unsigned long long*>(scratch);
+  int32_t* invalid = scratch + 2;
+  if (m)
+    hipLaunchKernelGGL(k_inc_rows, dim3(grid_capped(m, kBlock, 4096)), dim3(kBlock), 0, 0, lp.A, old_rows, old_cols,
+                       plan.keep_row_mask, plan.keep_col_mask, s.col_nnz, touched_cols, removed, invalid);
+  if (n)
+    hipLaunchKernelGGL(k_inc_cols, dim3(grid_capped(n, kBlock, 4096)), dim3(kBlock), 0, 0, lp.AT, old_cols,
+                       plan.keep_row_mask, plan.keep_col_mask, s.row_nnz, touched_rows, removed, invalid);
+  const int64_t most = m > n ? m : n;
+  if (most)
+    hipLaunchKernelGGL(k_inc_finalize, dim3(grid_for(most)), dim3(kBlock), 0, 0, lp.A, lp.AT, old_rows, old_cols,
+                       plan.keep_row_mask, plan.keep_col_mask, touched_rows, touched_cols, s, invalid);
+  hipLaunchKernelGGL(k_inc_total, dim3(1), dim3(1), 0, 0, device_total, removed, invalid);
+  GP_LAUNCH_CHECK("incremental live stats");
+  return read1(invalid) == 0;
+}
+
 // ---- row activity summary ------------------------------------------------------------------
 namespace {
 __global__ void k_activity(gp_csr A, const double* l, const double* u, double zero_tol, const uint8_t* keep_row,

@@ -87,6 +87,7 @@ class GpParams(C.Structure):
         ("bound_strengthening_bound_only_nnz_round_budget", C.c_int64),
         ("collect_trace", C.c_int32), ("learned_schedule", C.c_int32),
         ("scheduler_weights", C.c_void_p), ("kernel_timing", C.c_int32),
+        ("incremental_live_stats", C.c_int32),
     ]
 
 

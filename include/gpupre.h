@@ -167,6 +167,10 @@ typedef struct gp_params {
   int32_t learned_schedule;  /* 1: consult the learned pass scheduler           */
   const float* scheduler_weights; /* host MLP weights when learned_schedule=1    */
   int32_t kernel_timing;     /* 1: time every kernel family with HIP events     */
+  int32_t incremental_live_stats; /* 1 (default): update live statistics from the
+                                     deleted lines instead of re-sweeping A and AT
+                                     (identical results; the reference's runtime
+                                     option of the same name, runtime_options.hpp) */
 } gp_params;
 
 /* One scheduler execution step (the reference prints these as

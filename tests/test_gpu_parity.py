@@ -249,3 +249,25 @@ def test_public_pipeline_contract():
             assert_same(getattr(after, k), getattr(before, k), f"input {k} unchanged")
         s.free()
         s.free()
+
+
+def test_incremental_live_stats_match_full_refresh():
+    """Live statistics updated from deleted lines (default) and re-swept in
+    full give the same presolve, tape and reduced LP."""
+    lp = G.generate_workload("medium", seed=1)
+    with G.DeviceLP(lp) as d:
+        out = []
+        for inc in (0, 1):
+            s = G.preprocess(d, G.parity_params(incremental_live_stats=inc, record_postsolve_tape_cpu=1))
+            try:
+                rec, red = s.record(), s.reduced()
+                out.append((s.iterations, s.obj_constant_delta, rec, red))
+            finally:
+                s.free()
+    (i0, o0, r0, d0), (i1, o1, r1, d1) = out
+    assert i0 == i1
+    assert_same(np.array([o1]), np.array([o0]), "obj constant")
+    for k in ("types", "index_starts", "value_starts", "indices", "vals", "row_red2org", "col_red2org"):
+        assert_same(getattr(r1, k), getattr(r0, k), f"record {k}")
+    for k in ("row_ptr", "col_idx", "vals", "c", "l", "u", "AL", "AU"):
+        assert_same(getattr(d1, k), getattr(d0, k), f"reduced {k}")
