@@ -325,6 +325,77 @@ int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new);
 // Sum in the fixed pairwise order (chunks of 1024, stride halving, recursive).
 double det_sum(const double* x, int64_t n);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n);
+// ---- LDS-staged row folds --------------------------------------------------------
+// A row fold visits every stored entry of a row in CSR order and folds the
+// entry's bound terms (a*l, a*u oriented by the sign of a) into per-row state
+// sequentially, so order-dependent sums come out as in a one-lane-per-row
+// sweep.  Rows go 256 to a block; their entries stream through LDS in chunks:
+// column indices, values and the columns' (l, u) pairs are loaded
+// entry-parallel (coalesced index/value loads, one 16-byte gather per entry),
+// then each lane folds its own row's staged terms.  The policy P supplies
+//   bool active(row)                        -- fold this row?
+//   bool use(a)                             -- entry contributes (else "skipped")
+//   void init(State&)
+//   void add(State&, pos_in_row, lo, hi, skipped)   -- called for every stored entry
+//   void finish(row, const State&, active)  -- called for every row of the grid
+constexpr int kFoldRows = 256;
+constexpr int kFoldChunk = 2048;
+
+template <class P>
+__global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2* lu, Bits keep_col, P pol) {
+  __shared__ double s_lo[kFoldChunk];
+  __shared__ double s_hi[kFoldChunk];
+  __shared__ uint8_t s_skip[kFoldChunk];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kFoldRows;
+  const int64_t r1 = r0 + kFoldRows < A.rows ? r0 + kFoldRows : A.rows;
+  const int64_t row = r0 + threadIdx.x;
+  const bool mine = row < r1 && pol.active(row);
+  typename P::State st;
+  pol.init(st);
+  if (__syncthreads_or(mine)) {
+    const gp_offset e0 = A.row_ptr[r0], e1 = A.row_ptr[r1];
+    const gp_offset p0 = mine ? A.row_ptr[row] : 0, p1 = mine ? A.row_ptr[row + 1] : 0;
+    for (gp_offset c0 = e0; c0 < e1; c0 += kFoldChunk) {
+      const gp_offset c1 = e1 - c0 > kFoldChunk ? c0 + kFoldChunk : e1;
+      for (gp_offset q = c0 + threadIdx.x; q < c1; q += kFoldRows) {
+        const int32_t j = A.col_idx[q];
+        const double a = A.vals[q];
+        const bool skip = !keep_col[j] || !pol.use(a);
+        double lo = 0.0, hi = 0.0;
+        if (!skip) {
+          const double2 b = lu[j];
+          lo = a >= 0.0 ? a * b.x : a * b.y;
+          hi = a >= 0.0 ? a * b.y : a * b.x;
+        }
+        s_lo[q - c0] = lo;
+        s_hi[q - c0] = hi;
+        s_skip[q - c0] = skip ? 1 : 0;
+      }
+      __syncthreads();
+      if (mine) {
+        const gp_offset a0 = p0 > c0 ? p0 : c0, a1 = p1 < c1 ? p1 : c1;
+        for (gp_offset q = a0; q < a1; ++q) pol.add(st, q - p0, s_lo[q - c0], s_hi[q - c0], s_skip[q - c0] != 0);
+      }
+      __syncthreads();
+    }
+  }
+  if (row < r1) pol.finish(row, st, mine);
+}
+
+template <class P>
+void launch_row_fold(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol) {
+  if (A.rows <= 0) return;
+  const int64_t blocks = (static_cast<int64_t>(A.rows) + kFoldRows - 1) / kFoldRows;
+  hipLaunchKernelGGL(k_row_fold<P>, dim3(static_cast<unsigned>(blocks)), dim3(kFoldRows), 0, 0, A, lu, keep_col, pol);
+}
+
+// (l[j], u[j]) interleaved, so a sweep gathers both bounds with one load.
+struct PackedBounds {
+  DVec<double2> lu;
+  PackedBounds(const double* l, const double* u, int32_t n);  // gp_primitives.hip
+  const double2* data() const { return lu.data(); }
+};
+
 // Stable ascending sort of (key, value) pairs on bits [0, end_bit) of the key,
 // in place (gp_sort.hip).
 void sort_keys_u64(uint64_t* keys, int64_t n, int end_bit = 64);
@@ -354,5 +425,17 @@ bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
 void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
                           const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
                           double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
+void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
+                          const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
+                          int32_t* max_pinf);
+
+// Lanes per row for a group-per-row sweep of rows with this mean length.
+inline int lanes_for_mean(double mean) {
+  if (mean <= 3.0) return 2;
+  if (mean <= 10.0) return 4;
+  if (mean <= 24.0) return 8;
+  if (mean <= 64.0) return 16;
+  return 64;
+}
 
 }  // namespace gp

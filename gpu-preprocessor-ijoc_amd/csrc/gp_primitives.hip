@@ -421,14 +421,20 @@ void csr_pool_free(gp_csr& A) {
 
 // ---- compaction ------------------------------------------------------------------------------
 namespace {
-__global__ void k_compact_count(const gp_offset* rp, const int32_t* col, int32_t m, const uint8_t* keep_row,
-                                Bits keep_col, int64_t* cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  int64_t c = 0;
-  if (!keep_row || keep_row[i])
-    for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) c += (!keep_col || keep_col[col[p]]) ? 1 : 0;
-  cnt[i] = c;
+// G lanes per row (a stable filter: the survivors of a row keep their order).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_compact_count(const gp_offset* rp, const int32_t* col, int32_t m,
+                                                           const uint8_t* keep_row, Bits keep_col, int64_t* cnt) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
+    int32_t c = 0;
+    if (!keep_row || keep_row[i])
+      for (gp_offset p = rp[i] + lane; p < rp[i + 1]; p += G) c += keep_col[col[p]] ? 1 : 0;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
+    if (lane == 0) cnt[i] = c;
+  }
 }
 __global__ void k_compact_rowptr(const int64_t* excl, const int32_t* row_o2n, int32_t m, gp_offset* out_rp) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -436,19 +442,35 @@ __global__ void k_compact_rowptr(const int64_t* excl, const int32_t* row_o2n, in
   const int32_t r = row_o2n ? row_o2n[i] : static_cast<int32_t>(i);
   if (r >= 0) out_rp[r] = static_cast<gp_offset>(excl[i]);
 }
-__global__ void k_compact_copy(const gp_offset* rp, const int32_t* col, const double* val, int32_t m,
-                               const int32_t* row_o2n, const int32_t* col_o2n, const int64_t* excl,
-                               int32_t* out_col, double* out_val) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  if (row_o2n && row_o2n[i] < 0) return;
-  int64_t q = excl[i];
-  for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) {
-    const int32_t j = col_o2n ? col_o2n[col[p]] : col[p];
-    if (j < 0) continue;
-    out_col[q] = j;
-    out_val[q] = val[p];
-    ++q;
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_compact_copy(const gp_offset* rp, const int32_t* col, const double* val,
+                                                          int32_t m, const int32_t* row_o2n, const int32_t* col_o2n,
+                                                          const int64_t* excl, int32_t* out_col, double* out_val) {
+  const int lane = threadIdx.x % G;
+  const int wl = static_cast<int>(__lane_id());
+  const unsigned long long group_bits = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (wl - lane);
+  const unsigned long long below = (1ull << wl) - 1ull;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
+    if (row_o2n && row_o2n[i] < 0) continue;
+    int64_t q = excl[i];
+    const gp_offset p1 = rp[i + 1];
+    for (gp_offset base = rp[i]; base < p1; base += G) {
+      const gp_offset p = base + lane;
+      int32_t j = -1;
+      double v = 0.0;
+      if (p < p1) {
+        j = col_o2n ? col_o2n[col[p]] : col[p];
+        v = val[p];
+      }
+      const unsigned long long keep = __ballot(j >= 0) & group_bits;
+      if (j >= 0) {
+        const int64_t at = q + __popcll(keep & below);
+        out_col[at] = j;
+        out_val[at] = v;
+      }
+      q += __popcll(keep);
+    }
   }
 }
 __global__ void k_set_last(gp_offset* rp, int32_t at, gp_offset v) { rp[at] = v; }
@@ -464,8 +486,15 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
     TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
     if (m > 0) {
       const PackedMask cols(keep_col, A.cols);
-      hipLaunchKernelGGL(k_compact_count, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m,
-                         keep_row, cols.view(), cnt.data());
+      const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
+      const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
+      switch (lanes) {
+        case 2: hipLaunchKernelGGL(k_compact_count<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
+        case 4: hipLaunchKernelGGL(k_compact_count<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
+        case 8: hipLaunchKernelGGL(k_compact_count<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
+        case 16: hipLaunchKernelGGL(k_compact_count<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
+        default: hipLaunchKernelGGL(k_compact_count<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
+      }
       GP_LAUNCH_CHECK("compact count");
     }
     exclusive_scan_i64(cnt.data(), m, &total);
@@ -481,8 +510,15 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   if (m > 0) {
     hipLaunchKernelGGL(k_compact_rowptr, dim3(grid_for(m)), dim3(kBlock), 0, 0, cnt.data(), row_o2n, m, out.row_ptr);
     GP_LAUNCH_CHECK("compact rowptr");
-    hipLaunchKernelGGL(k_compact_copy, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
-                       row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals);
+    const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
+    const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
+    switch (lanes) {
+      case 2: hipLaunchKernelGGL(k_compact_copy<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+      case 4: hipLaunchKernelGGL(k_compact_copy<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+      case 8: hipLaunchKernelGGL(k_compact_copy<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+      case 16: hipLaunchKernelGGL(k_compact_copy<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+      default: hipLaunchKernelGGL(k_compact_copy<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+    }
     GP_LAUNCH_CHECK("compact copy");
   }
   hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, 0, out.row_ptr, m_new, static_cast<gp_offset>(total));
@@ -750,51 +786,78 @@ bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
   return read1(invalid) == 0;
 }
 
-// ---- row activity summary ------------------------------------------------------------------
+// ---- interleaved bounds ----------------------------------------------------------------
 namespace {
-__global__ void k_activity(gp_csr A, const double* l, const double* u, double zero_tol, const uint8_t* keep_row,
-                           Bits keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
-                           int32_t* max_pinf) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= A.rows) return;
-  RowAcc lo, hi;
-  int32_t ni = 0, pi = 0;
-  const bool live = !keep_row || keep_row[i];
-  if (live) {
-    const gp_offset p0 = A.row_ptr[i], p1 = A.row_ptr[i + 1];
-    for (gp_offset p = p0; p < p1; ++p) {
-      lo.pos(p - p0);
-      hi.pos(p - p0);
-      const int32_t j = A.col_idx[p];
-      if (keep_col && !keep_col[j]) continue;
-      const double a = A.vals[p];
-      if (fabs(a) <= zero_tol) continue;
-      const double lj = l[j], uj = u[j];
-      const double tlo = a >= 0.0 ? a * lj : a * uj;
-      const double thi = a >= 0.0 ? a * uj : a * lj;
-      if (isfinite(tlo)) lo.add(tlo);
-      else if (tlo < 0.0) ++ni;
-      if (isfinite(thi)) hi.add(thi);
-      else if (thi > 0.0) ++pi;
-    }
-  }
-  min_fin[i] = lo.value();
-  max_fin[i] = hi.value();
-  min_ninf[i] = ni;
-  max_pinf[i] = pi;
+__global__ void k_pack_bounds(const double* l, const double* u, int32_t n, double2* lu) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n) lu[j] = make_double2(l[j], u[j]);
 }
 }  // namespace
+
+PackedBounds::PackedBounds(const double* l, const double* u, int32_t n) : lu(static_cast<size_t>(n > 0 ? n : 1)) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu.data());
+  GP_LAUNCH_CHECK("pack bounds");
+}
+
+// ---- row activity summary ------------------------------------------------------------------
+// Per live row: the finite parts of the minimum / maximum activity (blocked
+// Neumaier sums in CSR order) and the counts of -inf / +inf terms; entries
+// of deleted columns and with |a| <= zero_tol are skipped.
+namespace {
+struct ActivityFold {
+  const uint8_t* keep_row;
+  double zero_tol;
+  double* min_fin;
+  double* max_fin;
+  int32_t* min_ninf;
+  int32_t* max_pinf;
+  struct State {
+    RowAcc lo, hi;
+    int32_t ni, pi;
+  };
+  __device__ bool active(int64_t i) const { return !keep_row || keep_row[i]; }
+  __device__ bool use(double a) const { return fabs(a) > zero_tol; }
+  __device__ void init(State& s) const {
+    s.lo = RowAcc{};
+    s.hi = RowAcc{};
+    s.ni = s.pi = 0;
+  }
+  __device__ void add(State& s, int64_t k, double lo, double hi, bool skip) const {
+    s.lo.pos(k);
+    s.hi.pos(k);
+    if (skip) return;
+    if (isfinite(lo)) s.lo.add(lo);
+    else if (lo < 0.0) ++s.ni;
+    if (isfinite(hi)) s.hi.add(hi);
+    else if (hi > 0.0) ++s.pi;
+  }
+  __device__ void finish(int64_t i, const State& s, bool) const {
+    min_fin[i] = s.lo.value();
+    max_fin[i] = s.hi.value();
+    min_ninf[i] = s.ni;
+    max_pinf[i] = s.pi;
+  }
+};
+}  // namespace
+
+void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
+                          const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
+                          int32_t* max_pinf) {
+  if (A.rows <= 0) return;
+  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (double)(A.rows + A.cols) + 24.0 * A.rows;
+  TimedScope ts(GP_KT_ACTIVITY, bytes);
+  const PackedMask cols(keep_col, A.cols);
+  launch_row_fold(A, lu, cols.view(), ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf});
+  GP_LAUNCH_CHECK("activity summary");
+}
 
 void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
                           const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin, double* max_fin,
                           int32_t* min_ninf, int32_t* max_pinf) {
   if (A.rows <= 0) return;
-  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (double)(A.rows + A.cols) + 24.0 * A.rows;
-  TimedScope ts(GP_KT_ACTIVITY, bytes);
-  const PackedMask cols(keep_col, A.cols);
-  hipLaunchKernelGGL(k_activity, dim3(grid_for(A.rows)), dim3(kBlock), 0, 0, A, l, u, zero_tol, keep_row, cols.view(),
-                     min_fin, max_fin, min_ninf, max_pinf);
-  GP_LAUNCH_CHECK("activity summary");
+  const PackedBounds lu(l, u, A.cols);
+  row_activity_summary(A, lu.data(), zero_tol, keep_row, keep_col, min_fin, max_fin, min_ninf, max_pinf);
 }
 
 }  // namespace gp

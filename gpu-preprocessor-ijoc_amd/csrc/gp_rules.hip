@@ -381,46 +381,62 @@ struct Act {
   }
 };
 
-__global__ void k_redundant_rows(gp_csr A, const uint8_t* keep_row, Bits keep_col, const double* AL,
-                                 const double* AU, const double* l, const double* u, const int32_t* row_nnz,
-                                 double tol, uint8_t* full, uint8_t* dl, uint8_t* du, int32_t* flags) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= A.rows) return;
-  full[i] = dl[i] = du[i] = 0;
-  if (!keep_row[i] || row_nnz[i] <= 1) return;
-  const double al = AL[i], au = AU[i];
-  const bool lf = isfinite(al), uf = isfinite(au);
-  if (lf && uf && fabs(au - al) <= tol) return;
-  Act lo, hi;
-  const gp_offset p0 = A.row_ptr[i];
-  for (gp_offset p = p0; p < A.row_ptr[i + 1]; ++p) {
-    lo.fin.pos(p - p0);
-    hi.fin.pos(p - p0);
-    const int32_t j = A.col_idx[p];
-    if (keep_col && !keep_col[j]) continue;
-    const double a = A.vals[p];
-    const double lj = l[j], uj = u[j];
-    lo.add(a >= 0.0 ? a * lj : a * uj);
-    hi.add(a >= 0.0 ? a * uj : a * lj);
+// Row fold of the redundancy test: full activity range of each live row
+// with two or more live entries that is not an equality.
+struct RedundantFold {
+  const uint8_t* keep_row;
+  const int32_t* row_nnz;
+  const double* AL;
+  const double* AU;
+  double tol;
+  uint8_t* full;
+  uint8_t* dl;
+  uint8_t* du;
+  int32_t* flags;
+  struct State {
+    Act lo, hi;
+  };
+  __device__ bool active(int64_t i) const {
+    if (!keep_row[i] || row_nnz[i] <= 1) return false;
+    const double al = AL[i], au = AU[i];
+    return !(isfinite(al) && isfinite(au) && fabs(au - al) <= tol);
   }
-  const double rmin = lo.value(), rmax = hi.value();
-  if ((lf && rmax < al - tol) || (uf && rmin > au + tol)) {
-    wave_flag(flags + 0, 1);
-    return;
+  __device__ bool use(double) const { return true; }
+  __device__ void init(State& s) const {
+    s.lo = Act{};
+    s.hi = Act{};
   }
-  const bool li = !lf || rmin >= al - tol;
-  const bool ui = !uf || rmax <= au + tol;
-  if (li && ui) {
-    full[i] = 1;
-    wave_flag(flags + 1, 1);
-  } else if (lf && li) {
-    dl[i] = 1;
-    wave_flag(flags + 2, 1);
-  } else if (uf && ui) {
-    du[i] = 1;
-    wave_flag(flags + 3, 1);
+  __device__ void add(State& s, int64_t k, double lo, double hi, bool skip) const {
+    s.lo.fin.pos(k);
+    s.hi.fin.pos(k);
+    if (skip) return;
+    s.lo.add(lo);
+    s.hi.add(hi);
   }
-}
+  __device__ void finish(int64_t i, const State& s, bool active) const {
+    full[i] = dl[i] = du[i] = 0;
+    if (!active) return;
+    const double al = AL[i], au = AU[i];
+    const bool lf = isfinite(al), uf = isfinite(au);
+    const double rmin = s.lo.value(), rmax = s.hi.value();
+    if ((lf && rmax < al - tol) || (uf && rmin > au + tol)) {
+      wave_flag(flags + 0, 1);
+      return;
+    }
+    const bool li = !lf || rmin >= al - tol;
+    const bool ui = !uf || rmax <= au + tol;
+    if (li && ui) {
+      full[i] = 1;
+      wave_flag(flags + 1, 1);
+    } else if (lf && li) {
+      dl[i] = 1;
+      wave_flag(flags + 2, 1);
+    } else if (uf && ui) {
+      du[i] = 1;
+      wave_flag(flags + 3, 1);
+    }
+  }
+};
 __global__ void k_redundant_apply(int32_t m, const uint8_t* full, const uint8_t* dl, const uint8_t* du,
                                   uint8_t* keep_row, double* AL, double* AU) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -478,10 +494,10 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   DVec<uint8_t> full(m), dl(m), du(m);
   Flags flags;
   const PackedMask live_cols(s.live_masks ? static_cast<const uint8_t*>(plan.keep_col_mask) : nullptr, lp.A.cols);
-  hipLaunchKernelGGL(k_redundant_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask,
-                     live_cols.view(), plan.new_AL,
-                     plan.new_AU, plan.new_l, plan.new_u, s.row_nnz, p.bound_tol, full.data(), dl.data(), du.data(),
-                     flags.d.data());
+  const PackedBounds lu(plan.new_l, plan.new_u, lp.A.cols);
+  launch_row_fold(lp.A, lu.data(), live_cols.view(),
+                  RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
+                                dl.data(), du.data(), flags.d.data()});
   GP_LAUNCH_CHECK("redundant rows");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -544,40 +560,77 @@ __device__ inline void implied_bounds(double a, double lj, double uj, double al,
   }
 }
 
-__global__ void k_bs_candidates(gp_csr A, const uint8_t* keep_row, Bits keep_col, const int32_t* row_nnz,
-                                const double* AL, const double* AU, const double* l, const double* u,
-                                const double* min_fin, const double* max_fin, const int32_t* min_ninf,
-                                const int32_t* max_pinf, double zero_tol, double* cl, double* cu) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= A.rows || !keep_row[i] || row_nnz[i] <= 1) return;
-  const double al = AL[i], au = AU[i], mnf = min_fin[i], mxf = max_fin[i];
-  const int32_t mni = min_ninf[i], mxp = max_pinf[i];
-  for (gp_offset p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) {
-    const int32_t j = A.col_idx[p];
-    if (keep_col && !keep_col[j]) continue;
-    const double a = A.vals[p];
-    if (fabs(a) <= zero_tol) continue;
-    const double lj = l[j], uj = u[j];
-    double il, iu;
-    implied_bounds(a, lj, uj, al, au, mnf, mni, mxf, mxp, il, iu);
-    double nl = lj, nu = uj;
-    if (isfinite(il)) nl = cmax(nl, il);
-    if (isfinite(iu)) nu = cmin(nu, iu);
-    if (nl > lj) atomic_total_max(cl + j, nl);
-    if (nu < uj) atomic_total_min(cu + j, nu);
+// Candidate bounds: G lanes per row, each entry independent; the per-column
+// result is a total-order max / min, so the lane schedule does not matter.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_bs_candidates(gp_csr A, const uint8_t* keep_row, Bits keep_col,
+                                                           const int32_t* row_nnz, const double* AL, const double* AU,
+                                                           const double2* lu, const double* min_fin,
+                                                           const double* max_fin, const int32_t* min_ninf,
+                                                           const int32_t* max_pinf, double zero_tol, double* cl,
+                                                           double* cu) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < A.rows; i += groups) {
+    if (!keep_row[i] || row_nnz[i] <= 1) continue;
+    const double al = AL[i], au = AU[i], mnf = min_fin[i], mxf = max_fin[i];
+    const int32_t mni = min_ninf[i], mxp = max_pinf[i];
+    const gp_offset p1 = A.row_ptr[i + 1];
+    for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) {
+      const int32_t j = A.col_idx[p];
+      if (!keep_col[j]) continue;
+      const double a = A.vals[p];
+      if (fabs(a) <= zero_tol) continue;
+      const double2 b = lu[j];
+      const double lj = b.x, uj = b.y;
+      double il, iu;
+      implied_bounds(a, lj, uj, al, au, mnf, mni, mxf, mxp, il, iu);
+      double nl = lj, nu = uj;
+      if (isfinite(il)) nl = cmax(nl, il);
+      if (isfinite(iu)) nu = cmin(nu, iu);
+      if (nl > lj) atomic_total_max(cl + j, nl);
+      if (nu < uj) atomic_total_min(cu + j, nu);
+    }
   }
 }
 
-__global__ void k_col_max_abs(gp_csr AT, Bits keep_row, double* vmax) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= AT.rows) return;
-  double v = 0.0;
-  for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) {
-    if (keep_row && !keep_row[AT.col_idx[p]]) continue;
-    v = cmax(v, fabs(AT.vals[p]));
+// max |a| over the live entries of each column, G lanes per column.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_col_max_abs(gp_csr AT, Bits keep_row, double* vmax) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; j < AT.rows; j += groups) {
+    double v = 0.0;
+    const gp_offset p1 = AT.row_ptr[j + 1];
+    for (gp_offset p = AT.row_ptr[j] + lane; p < p1; p += G) {
+      if (!keep_row[AT.col_idx[p]]) continue;
+      v = cmax(v, fabs(AT.vals[p]));
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v = cmax(v, __shfl_xor(v, o, G));
+    if (lane == 0) vmax[j] = v;
   }
-  vmax[j] = v;
 }
+
+template <template <int> class K, class... Args>
+void launch_grouped(int lanes, int64_t rows, Args... args) {
+  const unsigned grid = grid_capped(rows * lanes, kBlock, 16384);
+  switch (lanes) {
+    case 2: hipLaunchKernelGGL(K<2>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 4: hipLaunchKernelGGL(K<4>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 8: hipLaunchKernelGGL(K<8>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 16: hipLaunchKernelGGL(K<16>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    default: hipLaunchKernelGGL(K<64>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+  }
+}
+template <int G>
+struct BsCandidatesK {
+  static constexpr auto kernel = k_bs_candidates<G>;
+};
+template <int G>
+struct ColMaxAbsK {
+  static constexpr auto kernel = k_col_max_abs<G>;
+};
 
 __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* l, const double* u,
                               const double* cl_in, const double* cu_in, const double* ref_l, const double* ref_u,
@@ -776,16 +829,19 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   DVec<uint8_t> lch(n), uch(n), fixed(n);
   d2d(cl.data(), plan.new_l, n);
   d2d(cu.data(), plan.new_u, n);
-  row_activity_summary(lp.A, plan.new_l, plan.new_u, p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(),
-                       max_fin.data(), min_ninf.data(), max_pinf.data());
+  const PackedBounds lu(plan.new_l, plan.new_u, n);
+  row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
+                       min_ninf.data(), max_pinf.data());
   {
     TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 40.0 * m + (double)(m + n) + 4.0 * m +
                                               16.0 * n + 16.0 * n);
     const PackedMask live_cols(keep_col, n);
-    hipLaunchKernelGGL(k_bs_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask,
-                       live_cols.view(),
-                       s.row_nnz, plan.new_AL, plan.new_AU, plan.new_l, plan.new_u, min_fin.data(), max_fin.data(),
-                       min_ninf.data(), max_pinf.data(), p.zero_tol, cl.data(), cu.data());
+    launch_grouped<BsCandidatesK>(lanes_for_mean(m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
+                                  plan.keep_row_mask, live_cols.view(), s.row_nnz, plan.new_AL, plan.new_AU,
+                                  static_cast<const double2*>(lu.data()), static_cast<const double*>(min_fin.data()),
+                                  static_cast<const double*>(max_fin.data()),
+                                  static_cast<const int32_t*>(min_ninf.data()),
+                                  static_cast<const int32_t*>(max_pinf.data()), p.zero_tol, cl.data(), cu.data());
     GP_LAUNCH_CHECK("bound candidates");
   }
   Flags flags;
@@ -793,7 +849,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   {
     TimedScope ts(GP_KT_RULES_OTHER, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + (double)m + 8.0 * n);
     const PackedMask live_rows(s.live_masks ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
-    hipLaunchKernelGGL(k_col_max_abs, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, live_rows.view(), vmax.data());
+    launch_grouped<ColMaxAbsK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
+                               live_rows.view(), vmax.data());
     GP_LAUNCH_CHECK("col max abs");
     const double* ref_l = plan.propagation_reference_l ? plan.propagation_reference_l : (lp.l ? lp.l : plan.new_l);
     const double* ref_u = plan.propagation_reference_u ? plan.propagation_reference_u : (lp.u ? lp.u : plan.new_u);

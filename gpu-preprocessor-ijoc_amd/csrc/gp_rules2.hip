@@ -69,44 +69,75 @@ __global__ void k_sc_count(int32_t n, const uint8_t* keep_col, const uint8_t* sm
 // (_kernel_singleton_row_activity_cache), then the bounds of the rest of the
 // row without column j (_singleton_column_activity_bounds_from_cache).  Every
 // singleton column of a row computes the same row activity.
-__device__ bool sc_rest_bounds(const gp_csr& A, int32_t row, Bits keep_col, const double* l, const double* u,
-                               double a, double lj, double uj, int32_t* live_out, double* rmin, double* rmax) {
-  RowAcc mn, mx;
-  int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, live = 0;
-  bool bad = false;
-  const gp_offset p0 = A.row_ptr[row];
-  for (gp_offset q = p0; q < A.row_ptr[row + 1]; ++q) {
-    mn.pos(q - p0);
-    mx.pos(q - p0);
-    const int32_t k = A.col_idx[q];
-    if (!keep_col[k]) continue;
-    ++live;
-    const double ak = A.vals[q];
-    const double tmin = ak >= 0.0 ? ak * l[k] : ak * u[k];
-    const double tmax = ak >= 0.0 ? ak * u[k] : ak * l[k];
+// Activity of a row over its live entries, in CSR order (the singleton-column
+// row activity cache, rule_singleton_columns.cu:555-754): finite parts of the
+// minimum / maximum activity, counts of -inf / +inf terms on each side, and
+// whether any term is NaN.
+struct ScAct {
+  double fmin, fmax;
+  int32_t min_neg, min_pos, max_neg, max_pos;
+  int32_t live, bad;
+};
+
+struct ScActivityFold {
+  const uint8_t* keep_row;
+  ScAct* out;
+  struct State {
+    RowAcc mn, mx;
+    int32_t c0, c1, c2, c3, live, bad;
+  };
+  __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
+  __device__ bool use(double) const { return true; }
+  __device__ void init(State& s) const {
+    s.mn = RowAcc{};
+    s.mx = RowAcc{};
+    s.c0 = s.c1 = s.c2 = s.c3 = s.live = s.bad = 0;
+  }
+  __device__ void add(State& s, int64_t k, double tmin, double tmax, bool skip) const {
+    s.mn.pos(k);
+    s.mx.pos(k);
+    if (skip) return;
+    ++s.live;
     if (isnan(tmin) || isnan(tmax)) {
-      bad = true;
-      continue;
+      s.bad = 1;
+      return;
     }
     if (isinf(tmin)) {
-      if (tmin < 0.0) ++c0;
-      else ++c1;
+      if (tmin < 0.0) ++s.c0;
+      else ++s.c1;
     } else {
-      mn.add(tmin);
+      s.mn.add(tmin);
     }
     if (isinf(tmax)) {
-      if (tmax < 0.0) ++c2;
-      else ++c3;
+      if (tmax < 0.0) ++s.c2;
+      else ++s.c3;
     } else {
-      mx.add(tmax);
+      s.mx.add(tmax);
     }
   }
-  *live_out = live;
-  if (bad) return false;
+  __device__ void finish(int64_t i, const State& s, bool active) const {
+    ScAct r;
+    r.fmin = s.mn.value();
+    r.fmax = s.mx.value();
+    r.min_neg = s.c0;
+    r.min_pos = s.c1;
+    r.max_neg = s.c2;
+    r.max_pos = s.c3;
+    r.live = active ? s.live : 0;
+    r.bad = s.bad;
+    out[i] = r;
+  }
+};
+
+// Activity of the support row without column j's own term
+// (_singleton_column_activity_bounds_from_cache, :756-816).
+__device__ bool sc_rest_bounds(const ScAct& act, double a, double lj, double uj, double* rmin, double* rmax) {
+  if (act.bad) return false;
   const double tmin = a >= 0.0 ? a * lj : a * uj;
   const double tmax = a >= 0.0 ? a * uj : a * lj;
   if (isnan(tmin) || isnan(tmax)) return false;
-  double vmin = mn.value(), vmax = mx.value();
+  int32_t c0 = act.min_neg, c1 = act.min_pos, c2 = act.max_neg, c3 = act.max_pos;
+  double vmin = act.fmin, vmax = act.fmax;
   if (isinf(tmin)) {
     if (tmin < 0.0) --c0;
     else --c1;
@@ -125,8 +156,8 @@ __device__ bool sc_rest_bounds(const gp_csr& A, int32_t row, Bits keep_col, cons
   return !isnan(*rmin) && !isnan(*rmax);
 }
 
-__global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col,
-                             Bits live_col, const uint8_t* smask, const int32_t* srow, const double* sval,
+__global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const uint8_t* keep_row,
+                             const uint8_t* keep_col, const uint8_t* smask, const int32_t* srow, const double* sval,
                              const double* c, const double* l, const double* u, const double* AL, const double* AU,
                              double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags) {
   const int64_t j = gtid();
@@ -136,10 +167,10 @@ __global__ void k_sc_propose(int32_t n, int32_t m, gp_csr A, const uint8_t* keep
   const double a = sval[j];
   if (fabs(a) <= zt) return;
   const double lhs = AL[row], rhs = AU[row], cj = c[j], lj = l[j], uj = u[j];
-  int32_t live = 0;
+  const ScAct act = row_act[row];
+  if (act.live <= 1) return;
   double rmin = 0.0, rmax = 0.0;
-  const bool rest_ok = sc_rest_bounds(A, row, live_col, l, u, a, lj, uj, &live, &rmin, &rmax);
-  if (live <= 1) return;
+  const bool rest_ok = sc_rest_bounds(act, a, lj, uj, &rmin, &rmax);
   if ((cj > zt && a > zt && !isfinite(lhs) && !isfinite(lj)) ||
       (cj > zt && a < -zt && !isfinite(rhs) && !isfinite(lj)) ||
       (cj < -zt && a < -zt && !isfinite(lhs) && !isfinite(uj)) ||
@@ -1398,9 +1429,15 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
   GP_LAUNCH_CHECK("singleton columns owner");
-  const PackedMask live_col(plan.keep_col_mask, n);
-  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, lp.A, plan.keep_row_mask,
-                     plan.keep_col_mask, live_col.view(), s.singleton_column_mask, s.singleton_column_row,
+  DVec<ScAct> row_act(m);
+  {
+    const PackedMask live_col(plan.keep_col_mask, n);
+    const PackedBounds lu(plan.new_l, plan.new_u, n);
+    launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, row_act.data()});
+    GP_LAUNCH_CHECK("singleton columns row activity");
+  }
+  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act.data(), plan.keep_row_mask,
+                     plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row,
                      s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
                      p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data());
   GP_LAUNCH_CHECK("singleton columns propose");
