@@ -188,15 +188,20 @@ inline unsigned grid_capped(int64_t n, int block = kBlock, int64_t cap = 4096) {
 // ---- kernel-family timing ---------------------------------------------------------
 // When enabled, every launch family is bracketed by HIP events on the null
 // stream; elapsed time and algorithmic bytes accumulate per family.
+// Scopes nest: an inner scope pauses the enclosing one, so each family's
+// time is exclusive of the families timed inside it.
 struct KernelClock {
   bool enabled = false;
   double ms[GP_KT_COUNT] = {};
   int64_t launches[GP_KT_COUNT] = {};
   double bytes[GP_KT_COUNT] = {};
   std::vector<hipEvent_t> pool;
-  struct Pending { int family; hipEvent_t a, b; double bytes; };
-  std::vector<Pending> pending;
+  struct Segment { int family; hipEvent_t a, b; };
+  std::vector<Segment> segments;  // recorded, not yet read
+  std::vector<int> open;          // families of the open scopes, innermost last
   hipEvent_t take();
+  void start_segment(int family);
+  void stop_segment();
   int begin(int family);                // returns a token (or -1)
   void end(int token, double algorithmic_bytes);
   void flush();                          // sync + accumulate

@@ -639,7 +639,8 @@ int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary
       }
       kc.flush();
     } catch (...) {
-      kc.pending.clear();
+      kc.segments.clear();
+      kc.open.clear();
       kc.enabled = false;
       gp_free_summary(out);
       throw;

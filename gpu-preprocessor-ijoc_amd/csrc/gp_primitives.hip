@@ -82,34 +82,44 @@ hipEvent_t KernelClock::take() {
   return e;
 }
 
+void KernelClock::start_segment(int family) {
+  Segment g{family, take(), take()};
+  GP_CHECK(hipEventRecord(g.a, 0));
+  segments.push_back(g);
+}
+
+void KernelClock::stop_segment() {
+  (void)hipEventRecord(segments.back().b, 0);
+}
+
 int KernelClock::begin(int family) {
   if (!enabled) return -1;
-  Pending p{family, take(), take(), 0.0};
-  GP_CHECK(hipEventRecord(p.a, 0));
-  pending.push_back(p);
-  return static_cast<int>(pending.size()) - 1;
+  if (!open.empty()) stop_segment();
+  open.push_back(family);
+  launches[family] += 1;
+  start_segment(family);
+  return family;
 }
 
 void KernelClock::end(int token, double algorithmic_bytes) {
-  if (token < 0 || !enabled) return;
-  Pending& p = pending[static_cast<size_t>(token)];
-  p.bytes = algorithmic_bytes;
-  (void)hipEventRecord(p.b, 0);
+  if (token < 0 || !enabled || open.empty()) return;
+  stop_segment();
+  bytes[token] += algorithmic_bytes;
+  open.pop_back();
+  if (!open.empty()) start_segment(open.back());
 }
 
 void KernelClock::flush() {
-  if (pending.empty()) return;
+  if (segments.empty()) return;
   GP_CHECK(hipDeviceSynchronize());
-  for (auto& p : pending) {
+  for (auto& g : segments) {
     float t = 0.f;
-    GP_CHECK(hipEventElapsedTime(&t, p.a, p.b));
-    ms[p.family] += t;
-    launches[p.family] += 1;
-    bytes[p.family] += p.bytes;
-    pool.push_back(p.a);
-    pool.push_back(p.b);
+    GP_CHECK(hipEventElapsedTime(&t, g.a, g.b));
+    ms[g.family] += t;
+    pool.push_back(g.a);
+    pool.push_back(g.b);
   }
-  pending.clear();
+  segments.clear();
 }
 
 KernelClock::~KernelClock() {}
@@ -765,7 +775,7 @@ bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
                        int32_t* scratch) {
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (!plan.keep_row_mask || !plan.keep_col_mask || !device_total) return false;
-  TimedScope ts(GP_KT_LIVE_STATS, 2.0 * (m + n) + 2.0 * (m + n) + 4.0 * (m + n));
+  TimedScope ts(GP_KT_LIVE_STATS_INC, 2.0 * (m + n) + 2.0 * (m + n) + 4.0 * (m + n));
   if (m) GP_CHECK(hipMemsetAsync(touched_rows, 0, static_cast<size_t>(m), 0));
   if (n) GP_CHECK(hipMemsetAsync(touched_cols, 0, static_cast<size_t>(n), 0));
   GP_CHECK(hipMemsetAsync(scratch, 0, 4 * sizeof(int32_t), 0));
@@ -845,9 +855,10 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf) {
   if (A.rows <= 0) return;
-  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (double)(A.rows + A.cols) + 24.0 * A.rows;
-  TimedScope ts(GP_KT_ACTIVITY, bytes);
   const PackedMask cols(keep_col, A.cols);
+  // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per row
+  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows;
+  TimedScope ts(GP_KT_ACTIVITY, bytes);
   launch_row_fold(A, lu, cols.view(), ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf});
   GP_LAUNCH_CHECK("activity summary");
 }

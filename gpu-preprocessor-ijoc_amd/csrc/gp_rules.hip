@@ -490,15 +490,19 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
   if (m == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + 16.0 * m);
+  TimedScope outer(GP_KT_RULES_OTHER, 0.0);
   DVec<uint8_t> full(m), dl(m), du(m);
   Flags flags;
   const PackedMask live_cols(s.live_masks ? static_cast<const uint8_t*>(plan.keep_col_mask) : nullptr, lp.A.cols);
   const PackedBounds lu(plan.new_l, plan.new_u, lp.A.cols);
+  {
+  // A, (l, u) and a mask bit per column, keep_row, row_nnz, AL, AU, 3 flags out per row
+  TimedScope ts(GP_KT_REDUNDANT, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + lp.A.cols / 8.0 + 24.0 * m);
   launch_row_fold(lp.A, lu.data(), live_cols.view(),
                   RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
                                 dl.data(), du.data(), flags.d.data()});
   GP_LAUNCH_CHECK("redundant rows");
+  }
   const int32_t* f = flags.read();
   if (f[0]) {
     plan.has_infeasible = 1;
@@ -833,9 +837,10 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data());
   {
-    TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 40.0 * m + (double)(m + n) + 4.0 * m +
-                                              16.0 * n + 16.0 * n);
     const PackedMask live_cols(keep_col, n);
+    // A, keep_row, row_nnz, AL, AU, the row summary (24 B), (l, u) and a mask bit per column,
+    // candidate bounds cl / cu
+    TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 45.0 * m + 32.0 * n + n / 8.0);
     launch_grouped<BsCandidatesK>(lanes_for_mean(m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
                                   plan.keep_row_mask, live_cols.view(), s.row_nnz, plan.new_AL, plan.new_AU,
                                   static_cast<const double2*>(lu.data()), static_cast<const double*>(min_fin.data()),
@@ -847,11 +852,15 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   Flags flags;
   const PackedMask rows_all(plan.keep_row_mask, m);
   {
-    TimedScope ts(GP_KT_RULES_OTHER, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + (double)m + 8.0 * n);
+    TimedScope outer(GP_KT_RULES_OTHER, 0.0);
     const PackedMask live_rows(s.live_masks ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
-    launch_grouped<ColMaxAbsK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
-                               live_rows.view(), vmax.data());
-    GP_LAUNCH_CHECK("col max abs");
+    {
+      // AT row_ptr, col_idx, vals, a mask bit per row, 8 bytes out per column
+      TimedScope ts(GP_KT_COL_MAX, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + m / 8.0 + 8.0 * n);
+      launch_grouped<ColMaxAbsK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
+                                 live_rows.view(), vmax.data());
+      GP_LAUNCH_CHECK("col max abs");
+    }
     const double* ref_l = plan.propagation_reference_l ? plan.propagation_reference_l : (lp.l ? lp.l : plan.new_l);
     const double* ref_u = plan.propagation_reference_u ? plan.propagation_reference_u : (lp.u ? lp.u : plan.new_u);
     hipLaunchKernelGGL(k_bs_finalize, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask, plan.new_l,

@@ -1433,6 +1433,8 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   {
     const PackedMask live_col(plan.keep_col_mask, n);
     const PackedBounds lu(plan.new_l, plan.new_u, n);
+    // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
+    TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
     launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, row_act.data()});
     GP_LAUNCH_CHECK("singleton columns row activity");
   }

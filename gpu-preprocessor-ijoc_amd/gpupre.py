@@ -43,7 +43,7 @@ RED_ROW_IMPLIED_BOUND_CHANGE = 12
 RED_DOUBLETON_EQUATION = 13
 PHASES = ["cleanup", "propagation", "aggregation", "parallel", "end"]
 KERNEL_FAMILIES = ["live_stats", "activity", "bound_candidates", "compact", "transpose",
-                   "rules_other", "hash_sort"]
+                   "rules_other", "hash_sort", "sc_activity", "redundant", "col_max", "live_stats_inc"]
 
 
 class GpCsr(C.Structure):
@@ -97,7 +97,7 @@ class GpRoundTrace(C.Structure):
                 ("fired_mask", C.c_int32), ("skipped_mask", C.c_int32), ("seconds", C.c_double)]
 
 
-GP_KT_COUNT = 7
+GP_KT_COUNT = 11
 
 
 class GpSummary(C.Structure):

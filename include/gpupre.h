@@ -189,14 +189,18 @@ typedef struct gp_round_trace {
 
 /* Per-kernel-family device time accumulated when params.kernel_timing=1. */
 enum {
-  GP_KT_LIVE_STATS = 0,
-  GP_KT_ACTIVITY = 1,
-  GP_KT_BOUND_CANDIDATES = 2,
-  GP_KT_COMPACT = 3,
-  GP_KT_TRANSPOSE = 4,
-  GP_KT_RULES_OTHER = 5,
-  GP_KT_HASH_SORT = 6,
-  GP_KT_COUNT = 7
+  GP_KT_LIVE_STATS = 0,       /* full live-statistics sweep of A and AT        */
+  GP_KT_ACTIVITY = 1,         /* row activity summary (row fold)               */
+  GP_KT_BOUND_CANDIDATES = 2, /* bound-strengthening candidates                */
+  GP_KT_COMPACT = 3,          /* CSR compaction at materialize                 */
+  GP_KT_TRANSPOSE = 4,        /* CSR transpose                                 */
+  GP_KT_RULES_OTHER = 5,      /* everything else inside the rules              */
+  GP_KT_HASH_SORT = 6,        /* parallel rows / columns hashing and sorts     */
+  GP_KT_SC_ACTIVITY = 7,      /* singleton-column row activity cache (row fold)*/
+  GP_KT_REDUNDANT = 8,        /* redundant-row activity test (row fold)        */
+  GP_KT_COL_MAX = 9,          /* column max |a|                                */
+  GP_KT_LIVE_STATS_INC = 10,  /* live statistics from deleted lines            */
+  GP_KT_COUNT = 11
 };
 
 /* FoldingPipelineSummary + GpuPresolveSummary, flattened.  Value semantics:
