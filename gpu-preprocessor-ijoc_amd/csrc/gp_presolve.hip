@@ -132,6 +132,7 @@ void plan_destroy(gp_plan& plan) {
   pool_free(plan.new_AU);
   if (plan.has_new_A) csr_pool_free(plan.new_A);
   delete plan_tape(plan);
+  rule_cache_free(plan.rule_cache);
   std::memset(&plan, 0, sizeof(plan));
 }
 

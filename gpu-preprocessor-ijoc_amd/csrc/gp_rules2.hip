@@ -129,6 +129,77 @@ struct ScActivityFold {
   }
 };
 
+// The same row activity for one row on one lane (the sparse-update path).
+__device__ ScAct sc_row_activity(const gp_csr& A, int64_t i, Bits keep_col, const double2* lu) {
+  ScActivityFold f{nullptr, nullptr};
+  ScActivityFold::State st;
+  f.init(st);
+  const gp_offset p0 = A.row_ptr[i], p1 = A.row_ptr[i + 1];
+  for (gp_offset q = p0; q < p1; ++q) {
+    const int32_t j = A.col_idx[q];
+    const double a = A.vals[q];
+    const bool skip = !keep_col[j];
+    double lo = 0.0, hi = 0.0;
+    if (!skip) {
+      const double2 b = lu[j];
+      lo = a >= 0.0 ? a * b.x : a * b.y;
+      hi = a >= 0.0 ? a * b.y : a * b.x;
+    }
+    f.add(st, q - p0, lo, hi, skip);
+  }
+  ScAct r;
+  r.fmin = st.mn.value();
+  r.fmax = st.mx.value();
+  r.min_neg = st.c0;
+  r.min_pos = st.c1;
+  r.max_neg = st.c2;
+  r.max_pos = st.c3;
+  r.live = st.live;
+  r.bad = st.bad;
+  return r;
+}
+
+// Row activities kept with the plan between singleton-column calls.  A row's
+// entry is recomputed only when one of its columns changed bounds or
+// liveness, or the row's own liveness changed, since the last call; the
+// values are those a full sweep would produce.
+struct ScCache {
+  int32_t m = -1, n = -1;
+  bool valid = false;
+  DVec<ScAct> act;
+  DVec<double2> lu;
+  DVec<uint8_t> keep_row, keep_col;
+};
+
+__global__ void k_sc_dirty_cols(gp_csr AT, const double2* lu, const double2* lu_old, const uint8_t* keep_col,
+                                const uint8_t* keep_col_old, uint8_t* dirty) {
+  const int64_t j = gtid();
+  if (j >= AT.rows) return;
+  const double2 a = lu[j], b = lu_old[j];
+  const bool same = __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
+                    __double_as_longlong(a.y) == __double_as_longlong(b.y) && keep_col[j] == keep_col_old[j];
+  if (same) return;
+  for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) dirty[AT.col_idx[p]] = 1;
+}
+
+__global__ void k_sc_dirty_rows(int32_t m, const uint8_t* keep_row, const uint8_t* keep_row_old, uint8_t* dirty,
+                                int32_t* list, int32_t* count) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  if (keep_row[i] != keep_row_old[i]) dirty[i] = 1;
+  if (dirty[i]) list[atomicAdd(count, 1)] = static_cast<int32_t>(i);
+}
+
+__global__ void k_sc_refresh_rows(gp_csr A, const int32_t* list, int32_t count, const uint8_t* keep_row,
+                                  Bits keep_col, const double2* lu, ScAct* act) {
+  const int64_t k = gtid();
+  if (k >= count) return;
+  const int32_t i = list[k];
+  ScAct r = sc_row_activity(A, i, keep_col, lu);
+  if (!keep_row[i]) r.live = 0;
+  act[i] = r;
+}
+
 // Activity of the support row without column j's own term
 // (_singleton_column_activity_bounds_from_cache, :756-816).
 __device__ bool sc_rest_bounds(const ScAct& act, double a, double lj, double uj, double* rmin, double* rmax) {
@@ -1429,16 +1500,54 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
   GP_LAUNCH_CHECK("singleton columns owner");
-  DVec<ScAct> row_act(m);
+  ScCache* cache = static_cast<ScCache*>(plan.rule_cache);
+  if (!cache || cache->m != m || cache->n != n) {
+    delete cache;
+    cache = new ScCache();
+    plan.rule_cache = cache;
+    cache->m = m;
+    cache->n = n;
+    cache->act.resize(m);
+    cache->keep_row.resize(m);
+    cache->keep_col.resize(n);
+  }
   {
     const PackedMask live_col(plan.keep_col_mask, n);
-    const PackedBounds lu(plan.new_l, plan.new_u, n);
-    // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
-    TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
-    launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, row_act.data()});
-    GP_LAUNCH_CHECK("singleton columns row activity");
+    PackedBounds lu(plan.new_l, plan.new_u, n);
+    bool full = !cache->valid;
+    if (!full) {
+      // rows touched since the last call; few of them -> recompute those on one lane each
+      DVec<uint8_t> dirty(m);
+      DVec<int32_t> list(m);
+      fill_u8(dirty.data(), 0, m);
+      Flags cnt;
+      hipLaunchKernelGGL(k_sc_dirty_cols, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, lu.data(), cache->lu.data(),
+                         plan.keep_col_mask, cache->keep_col.data(), dirty.data());
+      hipLaunchKernelGGL(k_sc_dirty_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, plan.keep_row_mask,
+                         cache->keep_row.data(), dirty.data(), list.data(), cnt.d.data());
+      GP_LAUNCH_CHECK("singleton columns dirty rows");
+      const int32_t count = cnt.read()[0];
+      if (static_cast<int64_t>(count) * 32 > m) {
+        full = true;
+      } else if (count > 0) {
+        hipLaunchKernelGGL(k_sc_refresh_rows, dim3(grid_for(count)), dim3(kBlock), 0, 0, lp.A, list.data(), count,
+                           plan.keep_row_mask, live_col.view(), lu.data(), cache->act.data());
+        GP_LAUNCH_CHECK("singleton columns row refresh");
+      }
+    }
+    if (full) {
+      // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
+      TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
+      launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, cache->act.data()});
+      GP_LAUNCH_CHECK("singleton columns row activity");
+    }
+    std::swap(cache->lu, lu.lu);
+    d2d(cache->keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
+    d2d(cache->keep_col.data(), plan.keep_col_mask, static_cast<size_t>(n));
+    cache->valid = true;
   }
-  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act.data(), plan.keep_row_mask,
+  const ScAct* row_act = cache->act.data();
+  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act, plan.keep_row_mask,
                      plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row,
                      s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
                      p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data());
@@ -1512,6 +1621,8 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   plan.has_col_action = plan.has_col_action || status1;
   plan.has_change = 1;
 }
+
+void rule_cache_free(void* cache) { delete static_cast<ScCache*>(cache); }
 
 // ---------------------------------------------------------------- dual fix
 void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {

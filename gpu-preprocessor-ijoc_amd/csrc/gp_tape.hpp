@@ -53,6 +53,8 @@ void tape_copy_to_host(const DeviceTape& t, int32_t* types, gp_offset* istarts, 
                        uint8_t* modes, int32_t* idx, double* vals);
 
 inline DeviceTape* plan_tape(const gp_plan& plan) { return static_cast<DeviceTape*>(plan.tape); }
+// Releases the rules' per-plan caches (gp_rules2.hip).
+void rule_cache_free(void* cache);
 
 // Rules (gp_rules.hip / gp_rules2.hip).
 void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params& p);

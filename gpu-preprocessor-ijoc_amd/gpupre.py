@@ -132,7 +132,7 @@ class GpPlan(C.Structure):
         ("has_row_action", C.c_int32), ("has_col_action", C.c_int32),
         ("has_redundant_row_deletion", C.c_int32), ("has_row_deletion", C.c_int32),
         ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32), ("has_new_A", C.c_int32),
-        ("new_A", GpCsr), ("tape", C.c_void_p),
+        ("new_A", GpCsr), ("tape", C.c_void_p), ("rule_cache", C.c_void_p),
     ]
 
 

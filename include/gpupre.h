@@ -296,6 +296,7 @@ typedef struct gp_plan {
   int32_t has_new_A;
   gp_csr new_A;   /* replacement matrix built by doubleton substitution */
   void* tape;     /* opaque device tape (local indices); NULL = do not record */
+  void* rule_cache; /* opaque per-plan caches of the rules (library-owned; NULL at creation) */
 } gp_plan;
 
 typedef struct gp_stats {

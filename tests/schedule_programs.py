@@ -54,7 +54,7 @@ class Run:
             self.iterations, self.stopped = s.iterations, s.stopped_at_iteration_limit
             self.rows, self.cols, self.nnz = s.reduced_rows, s.reduced_cols, s.reduced_nnz
             self.reduced = s.reduced()
-            self.trace = [(PHASE[t.phase], t.nnz_before, t.nnz_after, PHASE[t.next_phase]) for t in s.trace]
+            self.trace = [(PHASE[t.phase], t.nnz_before, t.nnz_after, PHASE[t.next_phase]) for t in s.trace()]
             rec = s.record()
             self.types, self.indices, self.vals = rec.types, rec.indices, rec.vals
             self.obj_delta = s.obj_constant_delta
