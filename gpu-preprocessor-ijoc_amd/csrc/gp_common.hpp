@@ -412,14 +412,18 @@ int32_t count_u8(const uint8_t* mask, int32_t n);
 
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
-void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col,
-                 const int32_t* row_old_to_new, const int32_t* col_old_to_new, int32_t m_new,
-                 int32_t n_new, gp_csr& out);
+// Stable compaction by keep masks (null = keep all); row_old_to_new is the
+// exclusive prefix of keep_row (null with keep_row null).  Column indices are
+// renumbered to their rank among the kept columns.
+void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_old_to_new,
+                 int32_t m_new, int32_t n_new, gp_csr& out);
 void csr_pool_free(gp_csr& A);
 
 // Statistics.
 void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
                         int64_t* device_total);
+// Live statistics when every row and column is live (all-ones masks).
+void live_stats_all_live(gp_stats& s, const gp_device_lp& lp, int64_t* device_total);
 // Bring live statistics from the masks (old_rows, old_cols) they were last
 // computed for to the plan's current masks, visiting only deleted lines.
 // `scratch` holds 4 int32.  Returns false (stats then need a full refresh)

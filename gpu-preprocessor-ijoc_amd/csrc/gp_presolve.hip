@@ -188,13 +188,13 @@ void apply_plan(Working& cur, gp_plan& plan, Record& r, bool captured) {
   const int32_t n_new = build_old_to_new(plan.keep_col_mask, n, col_o2n.data());
   const gp_csr& src = plan.has_new_A ? plan.new_A : cur.lp.A;
   gp_device_lp next{};
-  compact_csr(src, plan.keep_row_mask, plan.keep_col_mask, row_o2n.data(), col_o2n.data(), m_new, n_new, next.A);
+  compact_csr(src, plan.keep_row_mask, plan.keep_col_mask, row_o2n.data(), m_new, n_new, next.A);
   // Without a rewritten matrix the transpose of the compacted A is the compacted
   // AT: dropping dead entries keeps each column's increasing row order.
   if (plan.has_new_A)
     transpose_csr(next.A, next.AT);
   else
-    compact_csr(cur.lp.AT, plan.keep_col_mask, plan.keep_row_mask, col_o2n.data(), row_o2n.data(), n_new, m_new,
+    compact_csr(cur.lp.AT, plan.keep_col_mask, plan.keep_row_mask, col_o2n.data(), n_new, m_new,
                 next.AT);
   {
     TimedScope ts(GP_KT_COMPACT, 2.0 * (24.0 * n + 16.0 * m) / 2.0 + 24.0 * n_new + 16.0 * m_new + 4.0 * (m + n));
@@ -236,6 +236,7 @@ struct Deferred {
   DVec<uint8_t> snap_rows, snap_cols, touched_rows, touched_cols;
   DVec<int32_t> inc_scratch;
   bool active = false, pending = false, stats_dirty = false, nnz_dirty = false, have_snapshot = false;
+  bool fresh = false;  // plan masks still all ones
   void reset() {
     if (active) {
       plan_destroy(plan);
@@ -249,7 +250,7 @@ struct Deferred {
       touched_cols.reset();
       inc_scratch.reset();
     }
-    active = pending = stats_dirty = nnz_dirty = have_snapshot = false;
+    active = pending = stats_dirty = nnz_dirty = have_snapshot = fresh = false;
   }
 };
 
@@ -279,7 +280,7 @@ struct Scheduler {
     d.reference_bounds.resize(2 * static_cast<size_t>(n) + 1);
     d.live_total.resize(1);
     d.plan.rule_entry_keep_col = d.entry_cols.data();
-    d.active = d.stats_dirty = d.nnz_dirty = true;
+    d.active = d.stats_dirty = d.nnz_dirty = d.fresh = true;
   }
 
   // Live statistics of the plan's masks: a full sweep for a new plan, then
@@ -292,8 +293,12 @@ struct Scheduler {
       done = update_live_stats(d.stats, cur.lp, d.plan, d.snap_rows.data(), d.snap_cols.data(),
                                d.touched_rows.data(), d.touched_cols.data(), d.live_total.data(),
                                d.inc_scratch.data());
-    if (!done) {
-      refresh_live_stats(d.stats, cur.lp, d.plan, d.live_total.data());
+    if (!done && d.fresh) {
+      live_stats_all_live(d.stats, cur.lp, d.live_total.data());
+      done = true;
+    }
+    if (!done || !d.have_snapshot) {
+      if (!done) refresh_live_stats(d.stats, cur.lp, d.plan, d.live_total.data());
       if (!d.have_snapshot) {
         d.snap_rows.resize(m ? m : 1);
         d.snap_cols.resize(n ? n : 1);
@@ -306,6 +311,7 @@ struct Scheduler {
     d2d(d.snap_rows.data(), d.plan.keep_row_mask, static_cast<size_t>(m));
     d2d(d.snap_cols.data(), d.plan.keep_col_mask, static_cast<size_t>(n));
     d.stats_dirty = false;
+    d.fresh = false;
   }
 
   int64_t live_nnz() {
@@ -800,16 +806,12 @@ int gp_compact_csr(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep
     DVec<int32_t> ro(A->rows ? A->rows : 1), co(A->cols ? A->cols : 1);
     int32_t m_new = A->rows, n_new = A->cols;
     const int32_t* rop = nullptr;
-    const int32_t* cop = nullptr;
     if (keep_row) {
       m_new = build_old_to_new(keep_row, A->rows, ro.data());
       rop = ro.data();
     }
-    if (keep_col) {
-      n_new = build_old_to_new(keep_col, A->cols, co.data());
-      cop = co.data();
-    }
-    compact_csr(*A, keep_row, keep_col, rop, cop, m_new, n_new, *out);
+    if (keep_col) n_new = build_old_to_new(keep_col, A->cols, co.data());
+    compact_csr(*A, keep_row, keep_col, rop, m_new, n_new, *out);
     GP_CHECK(hipDeviceSynchronize());
   });
 }

@@ -452,9 +452,22 @@ __global__ void k_compact_rowptr(const int64_t* excl, const int32_t* row_o2n, in
   const int32_t r = row_o2n ? row_o2n[i] : static_cast<int32_t>(i);
   if (r >= 0) out_rp[r] = static_cast<gp_offset>(excl[i]);
 }
+// New index of column j: its rank among the kept columns, from the packed
+// keep bits and a per-word prefix (both L2-resident), instead of an HBM
+// gather into an n-entry old-to-new map.  -1 for a dropped column.
+struct ColRank {
+  const uint32_t* words;  // null: keep every column, identity map
+  const int32_t* prefix;
+  __device__ int32_t operator()(int32_t j) const {
+    if (!words) return j;
+    const uint32_t w = words[j >> 5], bit = 1u << (j & 31);
+    return (w & bit) ? prefix[j >> 5] + __popc(w & (bit - 1u)) : -1;
+  }
+};
+
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_compact_copy(const gp_offset* rp, const int32_t* col, const double* val,
-                                                          int32_t m, const int32_t* row_o2n, const int32_t* col_o2n,
+                                                          int32_t m, const int32_t* row_o2n, ColRank col_rank,
                                                           const int64_t* excl, int32_t* out_col, double* out_val) {
   const int lane = threadIdx.x % G;
   const int wl = static_cast<int>(__lane_id());
@@ -470,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_copy(const gp_offset* rp, co
       int32_t j = -1;
       double v = 0.0;
       if (p < p1) {
-        j = col_o2n ? col_o2n[col[p]] : col[p];
+        j = col_rank(col[p]);
         v = val[p];
       }
       const unsigned long long keep = __ballot(j >= 0) & group_bits;
@@ -484,18 +497,30 @@ __global__ void __launch_bounds__(kBlock) k_compact_copy(const gp_offset* rp, co
   }
 }
 __global__ void k_set_last(gp_offset* rp, int32_t at, gp_offset v) { rp[at] = v; }
+__global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w < nw) out[w] = __popc(words[w]);
+}
 }  // namespace
 
-void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col,
-                 const int32_t* row_o2n, const int32_t* col_o2n, int32_t m_new, int32_t n_new,
-                 gp_csr& out) {
+void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_o2n,
+                 int32_t m_new, int32_t n_new, gp_csr& out) {
   const int32_t m = A.rows;
   DVec<int64_t> cnt(static_cast<size_t>(m) + 1);
   int64_t total = 0;
+  const PackedMask cols(keep_col, A.cols);
+  // column ranks: exclusive prefix of the kept bits per 32-column word
+  const int64_t nw = (static_cast<int64_t>(A.cols) + 31) / 32;
+  DVec<int32_t> word_prefix(static_cast<size_t>(nw > 0 ? nw : 1));
+  if (keep_col && nw > 0) {
+    hipLaunchKernelGGL(k_word_popc, dim3(grid_for(nw)), dim3(kBlock), 0, 0, cols.words.data(), nw, word_prefix.data());
+    GP_LAUNCH_CHECK("compact word counts");
+    exclusive_scan_i32_total(word_prefix.data(), nw);
+  }
+  const ColRank rank{keep_col ? cols.words.data() : nullptr, word_prefix.data()};
   {
     TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
     if (m > 0) {
-      const PackedMask cols(keep_col, A.cols);
       const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
       const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
       switch (lanes) {
@@ -523,11 +548,11 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
     const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
     const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
     switch (lanes) {
-      case 2: hipLaunchKernelGGL(k_compact_copy<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
-      case 4: hipLaunchKernelGGL(k_compact_copy<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
-      case 8: hipLaunchKernelGGL(k_compact_copy<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
-      case 16: hipLaunchKernelGGL(k_compact_copy<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
-      default: hipLaunchKernelGGL(k_compact_copy<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, col_o2n, cnt.data(), out.col_idx, out.vals); break;
+      case 2: hipLaunchKernelGGL(k_compact_copy<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
+      case 4: hipLaunchKernelGGL(k_compact_copy<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
+      case 8: hipLaunchKernelGGL(k_compact_copy<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
+      case 16: hipLaunchKernelGGL(k_compact_copy<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
+      default: hipLaunchKernelGGL(k_compact_copy<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
     }
     GP_LAUNCH_CHECK("compact copy");
   }
@@ -648,6 +673,39 @@ void refresh_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan
                     s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val, nullptr);
     GP_LAUNCH_CHECK("live stats cols");
   }
+  s.live_masks = 1;
+}
+
+// ---- live statistics of a fresh plan ------------------------------------------------------
+// With every row and column live (a plan just created over a materialized
+// LP) the live statistics are the stored row / column lengths; no mask is
+// read and no entry is visited except the one entry of a singleton.
+namespace {
+__global__ void k_stats_from_lengths(gp_csr M, int32_t* degree, uint8_t* empty, uint8_t* singleton, int32_t* support,
+                                     double* coefficient) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= M.rows) return;
+  const gp_offset p0 = M.row_ptr[r];
+  const int32_t len = static_cast<int32_t>(M.row_ptr[r + 1] - p0);
+  degree[r] = len;
+  empty[r] = len == 0;
+  singleton[r] = len == 1;
+  support[r] = len == 1 ? M.col_idx[p0] : -1;
+  coefficient[r] = len == 1 ? M.vals[p0] : 0.0;
+}
+__global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+}  // namespace
+
+void live_stats_all_live(gp_stats& s, const gp_device_lp& lp, int64_t* device_total) {
+  TimedScope ts(GP_KT_LIVE_STATS, 4.0 * (lp.A.rows + 1) + 4.0 * (lp.AT.rows + 1) + 18.0 * (lp.A.rows + lp.A.cols));
+  if (lp.A.rows > 0)
+    hipLaunchKernelGGL(k_stats_from_lengths, dim3(grid_for(lp.A.rows)), dim3(kBlock), 0, 0, lp.A, s.row_nnz,
+                       s.empty_row_mask, s.singleton_row_mask, s.singleton_row_col, s.singleton_row_val);
+  if (lp.AT.rows > 0)
+    hipLaunchKernelGGL(k_stats_from_lengths, dim3(grid_for(lp.AT.rows)), dim3(kBlock), 0, 0, lp.AT, s.col_nnz,
+                       s.empty_col_mask, s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val);
+  if (device_total) hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(1), 0, 0, device_total, static_cast<int64_t>(lp.A.nnz));
+  GP_LAUNCH_CHECK("live stats from lengths");
   s.live_masks = 1;
 }
 

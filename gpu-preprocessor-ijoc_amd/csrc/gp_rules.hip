@@ -616,6 +616,39 @@ __global__ void __launch_bounds__(kBlock) k_col_max_abs(gp_csr AT, Bits keep_row
   }
 }
 
+// Column max |a| kept with the plan: after the first sweep, only columns
+// with an entry in a row whose liveness changed since the last call are
+// recomputed (a row deletion can only lower a column's maximum).
+struct ColMaxCache {
+  int32_t m = -1, n = -1;
+  bool valid = false, masked = false;
+  DVec<double> vmax;
+  DVec<uint8_t> keep_row;
+};
+
+__global__ void k_cm_dirty(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_row_old, uint8_t* dirty) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= A.rows || keep_row[i] == keep_row_old[i]) return;
+  for (gp_offset p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) dirty[A.col_idx[p]] = 1;
+}
+
+__global__ void k_cm_list(int32_t n, const uint8_t* dirty, int32_t* list, int32_t* count) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n && dirty[j]) list[atomicAdd(count, 1)] = static_cast<int32_t>(j);
+}
+
+__global__ void k_cm_refresh(gp_csr AT, Bits keep_row, const int32_t* list, int32_t count, double* vmax) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int32_t j = list[k];
+  double v = 0.0;
+  for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) {
+    if (!keep_row[AT.col_idx[p]]) continue;
+    v = cmax(v, fabs(AT.vals[p]));
+  }
+  vmax[j] = v;
+}
+
 template <template <int> class K, class... Args>
 void launch_grouped(int lanes, int64_t rows, Args... args) {
   const unsigned grid = grid_capped(rows * lanes, kBlock, 16384);
@@ -829,7 +862,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   const uint8_t* keep_col = s.live_masks ? plan.keep_col_mask : nullptr;
   DVec<double> min_fin(m), max_fin(m);
   DVec<int32_t> min_ninf(m), max_pinf(m);
-  DVec<double> cl(n), cu(n), fl(n), fu(n), vmax(n), fval(n);
+  DVec<double> cl(n), cu(n), fl(n), fu(n), fval(n);
   DVec<uint8_t> lch(n), uch(n), fixed(n);
   d2d(cl.data(), plan.new_l, n);
   d2d(cu.data(), plan.new_u, n);
@@ -853,18 +886,52 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   const PackedMask rows_all(plan.keep_row_mask, m);
   {
     TimedScope outer(GP_KT_RULES_OTHER, 0.0);
-    const PackedMask live_rows(s.live_masks ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
-    {
+    const bool masked = s.live_masks != 0;
+    const PackedMask live_rows(masked ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
+    RuleCaches& rc = rule_caches(plan);
+    if (!rc.col_max) rc.col_max = new ColMaxCache();
+    ColMaxCache& cm = *static_cast<ColMaxCache*>(rc.col_max);
+    if (cm.m != m || cm.n != n || cm.masked != masked) {
+      cm.valid = false;
+      cm.m = m;
+      cm.n = n;
+      cm.masked = masked;
+      cm.vmax.resize(n);
+      cm.keep_row.resize(m);
+    }
+    bool full = !cm.valid;
+    if (!full && masked) {
+      DVec<uint8_t> dirty(n);
+      DVec<int32_t> list(n);
+      fill_u8(dirty.data(), 0, n);
+      Flags cnt;
+      hipLaunchKernelGGL(k_cm_dirty, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask,
+                         cm.keep_row.data(), dirty.data());
+      hipLaunchKernelGGL(k_cm_list, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, dirty.data(), list.data(), cnt.d.data());
+      GP_LAUNCH_CHECK("col max dirty");
+      const int32_t count = cnt.read()[0];
+      if (static_cast<int64_t>(count) * 16 > n) {
+        full = true;
+      } else if (count > 0) {
+        hipLaunchKernelGGL(k_cm_refresh, dim3(grid_for(count)), dim3(kBlock), 0, 0, lp.AT, live_rows.view(),
+                           list.data(), count, cm.vmax.data());
+        GP_LAUNCH_CHECK("col max refresh");
+      }
+    }
+    if (full) {
       // AT row_ptr, col_idx, vals, a mask bit per row, 8 bytes out per column
       TimedScope ts(GP_KT_COL_MAX, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + m / 8.0 + 8.0 * n);
       launch_grouped<ColMaxAbsK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
-                                 live_rows.view(), vmax.data());
+                                 live_rows.view(), cm.vmax.data());
       GP_LAUNCH_CHECK("col max abs");
     }
+    if (masked) d2d(cm.keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
+    cm.valid = true;
+    const double* vmax_d = cm.vmax.data();
     const double* ref_l = plan.propagation_reference_l ? plan.propagation_reference_l : (lp.l ? lp.l : plan.new_l);
     const double* ref_u = plan.propagation_reference_u ? plan.propagation_reference_u : (lp.u ? lp.u : plan.new_u);
     hipLaunchKernelGGL(k_bs_finalize, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask, plan.new_l,
-                       plan.new_u, cl.data(), cu.data(), ref_l, ref_u, plan.new_c, vmax.data(),
+                       plan.new_u, cl.data(), cu.data(), ref_l, ref_u, plan.new_c, vmax_d,
                        p.bound_strengthening_preserve_free_zero_cost ? 1 : 0, p.zero_tol, p.feasibility_tol,
                        p.bound_strengthening_upper_relaxation_factor, fl.data(), fu.data(), lch.data(), uch.data(),
                        fixed.data(), fval.data(), flags.d.data());
@@ -921,6 +988,21 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   plan.has_row_action = plan.has_row_action || f2;
   plan.has_col_action = plan.has_col_action || f2;
   plan.has_change = 1;
+}
+
+RuleCaches& rule_caches(gp_plan& plan) {
+  if (!plan.rule_cache) plan.rule_cache = new RuleCaches();
+  return *static_cast<RuleCaches*>(plan.rule_cache);
+}
+
+void col_max_cache_free(void* cache) { delete static_cast<ColMaxCache*>(cache); }
+
+void rule_cache_free(void* cache) {
+  if (!cache) return;
+  RuleCaches* rc = static_cast<RuleCaches*>(cache);
+  sc_cache_free(rc->sc);
+  col_max_cache_free(rc->col_max);
+  delete rc;
 }
 
 void apply_rule(int32_t rule, gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {

@@ -1500,11 +1500,12 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
   GP_LAUNCH_CHECK("singleton columns owner");
-  ScCache* cache = static_cast<ScCache*>(plan.rule_cache);
+  RuleCaches& rc = rule_caches(plan);
+  ScCache* cache = static_cast<ScCache*>(rc.sc);
   if (!cache || cache->m != m || cache->n != n) {
     delete cache;
     cache = new ScCache();
-    plan.rule_cache = cache;
+    rc.sc = cache;
     cache->m = m;
     cache->n = n;
     cache->act.resize(m);
@@ -1622,7 +1623,7 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   plan.has_change = 1;
 }
 
-void rule_cache_free(void* cache) { delete static_cast<ScCache*>(cache); }
+void sc_cache_free(void* cache) { delete static_cast<ScCache*>(cache); }
 
 // ---------------------------------------------------------------- dual fix
 void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {

@@ -53,8 +53,16 @@ void tape_copy_to_host(const DeviceTape& t, int32_t* types, gp_offset* istarts, 
                        uint8_t* modes, int32_t* idx, double* vals);
 
 inline DeviceTape* plan_tape(const gp_plan& plan) { return static_cast<DeviceTape*>(plan.tape); }
-// Releases the rules' per-plan caches (gp_rules2.hip).
+// Per-plan caches of the rules, hung off gp_plan::rule_cache.  Each rule
+// source file owns the type of its slot.
+struct RuleCaches {
+  void* sc = nullptr;       // singleton-column row activities (gp_rules2.hip)
+  void* col_max = nullptr;  // column max |a| (gp_rules.hip)
+};
+RuleCaches& rule_caches(gp_plan& plan);  // created on first use
 void rule_cache_free(void* cache);
+void sc_cache_free(void* cache);         // gp_rules2.hip
+void col_max_cache_free(void* cache);    // gp_rules.hip
 
 // Rules (gp_rules.hip / gp_rules2.hip).
 void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params& p);
