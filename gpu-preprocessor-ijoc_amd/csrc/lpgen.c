@@ -473,6 +473,133 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
   return 0;
 }
 
+/* ---- Adams-Johnson QAP LP ------------------------------------------------- */
+/* lpgen_qap restates the LP that scripts/generate_qap20.jl:25-109 writes as MPS
+ * (same row order, column order, signs, sides and free bounds):
+ *   rows [0, n^4)                 -y_q <= 0                       (explicit bound rows)
+ *   rows n^4 + [0, n)             sum_j x[i,j] = 1
+ *   rows n^4 + n + [0, n)         sum_i x[i,j] = 1
+ *   rows n^4 + 2n + idx3(t,i,j)   sum_a y[a,t,i,j] - x[i,j] = 0
+ *   rows ... + n^3 + idx3(a,b,c)  sum_b' y[a,b',b,c] - x[b,c] = 0
+ *   rows ... + 2n^3 + q           y_q - y_rq = 0 at min(q,rq), its negation at
+ *                                 max(q,rq); empty (= 0) when q == rq
+ *   rows ... + n^4 + idx2(i,j)    0 <= x[i,j] <= 1                (L row with range 1)
+ * with idx2/idx3/idx4 the column-major indices of the script (0-based here),
+ * columns x[i,j] at idx2(i,j) and y[i,j,k,l] at n^2 + idx4(i,j,k,l), all free,
+ * and cost flow[i][k] * distance[j][l] on y[i,j,k,l].  flow and distance are
+ * seeded integers U{0..100} with zero diagonals; each off-diagonal flow is
+ * zero with probability 1/2.  x0 = 1/n, y0 = 1/n^2 is a feasible point. */
+int lpgen_qap(int32_t nq, uint64_t seed, lpgen_lp* out) {
+  memset(out, 0, sizeof(*out));
+  if (nq < 2 || nq > 40) return 1;
+  const int64_t n = nq, n2 = n * n, n3 = n2 * n, n4 = n3 * n;
+  const int64_t m64 = 2 * n4 + 2 * n3 + n2 + 2 * n, ncols64 = n4 + n2;
+  const int64_t nnz64 = n2 * (2 + 2 * n + 1) + 3 * n4 + 2 * (n4 - n2);
+  if (m64 > INT32_MAX || nnz64 > INT32_MAX) return 1;
+  rng r = {seed * 0x2545f4914f6cdd1dULL + 0x51ed270b27a1f3c5ULL};
+  int32_t* flow = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)n2);
+  int32_t* dist = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)n2);
+  for (int64_t a = 0; a < n; ++a)
+    for (int64_t b = 0; b < n; ++b) {
+      const int zero_flow = unif(&r) < 0.5;
+      const int32_t f = (int32_t)below(&r, 101), d = (int32_t)below(&r, 101);
+      flow[a * n + b] = (a == b || zero_flow) ? 0 : f;
+      dist[a * n + b] = a == b ? 0 : d;
+    }
+  const int32_t m = (int32_t)m64, ncols = (int32_t)ncols64;
+  const int64_t eq0 = n4, sym0 = n4 + 2 * n + 2 * n3, rng0 = n4 + 2 * n + 2 * n3 + n4;
+  /* column-major entry lists, turned into CSR by a counting pass over rows */
+  int32_t* cnt = (int32_t*)calloc((size_t)m + 1, sizeof(int32_t));
+  int32_t* crow = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)nnz64);
+  double* cval = (double*)xmalloc(sizeof(double) * (size_t)nnz64);
+  int32_t* cptr = (int32_t*)xmalloc(sizeof(int32_t) * ((size_t)ncols + 1));
+  int64_t e = 0;
+#define QAP_PUT(row, v) (crow[e] = (int32_t)(row), cval[e] = (v), cnt[(row) + 1]++, ++e)
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < n; ++i) {
+      cptr[i + n * j] = (int32_t)e;
+      QAP_PUT(eq0 + i, 1.0);
+      QAP_PUT(eq0 + n + j, 1.0);
+      for (int64_t t = 0; t < n; ++t) QAP_PUT(eq0 + 2 * n + (t + n * i + n2 * j), -1.0);
+      for (int64_t t = 0; t < n; ++t) QAP_PUT(eq0 + 2 * n + n3 + (t + n * i + n2 * j), -1.0);
+      QAP_PUT(rng0 + (i + n * j), 1.0);
+    }
+  double* c = (double*)calloc((size_t)ncols, sizeof(double));
+  for (int64_t l = 0; l < n; ++l)
+    for (int64_t k = 0; k < n; ++k)
+      for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+          const int64_t q = i + n * j + n2 * k + n3 * l, rq = k + n * l + n2 * i + n3 * j;
+          cptr[n2 + q] = (int32_t)e;
+          QAP_PUT(q, -1.0);
+          QAP_PUT(eq0 + 2 * n + (j + n * k + n2 * l), 1.0);
+          QAP_PUT(eq0 + 2 * n + n3 + (i + n * k + n2 * l), 1.0);
+          if (q != rq) {
+            const int64_t first = q < rq ? q : rq, second = q < rq ? rq : q;
+            const double sg = q < rq ? 1.0 : -1.0;
+            QAP_PUT(sym0 + first, sg);
+            QAP_PUT(sym0 + second, -sg);
+          }
+          c[n2 + q] = (double)flow[i * n + k] * (double)dist[j * n + l];
+        }
+#undef QAP_PUT
+  cptr[ncols] = (int32_t)e;
+  for (int32_t i = 0; i < m; ++i) cnt[i + 1] += cnt[i];
+  int32_t* row_ptr = (int32_t*)xmalloc(sizeof(int32_t) * ((size_t)m + 1));
+  memcpy(row_ptr, cnt, sizeof(int32_t) * ((size_t)m + 1));
+  int32_t* col_idx = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)e);
+  double* vals = (double*)xmalloc(sizeof(double) * (size_t)e);
+  for (int32_t col = 0; col < ncols; ++col) /* columns in order: rows come out sorted */
+    for (int32_t p = cptr[col]; p < cptr[col + 1]; ++p) {
+      const int32_t pos = cnt[crow[p]]++;
+      col_idx[pos] = col;
+      vals[pos] = cval[p];
+    }
+  double* AL = (double*)xmalloc(sizeof(double) * (size_t)m);
+  double* AU = (double*)xmalloc(sizeof(double) * (size_t)m);
+  for (int64_t i = 0; i < m; ++i) {
+    if (i < eq0) {
+      AL[i] = -INFINITY;
+      AU[i] = 0.0;
+    } else if (i < rng0) {
+      AL[i] = AU[i] = (i < eq0 + 2 * n) ? 1.0 : 0.0;
+    } else {
+      AL[i] = 0.0;
+      AU[i] = 1.0;
+    }
+  }
+  double* l = (double*)xmalloc(sizeof(double) * (size_t)ncols);
+  double* u = (double*)xmalloc(sizeof(double) * (size_t)ncols);
+  double* x0 = (double*)xmalloc(sizeof(double) * (size_t)ncols);
+  for (int64_t j = 0; j < ncols; ++j) {
+    l[j] = -INFINITY;
+    u[j] = INFINITY;
+    x0[j] = j < n2 ? 1.0 / (double)n : 1.0 / (double)n2;
+  }
+  out->m = m;
+  out->n = ncols;
+  out->nnz = (int32_t)e;
+  out->row_ptr = row_ptr;
+  out->col_idx = col_idx;
+  out->vals = vals;
+  out->c = c;
+  out->l = l;
+  out->u = u;
+  out->AL = AL;
+  out->AU = AU;
+  out->x0 = x0;
+  out->y0 = (double*)calloc((size_t)m, sizeof(double));
+  out->obj_constant = 0.0;
+  out->zipf_scale = 0.0;
+  free(flow);
+  free(dist);
+  free(cnt);
+  free(crow);
+  free(cval);
+  free(cptr);
+  return 0;
+}
+
 void lpgen_free(lpgen_lp* lp) {
   free(lp->row_ptr);
   free(lp->col_idx);

@@ -44,6 +44,8 @@ typedef struct lpgen_lp {
 
 void lpgen_default_spec(lpgen_spec* spec, int32_t m, int32_t n, uint64_t seed);
 int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out);
+/* Adams-Johnson LP linearisation of a seeded QAP of size n (2..40). */
+int lpgen_qap(int32_t n, uint64_t seed, lpgen_lp* out);
 void lpgen_free(lpgen_lp* lp);
 
 #ifdef __cplusplus

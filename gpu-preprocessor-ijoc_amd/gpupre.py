@@ -482,6 +482,8 @@ def _load_lpgen():
         g.lpgen_generate.restype = C.c_int
         g.lpgen_free.argtypes = [C.POINTER(LpgenLp)]
         g.lpgen_free.restype = None
+        g.lpgen_qap.argtypes = [C.c_int32, C.c_uint64, C.POINTER(LpgenLp)]
+        g.lpgen_qap.restype = C.c_int
         _lpgen = g
     return _lpgen
 
@@ -515,7 +517,20 @@ def generate(m: int, n: int, seed: int = 0, **spec_overrides) -> HostLP:
     raw = LpgenLp()
     if g.lpgen_generate(C.byref(spec), C.byref(raw)) != 0:
         raise RuntimeError("lpgen_generate failed")
+    return _take_lpgen(g, raw)
 
+
+def generate_qap(n: int, seed: int = 0) -> HostLP:
+    """Adams-Johnson LP linearisation of a seeded n x n QAP (rows 2n^4+2n^3+n^2+2n,
+    columns n^4+n^2), in the row and column order of scripts/generate_qap20.jl:25-109."""
+    g = _load_lpgen()
+    raw = LpgenLp()
+    if g.lpgen_qap(int(n), int(seed), C.byref(raw)) != 0:
+        raise RuntimeError(f"lpgen_qap failed for n={n}")
+    return _take_lpgen(g, raw)
+
+
+def _take_lpgen(g, raw) -> HostLP:
     def arr(p, k, dt):
         if k == 0:
             return np.empty(0, dtype=dt)
