@@ -438,6 +438,20 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf);
 
+// ---- learned pass scheduler (gp_learned.hip) ---------------------------------------------
+int rule_group(int32_t rule);
+void schedule_features(const gp_round_trace* hist, int32_t done, int32_t phase, int64_t nnz_now, int64_t nnz0,
+                       int32_t m0, int32_t n0, int32_t bound_only, int32_t bound_limit, float* features);
+// The network on the device plus the per-group logit thresholds; decide()
+// returns the mask of rule groups (bit GP_GROUP_*) to run in the next round.
+struct LearnedGate {
+  DVec<float> params, io;
+  float threshold[GP_MLP_OUT] = {};
+  bool ready = false;
+  void init(const float* blob);
+  uint32_t decide(const float* features, float* logits = nullptr);
+};
+
 // Lanes per row for a group-per-row sweep of rows with this mean length.
 inline int lanes_for_mean(double mean) {
   if (mean <= 3.0) return 2;

@@ -432,6 +432,16 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
         if (two > limit) limit = two;
       }
     }
+    // Learned pass scheduler: before each round the network scores the rule
+    // groups and only groups at or above their threshold run.  When the gated
+    // loop stops, the plain schedule runs on from that state (fixpoint check
+    // and fallback) until its own stop.
+    const bool learned = p.learned_schedule != 0;
+    LearnedGate gate;
+    if (learned) gate.init(p.scheduler_weights);
+    bool gating = learned;
+    std::vector<gp_round_trace> hist;
+    const int32_t m0 = lp.A.rows, n0 = lp.A.cols;
     while (p.max_iters > 0 && !s.terminal && !s.time_exceeded()) {
       if (phase != GP_PHASE_CLEANUP) {
         if (out.iterations >= p.max_iters) {
@@ -445,23 +455,44 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
       gp_round_trace tr{};
       tr.phase = phase;
       tr.nnz_before = before;
+      tr.bound_only_rounds = bound_only_rounds;
+      tr.bound_only_limit = limit;
+      tr.final_sweep = learned && !gating;
+      uint32_t run = ~0u;
+      if (gating) {
+        float f[GP_MLP_IN];
+        schedule_features(hist.data(), static_cast<int32_t>(hist.size()), phase, before, initial_live, m0, n0,
+                          bound_only_rounds, limit, f);
+        run = gate.decide(f);
+      }
+      // rule enabled by the parameters and its group chosen for this round
+      auto on = [&](int32_t enabled, int32_t rule) -> bool {
+        if (!enabled) return false;
+        if ((run >> rule_group(rule)) & 1u) return true;
+        tr.skipped_mask |= 1 << rule;
+        return false;
+      };
       s.fired = 0;
       int64_t after = before;
       bool stop = false;
       if (phase == GP_PHASE_CLEANUP) {
         const uint64_t serial = s.deletion_serial;
-        bool changed = s.deferred_rule(p.enable_infeasible_and_fixed_variables, GP_RULE_FIXED_VARIABLES);
-        changed = s.deferred_rule(p.enable_empty_columns, GP_RULE_EMPTY_COLUMNS) || changed;
-        changed = s.deferred_rule(p.enable_empty_rows, GP_RULE_EMPTY_ROWS) || changed;
-        changed = s.deferred_rule(p.enable_singleton_rows, GP_RULE_SINGLETON_ROWS) || changed;
-        changed = s.deferred_rule(p.enable_singleton_columns, GP_RULE_SINGLETON_COLUMNS) || changed;
+        bool changed = s.deferred_rule(on(p.enable_infeasible_and_fixed_variables, GP_RULE_FIXED_VARIABLES),
+                                       GP_RULE_FIXED_VARIABLES);
+        changed = s.deferred_rule(on(p.enable_empty_columns, GP_RULE_EMPTY_COLUMNS), GP_RULE_EMPTY_COLUMNS) || changed;
+        changed = s.deferred_rule(on(p.enable_empty_rows, GP_RULE_EMPTY_ROWS), GP_RULE_EMPTY_ROWS) || changed;
+        changed = s.deferred_rule(on(p.enable_singleton_rows, GP_RULE_SINGLETON_ROWS), GP_RULE_SINGLETON_ROWS) ||
+                  changed;
+        changed = s.deferred_rule(on(p.enable_singleton_columns, GP_RULE_SINGLETON_COLUMNS),
+                                  GP_RULE_SINGLETON_COLUMNS) ||
+                  changed;
         if (s.deletion_serial != serial) bound_only_rounds = 0;
         after = s.live_nnz();
         phase = changed ? GP_PHASE_CLEANUP : GP_PHASE_PROPAGATION;
       } else if (phase == GP_PHASE_PROPAGATION) {
         const uint64_t serial = s.deletion_serial;
         bool bounds_changed = false;
-        if (p.enable_bound_strengthening && !s.terminal && !s.time_exceeded()) {
+        if (on(p.enable_bound_strengthening, GP_RULE_BOUND_STRENGTHENING) && !s.terminal && !s.time_exceeded()) {
           s.prepare_deferred();
           const int32_t n = s.cur.lp.A.cols;
           if (n) {
@@ -472,8 +503,8 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
           }
           bounds_changed = s.deferred_rule(true, GP_RULE_BOUND_STRENGTHENING);
         }
-        s.deferred_rule(p.enable_infeasible_and_redundant_rows, GP_RULE_REDUNDANT_ROWS);
-        s.deferred_rule(p.enable_dual_fix, GP_RULE_DUAL_FIX);
+        s.deferred_rule(on(p.enable_infeasible_and_redundant_rows, GP_RULE_REDUNDANT_ROWS), GP_RULE_REDUNDANT_ROWS);
+        s.deferred_rule(on(p.enable_dual_fix, GP_RULE_DUAL_FIX), GP_RULE_DUAL_FIX);
         const bool deleted = s.deletion_serial != serial;
         bound_only_rounds = deleted ? 0 : bound_only_rounds + 1;
         s.materialize();
@@ -483,28 +514,33 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
         phase = nnz_progress || bound_progress ? GP_PHASE_CLEANUP : GP_PHASE_AGGREGATION;
         if (phase == GP_PHASE_AGGREGATION) bound_only_rounds = 0;
       } else if (phase == GP_PHASE_AGGREGATION) {
-        s.materialized_rule(p.enable_doubleton_equations, GP_RULE_DOUBLETON_EQUATIONS);
+        s.materialized_rule(on(p.enable_doubleton_equations, GP_RULE_DOUBLETON_EQUATIONS),
+                            GP_RULE_DOUBLETON_EQUATIONS);
         after = s.live_nnz();
         phase = good_progress(before, after, min_dec) ? GP_PHASE_CLEANUP : GP_PHASE_PARALLEL;
       } else {
-        bool changed = s.deferred_rule(p.enable_parallel_rows, GP_RULE_PARALLEL_ROWS);
-        changed = s.deferred_rule(p.enable_parallel_columns, GP_RULE_PARALLEL_COLUMNS) || changed;
+        bool changed = s.deferred_rule(on(p.enable_parallel_rows, GP_RULE_PARALLEL_ROWS), GP_RULE_PARALLEL_ROWS);
+        changed = s.deferred_rule(on(p.enable_parallel_columns, GP_RULE_PARALLEL_COLUMNS), GP_RULE_PARALLEL_COLUMNS) ||
+                  changed;
         s.materialize();
         after = s.live_nnz();
         const bool cont = changed && good_progress(before, after, min_dec);
         if (!cont) stop = true;
         else phase = GP_PHASE_CLEANUP;
       }
-      if (p.collect_trace) {
-        tr.next_phase = stop ? GP_PHASE_END : phase;
-        tr.nnz_after = after;
-        tr.fired_mask = s.fired;
-        tr.rows_live = s.cur.lp.A.rows;
-        tr.cols_live = s.cur.lp.A.cols;
-        tr.seconds = now_s() - t0;
-        rec->trace.push_back(tr);
+      tr.next_phase = stop ? GP_PHASE_END : phase;
+      tr.nnz_after = after;
+      tr.fired_mask = s.fired;
+      tr.rows_live = s.cur.lp.A.rows;
+      tr.cols_live = s.cur.lp.A.cols;
+      tr.seconds = now_s() - t0;
+      if (learned) hist.push_back(tr);
+      if (p.collect_trace) rec->trace.push_back(tr);
+      if (stop) {
+        if (!gating) break;
+        gating = false;  // the plain schedule takes over from here
+        phase = GP_PHASE_CLEANUP;
       }
-      if (stop) break;
     }
     s.materialize();
     GP_CHECK(hipDeviceSynchronize());

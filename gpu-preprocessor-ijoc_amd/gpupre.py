@@ -94,10 +94,17 @@ class GpParams(C.Structure):
 class GpRoundTrace(C.Structure):
     _fields_ = [("phase", C.c_int32), ("next_phase", C.c_int32), ("nnz_before", C.c_int64),
                 ("nnz_after", C.c_int64), ("rows_live", C.c_int32), ("cols_live", C.c_int32),
-                ("fired_mask", C.c_int32), ("skipped_mask", C.c_int32), ("seconds", C.c_double)]
+                ("fired_mask", C.c_int32), ("skipped_mask", C.c_int32), ("seconds", C.c_double),
+                ("bound_only_rounds", C.c_int32), ("bound_only_limit", C.c_int32), ("final_sweep", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 GP_KT_COUNT = 11
+
+# learned pass scheduler network (gpupre.h GP_MLP_*)
+GP_MLP_IN, GP_MLP_H1, GP_MLP_H2, GP_MLP_OUT = 32, 64, 64, 7
+GP_MLP_PARAMS = GP_MLP_H1 * GP_MLP_IN + GP_MLP_H1 + GP_MLP_H2 * GP_MLP_H1 + GP_MLP_H2 + GP_MLP_OUT * GP_MLP_H2 + GP_MLP_OUT
+GP_MLP_BLOB = GP_MLP_PARAMS + GP_MLP_OUT
 
 
 class GpSummary(C.Structure):
@@ -253,6 +260,12 @@ def load() -> C.CDLL:
         "gp_lp_upload": (C.c_int, [C.c_int32, C.c_int32, OFFSET, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                     C.POINTER(GpDeviceLp)]),
         "gp_lp_free": (None, [C.POINTER(GpDeviceLp)]),
+        "gp_rule_group": (C.c_int, [C.c_int32]),
+        "gp_schedule_features": (None, [C.POINTER(GpRoundTrace), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
+                                        C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
+        "gp_mlp_forward": (C.c_int, [vp, vp, C.c_int32, vp]),
+        "gp_mlp_loss_grad": (C.c_int, [vp, vp, vp, vp, C.c_int32, vp, vp]),
+        "gp_mlp_adam_step": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -424,7 +437,7 @@ class Summary:
         return device_lp_to_host(self.s.reduced_lp)
 
     def trace(self):
-        return [self.s.trace[i] for i in range(self.s.trace_count)]
+        return [GpRoundTrace.from_buffer_copy(self.s.trace[i]) for i in range(self.s.trace_count)]
 
     def kernel_stats(self):
         return {name: dict(ms=self.s.kernel_ms[k], launches=int(self.s.kernel_launches[k]),

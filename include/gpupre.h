@@ -164,8 +164,8 @@ typedef struct gp_params {
   int64_t bound_strengthening_bound_only_nnz_round_budget;
   /* Engine extensions (no reference counterpart). */
   int32_t collect_trace;     /* record one gp_round_trace per scheduler step    */
-  int32_t learned_schedule;  /* 1: consult the learned pass scheduler           */
-  const float* scheduler_weights; /* host MLP weights when learned_schedule=1    */
+  int32_t learned_schedule;  /* 1: gate rule groups with the learned pass scheduler */
+  const float* scheduler_weights; /* host blob of GP_MLP_BLOB floats (learned_schedule=1) */
   int32_t kernel_timing;     /* 1: time every kernel family with HIP events     */
   int32_t incremental_live_stats; /* 1 (default): update live statistics from the
                                      deleted lines instead of re-sweeping A and AT
@@ -185,6 +185,10 @@ typedef struct gp_round_trace {
   int32_t fired_mask;    /* bit r set when rule r changed the plan */
   int32_t skipped_mask;  /* bit r set when the learned scheduler skipped rule r */
   double seconds;
+  int32_t bound_only_rounds; /* bound-only propagation rounds in a row, at round start */
+  int32_t bound_only_limit;  /* the run's bound-only round limit                      */
+  int32_t final_sweep;       /* 1: round of the plain fixpoint sweep after a learned phase */
+  int32_t reserved;
 } gp_round_trace;
 
 /* Per-kernel-family device time accumulated when params.kernel_timing=1. */
@@ -201,6 +205,33 @@ enum {
   GP_KT_COL_MAX = 9,          /* column max |a|                                */
   GP_KT_LIVE_STATS_INC = 10,  /* live statistics from deleted lines            */
   GP_KT_COUNT = 11
+};
+
+/* ---- learned pass scheduler (no reference counterpart) --------------------
+ * An MLP 32-64-64-7 in FP32 (ReLU hidden layers, FP32 accumulation) reads the
+ * per-round features of gp_schedule_features and scores the 7 rule groups
+ * below; a group runs in a round only when its logit reaches the group's
+ * threshold.  When the gated loop stops, the plain schedule runs on from that
+ * state as the fixpoint check and fallback.  Parameter blob (FP32, row-major
+ * out x in): W1[64][32] b1[64] W2[64][64] b2[64] W3[7][64] b3[7], then the 7
+ * logit thresholds. */
+#define GP_MLP_IN 32
+#define GP_MLP_H1 64
+#define GP_MLP_H2 64
+#define GP_MLP_OUT 7
+#define GP_MLP_PARAMS (GP_MLP_H1 * GP_MLP_IN + GP_MLP_H1 + GP_MLP_H2 * GP_MLP_H1 + GP_MLP_H2 + \
+                       GP_MLP_OUT * GP_MLP_H2 + GP_MLP_OUT)
+#define GP_MLP_BLOB (GP_MLP_PARAMS + GP_MLP_OUT)
+
+/* Rule groups scored by the network. */
+enum {
+  GP_GROUP_FIXED_EMPTY = 0,      /* fixed variables, empty columns, empty rows */
+  GP_GROUP_SINGLETON_ROWS = 1,
+  GP_GROUP_SINGLETON_COLUMNS = 2,
+  GP_GROUP_BOUNDS_REDUNDANT = 3, /* bound strengthening, redundant rows        */
+  GP_GROUP_DUAL_FIX = 4,
+  GP_GROUP_DOUBLETON = 5,
+  GP_GROUP_PARALLEL = 6          /* parallel rows, parallel columns            */
 };
 
 /* FoldingPipelineSummary + GpuPresolveSummary, flattened.  Value semantics:
@@ -336,6 +367,29 @@ int gp_plan_enable_tape(gp_plan* plan);
 int gp_row_activity_summary(const gp_csr* A, const double* l, const double* u, double zero_tol,
                             const uint8_t* keep_row, const uint8_t* keep_col,
                             double* min_fin, double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
+
+/* ---- learned pass scheduler ----------------------------------------------- */
+
+/* Group of a rule (GP_RULE_* -> GP_GROUP_*). */
+int gp_rule_group(int32_t rule);
+/* Features of the round about to run (host memory, no device call).
+ * history[0..rounds_done) are the rounds already run (trace records), phase the
+ * phase of the next round, nnz_now its live nnz, nnz0/m0/n0 the input's size,
+ * bound_only_rounds/limit the scheduler's bound-only counter and limit. */
+void gp_schedule_features(const gp_round_trace* history, int32_t rounds_done, int32_t phase,
+                          int64_t nnz_now, int64_t nnz0, int32_t m0, int32_t n0,
+                          int32_t bound_only_rounds, int32_t bound_only_limit, float* features);
+/* Training and inference kernels; every pointer is device memory.
+ * forward: logits[batch][7] of x[batch][32]. */
+int gp_mlp_forward(const float* params, const float* x, int32_t batch, float* logits);
+/* Masked binary cross-entropy with logits, mean over the mask's ones:
+ * loss[1] and grad[GP_MLP_PARAMS] of params at (x, y, mask), each [batch][7]
+ * for y and mask. */
+int gp_mlp_loss_grad(const float* params, const float* x, const float* y, const float* mask,
+                     int32_t batch, float* loss, float* grad);
+/* One Adam step (bias-corrected, step t >= 1) over GP_MLP_PARAMS values. */
+int gp_mlp_adam_step(float* params, const float* grad, float* m, float* v, int32_t t, float lr,
+                     float beta1, float beta2, float eps);
 
 /* ---- matrix primitives --------------------------------------------------- */
 

@@ -94,6 +94,17 @@ void or_result_free(or_result* r);
 int or_postsolve(const or_result* r, const gp_device_lp* original, const double* x,
                  const double* y, const double* z, double* x_out, double* y_out, double* z_out);
 
+/* Learned pass scheduler (oracle_mlp.c; no reference counterpart). */
+int or_rule_group(int32_t rule);
+void or_schedule_features(const gp_round_trace* hist, int32_t done, int32_t phase, int64_t nnz_now,
+                          int64_t nnz0, int32_t m0, int32_t n0, int32_t bound_only, int32_t limit,
+                          float* features);
+void or_mlp_forward(const float* params, const float* x, int32_t batch, float* logits);
+void or_mlp_loss_grad(const float* params, const float* x, const float* y, const float* mask, int32_t batch,
+                      double* loss, float* grad);
+void or_mlp_adam_step(float* params, const float* grad, float* m, float* v, int32_t t, float lr, float beta1,
+                      float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,12 @@ def load():
             "or_row_sum_finite": (C.c_double, [vp, C.c_int64]),
             "or_transpose": (None, [C.POINTER(G.GpCsr), C.POINTER(G.GpCsr)]),
             "or_csr_free": (None, [C.POINTER(G.GpCsr)]),
+            "or_rule_group": (C.c_int, [C.c_int32]),
+            "or_schedule_features": (None, [C.POINTER(G.GpRoundTrace), C.c_int32, C.c_int32, C.c_int64,
+                                            C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
+            "or_mlp_forward": (None, [vp, vp, C.c_int32, vp]),
+            "or_mlp_loss_grad": (None, [vp, vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]),
+            "or_mlp_adam_step": (None, [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -120,7 +126,7 @@ class Result:
         self.has_infeasible = r.has_infeasible
         self.has_unbounded = r.has_unbounded
         self.obj_constant_delta = r.obj_constant_delta
-        self.trace = [r.trace[i] for i in range(r.trace_count)]
+        self.trace = [G.GpRoundTrace.from_buffer_copy(r.trace[i]) for i in range(r.trace_count)]
         d = r.reduced
         m, n, nnz = d.A.rows, d.A.cols, d.A.nnz
         self.reduced = G.HostLP(m, n, _arr(d.A.row_ptr, m + 1, G.NP_OFFSET), _arr(d.A.col_idx, nnz, np.int32),
@@ -252,3 +258,38 @@ class Plan:
             self.free()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- learned pass scheduler
+def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Logits [batch][7] of the scheduler MLP (sequential FP32 restatement)."""
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, G.GP_MLP_IN)
+    out = np.empty((X.shape[0], G.GP_MLP_OUT), dtype=np.float32)
+    load().or_mlp_forward(_p(P), _p(X), X.shape[0], _p(out))
+    return out
+
+
+def mlp_loss_grad(params, x, y, mask):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, G.GP_MLP_IN)
+    Y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, G.GP_MLP_OUT)
+    M = np.ascontiguousarray(mask, dtype=np.float32).reshape(-1, G.GP_MLP_OUT)
+    grad = np.empty(G.GP_MLP_PARAMS, dtype=np.float32)
+    loss = C.c_double(0.0)
+    load().or_mlp_loss_grad(_p(P), _p(X), _p(Y), _p(M), X.shape[0], C.byref(loss), _p(grad))
+    return float(loss.value), grad
+
+
+def mlp_adam_step(params, grad, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """In-place Adam step on float32 arrays."""
+    load().or_mlp_adam_step(_p(params), _p(np.ascontiguousarray(grad, dtype=np.float32)), _p(m), _p(v), int(t),
+                            float(lr), float(beta1), float(beta2), float(eps))
+
+
+def schedule_features(trace, done, phase, nnz_now, nnz0, m0, n0, bound_only, limit) -> np.ndarray:
+    arr = (G.GpRoundTrace * max(1, len(trace)))(*trace) if trace else (G.GpRoundTrace * 1)()
+    f = np.empty(G.GP_MLP_IN, dtype=np.float32)
+    load().or_schedule_features(arr, int(done), int(phase), int(nnz_now), int(nnz0), int(m0), int(n0),
+                                int(bound_only), int(limit), _p(f))
+    return f

@@ -257,6 +257,14 @@ static int materialized_rule(or_sched* s, int enabled, int32_t rule) {
   return changed;
 }
 
+/* rule enabled by the parameters and its group chosen for this round */
+static int gate_on(int enabled, int32_t rule, uint32_t run, gp_round_trace* tr) {
+  if (!enabled) return 0;
+  if ((run >> or_rule_group(rule)) & 1u) return 1;
+  tr->skipped_mask |= 1 << rule;
+  return 0;
+}
+
 static int good_progress(int64_t before, int64_t after, int64_t minimum) {
   return before >= minimum && after <= before - minimum;
 }
@@ -306,6 +314,15 @@ int or_presolve(const gp_device_lp* lp, const gp_params* p, or_result* out) {
     }
   }
 
+  /* Learned pass scheduler (no reference counterpart): the network scores the
+   * rule groups before each round and groups below their threshold are
+   * skipped; when the gated loop stops, the plain schedule runs on. */
+  const int learned = p->learned_schedule != 0;
+  if (learned && !p->scheduler_weights) return GP_ERR_INVALID_ARGUMENT;
+  int gating = learned;
+  gp_round_trace* hist = NULL;
+  int32_t hist_count = 0, hist_cap = 0;
+  const int32_t m_in = lp->A.rows, n_in = lp->A.cols;
   while (p->max_iters > 0 && !s.terminal && !time_exceeded(&s)) {
     if (phase != GP_PHASE_CLEANUP) {
       if (out->iterations >= p->max_iters) {
@@ -320,23 +337,40 @@ int or_presolve(const gp_device_lp* lp, const gp_params* p, or_result* out) {
     memset(&tr, 0, sizeof(tr));
     tr.phase = phase;
     tr.nnz_before = before;
+    tr.bound_only_rounds = bound_only_rounds;
+    tr.bound_only_limit = limit;
+    tr.final_sweep = learned && !gating;
+    uint32_t run = ~0u;
+    if (gating) {
+      float f[GP_MLP_IN], logits[GP_MLP_OUT];
+      or_schedule_features(hist, hist_count, phase, before, initial_live, m_in, n_in, bound_only_rounds, limit, f);
+      or_mlp_forward(p->scheduler_weights, f, 1, logits);
+      run = 0;
+      for (int g = 0; g < GP_MLP_OUT; ++g)
+        if (logits[g] >= p->scheduler_weights[GP_MLP_PARAMS + g]) run |= 1u << g;
+    }
+#define ON(enabled, rule) gate_on(enabled, rule, run, &tr)
     s.fired_mask = 0;
     int64_t after = before;
     int stop = 0;
     if (phase == GP_PHASE_CLEANUP) {
       const uint64_t serial = s.deletion_serial;
-      int changed = deferred_rule(&s, p->enable_infeasible_and_fixed_variables, GP_RULE_FIXED_VARIABLES);
-      changed = deferred_rule(&s, p->enable_empty_columns, GP_RULE_EMPTY_COLUMNS) || changed;
-      changed = deferred_rule(&s, p->enable_empty_rows, GP_RULE_EMPTY_ROWS) || changed;
-      changed = deferred_rule(&s, p->enable_singleton_rows, GP_RULE_SINGLETON_ROWS) || changed;
-      changed = deferred_rule(&s, p->enable_singleton_columns, GP_RULE_SINGLETON_COLUMNS) || changed;
+      int changed = deferred_rule(&s, ON(p->enable_infeasible_and_fixed_variables, GP_RULE_FIXED_VARIABLES),
+                                  GP_RULE_FIXED_VARIABLES);
+      changed = deferred_rule(&s, ON(p->enable_empty_columns, GP_RULE_EMPTY_COLUMNS), GP_RULE_EMPTY_COLUMNS) || changed;
+      changed = deferred_rule(&s, ON(p->enable_empty_rows, GP_RULE_EMPTY_ROWS), GP_RULE_EMPTY_ROWS) || changed;
+      changed = deferred_rule(&s, ON(p->enable_singleton_rows, GP_RULE_SINGLETON_ROWS), GP_RULE_SINGLETON_ROWS) ||
+                changed;
+      changed = deferred_rule(&s, ON(p->enable_singleton_columns, GP_RULE_SINGLETON_COLUMNS),
+                              GP_RULE_SINGLETON_COLUMNS) ||
+                changed;
       if (s.deletion_serial != serial) bound_only_rounds = 0;
       after = live_nnz(&s);
       phase = changed ? GP_PHASE_CLEANUP : GP_PHASE_PROPAGATION;
     } else if (phase == GP_PHASE_PROPAGATION) {
       const uint64_t serial = s.deletion_serial;
       int bounds_changed = 0;
-      if (p->enable_bound_strengthening && !s.terminal && !time_exceeded(&s)) {
+      if (ON(p->enable_bound_strengthening, GP_RULE_BOUND_STRENGTHENING) && !s.terminal && !time_exceeded(&s)) {
         prepare_deferred(&s);
         const int32_t n = s.cur.lp.A.cols;
         if (n) {
@@ -347,8 +381,8 @@ int or_presolve(const gp_device_lp* lp, const gp_params* p, or_result* out) {
         }
         bounds_changed = deferred_rule(&s, 1, GP_RULE_BOUND_STRENGTHENING);
       }
-      deferred_rule(&s, p->enable_infeasible_and_redundant_rows, GP_RULE_REDUNDANT_ROWS);
-      deferred_rule(&s, p->enable_dual_fix, GP_RULE_DUAL_FIX);
+      deferred_rule(&s, ON(p->enable_infeasible_and_redundant_rows, GP_RULE_REDUNDANT_ROWS), GP_RULE_REDUNDANT_ROWS);
+      deferred_rule(&s, ON(p->enable_dual_fix, GP_RULE_DUAL_FIX), GP_RULE_DUAL_FIX);
       const int deleted = s.deletion_serial != serial;
       bound_only_rounds = deleted ? 0 : bound_only_rounds + 1;
       materialize(&s);
@@ -358,29 +392,42 @@ int or_presolve(const gp_device_lp* lp, const gp_params* p, or_result* out) {
       phase = nnz_progress || bound_progress ? GP_PHASE_CLEANUP : GP_PHASE_AGGREGATION;
       if (phase == GP_PHASE_AGGREGATION) bound_only_rounds = 0;
     } else if (phase == GP_PHASE_AGGREGATION) {
-      materialized_rule(&s, p->enable_doubleton_equations, GP_RULE_DOUBLETON_EQUATIONS);
+      materialized_rule(&s, ON(p->enable_doubleton_equations, GP_RULE_DOUBLETON_EQUATIONS),
+                        GP_RULE_DOUBLETON_EQUATIONS);
       after = live_nnz(&s);
       phase = good_progress(before, after, min_dec) ? GP_PHASE_CLEANUP : GP_PHASE_PARALLEL;
     } else {
-      int changed = deferred_rule(&s, p->enable_parallel_rows, GP_RULE_PARALLEL_ROWS);
-      changed = deferred_rule(&s, p->enable_parallel_columns, GP_RULE_PARALLEL_COLUMNS) || changed;
+      int changed = deferred_rule(&s, ON(p->enable_parallel_rows, GP_RULE_PARALLEL_ROWS), GP_RULE_PARALLEL_ROWS);
+      changed = deferred_rule(&s, ON(p->enable_parallel_columns, GP_RULE_PARALLEL_COLUMNS), GP_RULE_PARALLEL_COLUMNS) ||
+                changed;
       materialize(&s);
       after = live_nnz(&s);
       const int cont = changed && good_progress(before, after, min_dec);
       if (!cont) stop = 1;
       else phase = GP_PHASE_CLEANUP;
     }
-    if (p->collect_trace) {
-      tr.next_phase = stop ? GP_PHASE_END : phase;
-      tr.nnz_after = after;
-      tr.fired_mask = s.fired_mask;
-      tr.rows_live = s.cur.lp.A.rows;
-      tr.cols_live = s.cur.lp.A.cols;
-      tr.seconds = now_seconds() - t0;
-      trace_push(out, &tr);
+#undef ON
+    tr.next_phase = stop ? GP_PHASE_END : phase;
+    tr.nnz_after = after;
+    tr.fired_mask = s.fired_mask;
+    tr.rows_live = s.cur.lp.A.rows;
+    tr.cols_live = s.cur.lp.A.cols;
+    tr.seconds = now_seconds() - t0;
+    if (learned) {
+      if (hist_count + 1 > hist_cap) {
+        hist_cap = 2 * hist_cap + 16;
+        hist = (gp_round_trace*)or_xrealloc(hist, sizeof(gp_round_trace) * (size_t)hist_cap);
+      }
+      hist[hist_count++] = tr;
     }
-    if (stop) break;
+    if (p->collect_trace) trace_push(out, &tr);
+    if (stop) {
+      if (!gating) break;
+      gating = 0; /* the plain schedule takes over from here */
+      phase = GP_PHASE_CLEANUP;
+    }
   }
+  free(hist);
   materialize(&s);
 
   out->m1 = s.cur.lp.A.rows;
