@@ -50,6 +50,99 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pct(v, q):
+    import numpy as np
+
+    return round(float(np.percentile(np.asarray(v) * 1e3, q)), 3)
+
+
+def batched_leg(count, seed):
+    """Batched presolve of `count` LPs (rows log-uniform in [1e3, 1e5], columns
+    1.5 x rows, uniform degree 6) resident in HBM: one gp_preprocess_batch
+    against the plain path presolving the same LPs one after another."""
+    import gpupre as G
+
+    t = time.time()
+    lps = G.batch_workload(count, seed)
+    dl = [G.DeviceLP(lp) for lp in lps]
+    nnz = sum(lp.nnz for lp in lps)
+    log(f"batched workload: {count} LPs, {sum(lp.m for lp in lps)} rows, {nnz} nnz "
+        f"(generated + uploaded in {time.time() - t:.1f}s)")
+    params = G.parity_params()
+    G.preprocess_batch(dl, params).free()  # warm-up (pool)
+    walls = []
+    for _ in range(3):
+        G.check(G.load().gp_synchronize())
+        t = time.perf_counter()
+        b = G.preprocess_batch(dl, params)
+        walls.append(time.perf_counter() - t)
+        w = b.whole()
+        red = (w.reduced_rows, w.reduced_cols, w.reduced_nnz, w.iterations)
+        asm, pre = b.s.assemble_seconds, b.s.presolve_seconds
+        b.free()
+    batch_s = min(walls)
+    log(f"batched: {batch_s * 1e3:.1f} ms (assembly {asm * 1e3:.1f} ms, presolve {pre * 1e3:.1f} ms) -> {red}")
+    lat = []
+    G.check(G.load().gp_synchronize())
+    t0 = time.perf_counter()
+    for d in dl:
+        t = time.perf_counter()
+        G.preprocess(d, params).free()
+        lat.append(time.perf_counter() - t)
+    serial_s = time.perf_counter() - t0
+    log(f"plain, one LP after another: {serial_s * 1e3:.1f} ms, p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms")
+    for d in dl:
+        d.free()
+    return {"lps": count, "nnz": nnz, "lps_per_s": round(count / batch_s, 2), "ms": round(batch_s * 1e3, 2),
+            "assembly_ms": round(asm * 1e3, 2), "latency_ms_p50": round(batch_s * 1e3, 2),
+            "latency_ms_p99": round(batch_s * 1e3, 2),
+            "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3]},
+            "plain_serial": {"lps_per_s": round(count / serial_s, 2), "ms": round(serial_s * 1e3, 2),
+                             "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)},
+            "speedup_vs_plain": round(serial_s / batch_s, 2),
+            "note": "every instance of a batch completes with the batch, so its per-instance latency is the "
+                    "batch wall time"}
+
+
+def learned_leg(seed):
+    """Learned pass scheduler against the plain schedule on held-out LPs of both
+    training families (QAP n in {14, 18, 22}; the synthetic generator at 200k
+    rows, 3 seeds): best-of-3 presolve wall time and rule launches."""
+    import gpupre as G
+    import learned as L
+
+    if not os.path.exists(L.WEIGHTS_PATH):
+        return None
+    blob = L.load_blob()
+    cases = [("qap", G.generate_qap(n, seed=1_000_000 + seed + n)) for n in (14, 18, 22)]
+    cases += [("gen200k", G.generate(200_000, 300_000, seed=1_000_001 + seed + k, degree_mode=0, mean_degree=6.0))
+              for k in range(3)]
+    out = {}
+    for fam, lp in cases:
+        with G.DeviceLP(lp) as d:
+            for name, p in (("plain", G.parity_params(collect_trace=1)), ("learned", L.learned_params(blob, collect_trace=1))):
+                G.preprocess(d, p).free()
+                best, launches, nnz = None, 0, 0
+                for _ in range(3):
+                    G.check(G.load().gp_synchronize())
+                    t = time.perf_counter()
+                    s = G.preprocess(d, p)
+                    dt = time.perf_counter() - t
+                    launches, nnz = L.rule_launches(s.trace()), s.reduced_nnz
+                    s.free()
+                    best = dt if best is None else min(best, dt)
+                row = out.setdefault(fam, {}).setdefault(name, {"ms": 0.0, "rule_launches": 0, "reduced_nnz": 0})
+                row["ms"] += best * 1e3
+                row["rule_launches"] += launches
+                row["reduced_nnz"] += nnz
+    for fam in out:
+        for name in out[fam]:
+            out[fam][name]["ms"] = round(out[fam][name]["ms"], 3)
+        out[fam]["speedup"] = round(out[fam]["plain"]["ms"] / out[fam]["learned"]["ms"], 3)
+    log(f"learned vs plain: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +152,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-presolve leg")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
@@ -190,6 +286,10 @@ def main():
             cpu = restatement
             restatement = None
 
+    dlp.free()
+    batched = None if args.no_batched else batched_leg(args.batch_count, args.seed)
+    learned = None if args.no_learned else learned_leg(args.seed)
+
     value = lp.nnz / (ms_per_step * 1e-3) / 1e6
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mnnz/s", "n_gpus": 1, "steps": args.steps,
@@ -202,9 +302,9 @@ def main():
         "presolve_core_ms_p50": round(float(np.median(core)) * 1e3, 3),
         "postsolve_ms": round(post_ms, 2), "postsolve_records": int(rec_count),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
+        "batched": batched, "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)
-    dlp.free()
 
 
 if __name__ == "__main__":

@@ -438,6 +438,27 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf);
 
+// ---- batched presolve (gp_batch.hip) --------------------------------------------------------
+// A batch is presolved as the block-diagonal direct sum of its instances.  The
+// rules need no change for that (no reduction couples two blocks); only the
+// objective constant is kept per instance as well.  While a batch runs, the
+// thread's BatchContext points at the physical -> original maps of the
+// working LP (updated at every materialization) and the instances' original
+// row / column offsets.
+struct BatchContext {
+  int32_t count = 0;
+  const DVec<int32_t>* row_map = nullptr;  // physical -> original row of the direct sum
+  const DVec<int32_t>* col_map = nullptr;
+  DVec<int32_t> row_off, col_off;          // [count+1] original offsets (device)
+  std::vector<double> obj;                 // per-instance objective deltas
+};
+BatchContext*& batch_context();
+// Objective delta of a rule: the fixed-order det_sum of contrib[0..n).  In a
+// batch the contributions are also summed per instance, in item order: item i
+// belongs to physical line item_line[i] (or i when item_line is null), a row
+// when item_rows is set, else a column; items must be in increasing line order.
+double obj_delta(const double* contrib, int64_t n, const int32_t* item_line = nullptr, bool item_rows = false);
+
 // ---- learned pass scheduler (gp_learned.hip) ---------------------------------------------
 int rule_group(int32_t rule);
 void schedule_features(const gp_round_trace* hist, int32_t done, int32_t phase, int64_t nnz_now, int64_t nnz0,

@@ -403,6 +403,10 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
   if (lp.A.cols) hipLaunchKernelGGL(k_iota, dim3(grid_for(lp.A.cols)), dim3(kBlock), 0, 0, rec->cols.data(), lp.A.cols);
   GP_LAUNCH_CHECK("identity maps");
   GP_CHECK(hipDeviceSynchronize());
+  if (BatchContext* bc = batch_context()) {
+    bc->row_map = &rec->rows;
+    bc->col_map = &rec->cols;
+  }
 
   Scheduler s;
   s.cur.lp = lp;

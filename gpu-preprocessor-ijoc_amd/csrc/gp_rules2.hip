@@ -1617,7 +1617,7 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
                      AL_new.data(), AU_new.data(), col_delete.data(), plan.keep_row_mask, plan.keep_col_mask,
                      plan.new_AL, plan.new_AU);
   GP_LAUNCH_CHECK("singleton columns commit");
-  plan.obj_constant_delta += det_sum(contrib.data(), n);
+  plan.obj_constant_delta += obj_delta(contrib.data(), n);
   plan.has_row_action = 1;
   plan.has_col_action = plan.has_col_action || status1;
   plan.has_change = 1;
@@ -1683,7 +1683,7 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
                      plan.new_u, plan.keep_col_mask, shift.data(), row_remove.data(), plan.new_AL, plan.new_AU,
                      plan.keep_row_mask);
   GP_LAUNCH_CHECK("dual fix apply");
-  plan.obj_constant_delta += det_sum(contrib.data(), n);
+  plan.obj_constant_delta += obj_delta(contrib.data(), n);
   plan.has_row_action = plan.has_row_action || row_changed;
   plan.has_col_action = 1;
   plan.has_change = 1;
@@ -1952,8 +1952,8 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   plan.has_row_action = subst_count > 0;
   plan.has_col_action = 1;
   plan.has_change = 1;
-  plan.obj_constant_delta += det_sum(obj_s.data(), k);
-  plan.obj_constant_delta += det_sum(obj_f.data(), k);
+  plan.obj_constant_delta += obj_delta(obj_s.data(), k, rows.data(), true);
+  plan.obj_constant_delta += obj_delta(obj_f.data(), k, rows.data(), true);
 }
 
 }  // namespace gp

@@ -227,7 +227,7 @@ double fixed_variable_records(DeviceTape* tape, const uint8_t* fixed, const doub
   DVec<double> contrib(static_cast<size_t>(n));
   hipLaunchKernelGGL(k_fixed_contrib, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, fval, c, n, contrib.data());
   GP_LAUNCH_CHECK("fixed contrib");
-  return det_sum(contrib.data(), n);
+  return obj_delta(contrib.data(), n);
 }
 
 }  // namespace gp

@@ -123,6 +123,14 @@ class GpSummary(C.Structure):
     ]
 
 
+class GpBatchSummary(C.Structure):
+    _fields_ = [("count", C.c_int32), ("whole", GpSummary), ("assembled", GpDeviceLp),
+                ("row_offsets", C.POINTER(C.c_int32)), ("col_offsets", C.POINTER(C.c_int32)),
+                ("reduced_row_offsets", C.POINTER(C.c_int32)), ("reduced_col_offsets", C.POINTER(C.c_int32)),
+                ("reduced_nnz_offsets", C.POINTER(OFFSET)), ("obj_constant", C.POINTER(C.c_double)),
+                ("assemble_seconds", C.c_double), ("presolve_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+
 class GpRecordSizes(C.Structure):
     _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
                 ("fixed_count", C.c_int32), ("record_count", C.c_int64),
@@ -260,6 +268,11 @@ def load() -> C.CDLL:
         "gp_lp_upload": (C.c_int, [C.c_int32, C.c_int32, OFFSET, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                     C.POINTER(GpDeviceLp)]),
         "gp_lp_free": (None, [C.POINTER(GpDeviceLp)]),
+        "gp_preprocess_batch": (C.c_int, [C.POINTER(GpDeviceLp), C.c_int32, C.POINTER(GpParams),
+                                          C.POINTER(GpBatchSummary)]),
+        "gp_postprocess_batch": (C.c_int, [vp, vp, vp, C.POINTER(GpBatchSummary), vp, vp, vp]),
+        "gp_batch_instance": (C.c_int, [C.POINTER(GpBatchSummary), C.c_int32, C.POINTER(GpDeviceLp)]),
+        "gp_free_batch_summary": (None, [C.POINTER(GpBatchSummary)]),
         "gp_rule_group": (C.c_int, [C.c_int32]),
         "gp_schedule_features": (None, [C.POINTER(GpRoundTrace), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
@@ -466,6 +479,79 @@ def postprocess(summary: Summary, x: np.ndarray, y: np.ndarray, z: np.ndarray, o
     return xo, yo, zo
 
 
+# ---------------------------------------------------------------- batched presolve
+class BatchSummary:
+    """Owner of a gp_batch_summary (presolve of several device LPs as one batch)."""
+
+    def __init__(self):
+        self.s = GpBatchSummary()
+
+    def free(self):
+        load().gp_free_batch_summary(C.byref(self.s))
+
+    @property
+    def count(self) -> int:
+        return self.s.count
+
+    def offsets(self, name) -> np.ndarray:
+        return np.ctypeslib.as_array(getattr(self.s, name), shape=(self.s.count + 1,)).copy()
+
+    def obj_constants(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.s.obj_constant, shape=(self.s.count,)).copy()
+
+    def whole(self) -> "Summary":
+        """The direct-sum summary as a Summary view (owned by this batch)."""
+        v = Summary()
+        v.s = self.s.whole
+        v.free = lambda: None
+        return v
+
+    def instance(self, k: int) -> HostLP:
+        """Reduced instance k, extracted on the device and copied to the host."""
+        d = GpDeviceLp()
+        check(load().gp_batch_instance(C.byref(self.s), int(k), C.byref(d)))
+        try:
+            return device_lp_to_host(d)
+        finally:
+            load().gp_lp_free(C.byref(d))
+
+    def postprocess(self, x, y, z):
+        """x, y, z over the reduced direct sum (host arrays, uploaded here); returns
+        host arrays over the original direct sum."""
+        m0, n0 = int(self.offsets("row_offsets")[-1]), int(self.offsets("col_offsets")[-1])
+        dx, dy, dz = (DeviceArray.from_host(np.asarray(v, dtype=np.float64)) for v in (x, y, z))
+        xo, yo, zo = np.empty(n0), np.empty(m0), np.empty(n0)
+        check(load().gp_postprocess_batch(dx.ptr, dy.ptr, dz.ptr, C.byref(self.s), _ptr(xo), _ptr(yo), _ptr(zo)))
+        for a in (dx, dy, dz):
+            a.free()
+        return xo, yo, zo
+
+
+def preprocess_batch(dlps, params: GpParams | None = None) -> BatchSummary:
+    arr = (GpDeviceLp * len(dlps))(*[d.lp for d in dlps])
+    out = BatchSummary()
+    check(load().gp_preprocess_batch(arr, len(dlps), C.byref(params or parity_params()), C.byref(out.s)))
+    return out
+
+
+def direct_sum(lps) -> HostLP:
+    """Host block-diagonal direct sum of host LPs (the batch's assembled LP)."""
+    rp = [np.zeros(1, dtype=NP_OFFSET)]
+    ci, va = [], []
+    nnz = 0
+    col = 0
+    for lp in lps:
+        rp.append(np.asarray(lp.row_ptr[1:], dtype=np.int64) + nnz)
+        ci.append(np.asarray(lp.col_idx, dtype=np.int64) + col)
+        va.append(lp.vals)
+        nnz += lp.nnz
+        col += lp.n
+    cat = np.concatenate
+    return HostLP(sum(lp.m for lp in lps), col, cat(rp).astype(NP_OFFSET), cat(ci).astype(np.int32), cat(va),
+                  cat([lp.c for lp in lps]), cat([lp.l for lp in lps]), cat([lp.u for lp in lps]),
+                  cat([lp.AL for lp in lps]), cat([lp.AU for lp in lps]), float(sum(lp.obj_constant for lp in lps)))
+
+
 # ---------------------------------------------------------------- generator
 class LpgenSpec(C.Structure):
     _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("seed", C.c_uint64), ("degree_mode", C.c_int32),
@@ -555,6 +641,24 @@ def _take_lpgen(g, raw) -> HostLP:
                 float(raw.obj_constant), arr(raw.x0, raw.n, np.float64), arr(raw.y0, raw.m, np.float64))
     g.lpgen_free(C.byref(raw))
     return lp
+
+
+def batch_workload(count: int = 512, seed: int = 0, min_rows: int = 1000, max_rows: int = 100_000,
+                   threads: int = 8) -> list:
+    """The batched-presolve workload: `count` independent LPs of the synthetic
+    generator (uniform column degree 6, planted structure), rows log-uniform in
+    [min_rows, max_rows], columns 1.5 x rows; instance k uses seed seed*100003+k."""
+    import concurrent.futures as cf
+
+    rng = np.random.default_rng(seed)
+    rows = np.exp(rng.uniform(np.log(min_rows), np.log(max_rows), size=count)).astype(np.int64)
+
+    def one(k):
+        m = int(rows[k])
+        return generate(m, m * 3 // 2, seed * 100003 + k, degree_mode=0, mean_degree=6.0)
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as pool:
+        return list(pool.map(one, range(count)))
 
 
 def generate_workload(name: str, seed: int = 0) -> HostLP:

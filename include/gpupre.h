@@ -265,6 +265,28 @@ typedef struct gp_summary {
   double kernel_bytes[GP_KT_COUNT]; /* algorithmic bytes per family */
 } gp_summary;
 
+/* Batched presolve (no reference counterpart; the reference presolves one LP
+ * per call).  The instances are presolved together as their block-diagonal
+ * direct sum: every round sweeps all instances in one launch per kernel.
+ * Instance k occupies rows [row_offsets[k], row_offsets[k+1]) and columns
+ * [col_offsets[k], col_offsets[k+1]) of the direct sum, and the same-numbered
+ * reduced_* ranges of whole.reduced_lp.  Scheduling decisions are taken for
+ * the batch; whole.has_infeasible / has_unbounded flag the batch. */
+typedef struct gp_batch_summary {
+  int32_t count;
+  gp_summary whole;            /* presolve of the direct sum (record, reduced LP) */
+  gp_device_lp assembled;      /* the direct sum (library-owned device memory)    */
+  int32_t* row_offsets;        /* host [count+1] */
+  int32_t* col_offsets;        /* host [count+1] */
+  int32_t* reduced_row_offsets;  /* host [count+1] */
+  int32_t* reduced_col_offsets;  /* host [count+1] */
+  gp_offset* reduced_nnz_offsets; /* host [count+1]: A entries of each reduced instance */
+  double* obj_constant;        /* host [count]: objective constant of each reduced instance */
+  double assemble_seconds;
+  double presolve_seconds;
+  double total_seconds;
+} gp_batch_summary;
+
 /* Host view of a recovery record. */
 typedef struct gp_record_sizes {
   int32_t m0, n0, m1, n1;
@@ -301,6 +323,20 @@ int gp_record_copy(const gp_summary* summary,
                    int32_t* fixed_idx, double* fixed_val,
                    int32_t* types, gp_offset* index_starts, gp_offset* value_starts,
                    uint8_t* dual_modes, int32_t* indices, double* vals);
+
+/* ---- batched presolve (no reference counterpart) ------------------------- */
+
+/* Presolve `count` device LPs (caller-owned, unmodified) as one batch. */
+int gp_preprocess_batch(const gp_device_lp* instances, int32_t count, const gp_params* params,
+                        gp_batch_summary* out);
+/* Postsolve of the whole batch: x, y, z are device arrays over the reduced
+ * direct sum; outputs are host arrays over the original direct sum. */
+int gp_postprocess_batch(const double* x, const double* y, const double* z,
+                         const gp_batch_summary* summary, double* x_out, double* y_out, double* z_out);
+/* Reduced instance k as its own device LP (library-allocated; gp_lp_free). */
+int gp_batch_instance(const gp_batch_summary* summary, int32_t k, gp_device_lp* out);
+/* Safe on NULL and on repeated calls. */
+void gp_free_batch_summary(gp_batch_summary* summary);
 
 /* ---- rule-level operator API (PresolvePlanGpu / PresolveStatsGpu) -------- */
 
