@@ -343,6 +343,14 @@ void fill_u8(uint8_t* p, uint8_t v, int64_t n);
 //   void init(State&)
 //   void add(State&, pos_in_row, lo, hi, skipped)   -- called for every stored entry
 //   void finish(row, const State&, active)  -- called for every row of the grid
+// Read-once matrix stream: the non-temporal hint asks L2 and the Infinity
+// Cache not to keep these lines, so the streamed CSR does not evict the
+// gathered bound and mask arrays.
+template <class T>
+__device__ inline T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 constexpr int kFoldRows = 256;
 constexpr int kFoldChunk = 2048;
 
@@ -363,8 +371,8 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
     for (gp_offset c0 = e0; c0 < e1; c0 += kFoldChunk) {
       const gp_offset c1 = e1 - c0 > kFoldChunk ? c0 + kFoldChunk : e1;
       for (gp_offset q = c0 + threadIdx.x; q < c1; q += kFoldRows) {
-        const int32_t j = A.col_idx[q];
-        const double a = A.vals[q];
+        const int32_t j = ld_stream(A.col_idx + q);
+        const double a = ld_stream(A.vals + q);
         const bool skip = !keep_col[j] || !pol.use(a);
         double lo = 0.0, hi = 0.0;
         if (!skip) {

@@ -581,9 +581,9 @@ __global__ void __launch_bounds__(kBlock) k_bs_candidates(gp_csr A, const uint8_
     const int32_t mni = min_ninf[i], mxp = max_pinf[i];
     const gp_offset p1 = A.row_ptr[i + 1];
     for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) {
-      const int32_t j = A.col_idx[p];
+      const int32_t j = ld_stream(A.col_idx + p);
+      const double a = ld_stream(A.vals + p);
       if (!keep_col[j]) continue;
-      const double a = A.vals[p];
       if (fabs(a) <= zero_tol) continue;
       const double2 b = lu[j];
       const double lj = b.x, uj = b.y;
@@ -607,8 +607,10 @@ __global__ void __launch_bounds__(kBlock) k_col_max_abs(gp_csr AT, Bits keep_row
     double v = 0.0;
     const gp_offset p1 = AT.row_ptr[j + 1];
     for (gp_offset p = AT.row_ptr[j] + lane; p < p1; p += G) {
-      if (!keep_row[AT.col_idx[p]]) continue;
-      v = cmax(v, fabs(AT.vals[p]));
+      const int32_t i = ld_stream(AT.col_idx + p);
+      const double a = ld_stream(AT.vals + p);
+      if (!keep_row[i]) continue;
+      v = cmax(v, fabs(a));
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v = cmax(v, __shfl_xor(v, o, G));
