@@ -351,10 +351,11 @@ __device__ inline T ld_stream(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Rows per workgroup (kFoldRows by default, or the tuned value) and the LDS
+// chunk of 8 entries per row.  The fold of a row is independent of both.
 constexpr int kFoldRows = 256;
-constexpr int kFoldChunk = 2048;
 
-template <class P>
+template <class P, int kFoldRows, int kFoldChunk = 8 * kFoldRows>
 __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2* lu, Bits keep_col, P pol) {
   __shared__ double s_lo[kFoldChunk];
   __shared__ double s_hi[kFoldChunk];
@@ -395,11 +396,28 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
   if (row < r1) pol.finish(row, st, mine);
 }
 
+// Launch-schedule knobs; every choice gives identical results.  lanes_*: lanes
+// per row of the grouped sweeps (64 / lanes rows per wavefront); fold_rows:
+// rows per workgroup of the row folds.  0 = derived from the matrix.
+struct Tuning {
+  int32_t lanes_candidates = 0, lanes_col_max = 0, lanes_compact = 0, lanes_live_stats = 0, fold_rows = 0;
+};
+Tuning& tuning();  // process-wide (gp_primitives.hip)
+
+template <class P, int R>
+void launch_row_fold_rows(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol) {
+  const int64_t blocks = (static_cast<int64_t>(A.rows) + R - 1) / R;
+  hipLaunchKernelGGL((k_row_fold<P, R>), dim3(static_cast<unsigned>(blocks)), dim3(R), 0, 0, A, lu, keep_col, pol);
+}
+
 template <class P>
 void launch_row_fold(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol) {
   if (A.rows <= 0) return;
-  const int64_t blocks = (static_cast<int64_t>(A.rows) + kFoldRows - 1) / kFoldRows;
-  hipLaunchKernelGGL(k_row_fold<P>, dim3(static_cast<unsigned>(blocks)), dim3(kFoldRows), 0, 0, A, lu, keep_col, pol);
+  switch (tuning().fold_rows) {
+    case 128: launch_row_fold_rows<P, 128>(A, lu, keep_col, pol); break;
+    case 512: launch_row_fold_rows<P, 512>(A, lu, keep_col, pol); break;
+    default: launch_row_fold_rows<P, kFoldRows>(A, lu, keep_col, pol); break;
+  }
 }
 
 // (l[j], u[j]) interleaved, so a sweep gathers both bounds with one load.
@@ -489,5 +507,7 @@ inline int lanes_for_mean(double mean) {
   if (mean <= 64.0) return 16;
   return 64;
 }
+// the tuned lane count when one is set, else the mean-length rule
+inline int pick_lanes(int32_t knob, double mean) { return knob > 0 ? knob : lanes_for_mean(mean); }
 
 }  // namespace gp

@@ -11,6 +11,11 @@ namespace gp {
 namespace {
 thread_local std::string g_last_error;
 }
+
+Tuning& tuning() {
+  static Tuning t;
+  return t;
+}
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error_cstr() { return g_last_error.c_str(); }
 
@@ -521,7 +526,7 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   {
     TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
     if (m > 0) {
-      const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
+      const int lanes = pick_lanes(tuning().lanes_compact, static_cast<double>(A.nnz) / m);
       const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
       switch (lanes) {
         case 2: hipLaunchKernelGGL(k_compact_count<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
@@ -545,7 +550,7 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   if (m > 0) {
     hipLaunchKernelGGL(k_compact_rowptr, dim3(grid_for(m)), dim3(kBlock), 0, 0, cnt.data(), row_o2n, m, out.row_ptr);
     GP_LAUNCH_CHECK("compact rowptr");
-    const int lanes = lanes_for_mean(static_cast<double>(A.nnz) / m);
+    const int lanes = pick_lanes(tuning().lanes_compact, static_cast<double>(A.nnz) / m);
     const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
     switch (lanes) {
       case 2: hipLaunchKernelGGL(k_compact_copy<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
@@ -648,11 +653,13 @@ void launch_live_stats(const gp_csr& M, const uint8_t* outer, Bits inner, int32_
 void live_stats_side(const gp_csr& M, const uint8_t* outer, Bits inner, int32_t* degree, uint8_t* empty,
                      uint8_t* singleton, int32_t* support, double* coefficient, unsigned long long* total) {
   const double mean = M.rows ? static_cast<double>(M.nnz) / M.rows : 0.0;
-  if (mean <= 3.0) launch_live_stats<2>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
-  else if (mean <= 10.0) launch_live_stats<4>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
-  else if (mean <= 24.0) launch_live_stats<8>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
-  else if (mean <= 64.0) launch_live_stats<16>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
-  else launch_live_stats<64>(M, outer, inner, degree, empty, singleton, support, coefficient, total);
+  switch (pick_lanes(tuning().lanes_live_stats, mean)) {
+    case 2: launch_live_stats<2>(M, outer, inner, degree, empty, singleton, support, coefficient, total); break;
+    case 4: launch_live_stats<4>(M, outer, inner, degree, empty, singleton, support, coefficient, total); break;
+    case 8: launch_live_stats<8>(M, outer, inner, degree, empty, singleton, support, coefficient, total); break;
+    case 16: launch_live_stats<16>(M, outer, inner, degree, empty, singleton, support, coefficient, total); break;
+    default: launch_live_stats<64>(M, outer, inner, degree, empty, singleton, support, coefficient, total); break;
+  }
 }
 }  // namespace
 

@@ -876,7 +876,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     // A, keep_row, row_nnz, AL, AU, the row summary (24 B), (l, u) and a mask bit per column,
     // candidate bounds cl / cu
     TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 45.0 * m + 32.0 * n + n / 8.0);
-    launch_grouped<BsCandidatesK>(lanes_for_mean(m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
+    launch_grouped<BsCandidatesK>(pick_lanes(tuning().lanes_candidates, m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
                                   plan.keep_row_mask, live_cols.view(), s.row_nnz, plan.new_AL, plan.new_AU,
                                   static_cast<const double2*>(lu.data()), static_cast<const double*>(min_fin.data()),
                                   static_cast<const double*>(max_fin.data()),
@@ -923,7 +923,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     if (full) {
       // AT row_ptr, col_idx, vals, a mask bit per row, 8 bytes out per column
       TimedScope ts(GP_KT_COL_MAX, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + m / 8.0 + 8.0 * n);
-      launch_grouped<ColMaxAbsK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
+      launch_grouped<ColMaxAbsK>(pick_lanes(tuning().lanes_col_max, n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
                                  live_rows.view(), cm.vmax.data());
       GP_LAUNCH_CHECK("col max abs");
     }

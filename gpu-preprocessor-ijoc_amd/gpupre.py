@@ -131,6 +131,14 @@ class GpBatchSummary(C.Structure):
                 ("assemble_seconds", C.c_double), ("presolve_seconds", C.c_double), ("total_seconds", C.c_double)]
 
 
+class GpTuning(C.Structure):
+    _fields_ = [("lanes_candidates", C.c_int32), ("lanes_col_max", C.c_int32), ("lanes_compact", C.c_int32),
+                ("lanes_live_stats", C.c_int32), ("fold_rows", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class GpRecordSizes(C.Structure):
     _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
                 ("fixed_count", C.c_int32), ("record_count", C.c_int64),
@@ -273,6 +281,10 @@ def load() -> C.CDLL:
         "gp_postprocess_batch": (C.c_int, [vp, vp, vp, C.POINTER(GpBatchSummary), vp, vp, vp]),
         "gp_batch_instance": (C.c_int, [C.POINTER(GpBatchSummary), C.c_int32, C.POINTER(GpDeviceLp)]),
         "gp_free_batch_summary": (None, [C.POINTER(GpBatchSummary)]),
+        "gp_get_tuning": (C.c_int, [C.POINTER(GpTuning)]),
+        "gp_set_tuning": (C.c_int, [C.POINTER(GpTuning)]),
+        "gp_autotune": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpParams), C.c_int32, C.POINTER(GpTuning),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gp_rule_group": (C.c_int, [C.c_int32]),
         "gp_schedule_features": (None, [C.POINTER(GpRoundTrace), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
@@ -525,6 +537,19 @@ class BatchSummary:
         for a in (dx, dy, dz):
             a.free()
         return xo, yo, zo
+
+
+def autotune(lp: GpDeviceLp, params: GpParams | None = None, reps: int = 1):
+    """(best GpTuning, tuned ms, default ms) measured on a device LP."""
+    best = GpTuning()
+    t, d = C.c_double(), C.c_double()
+    check(load().gp_autotune(C.byref(lp), C.byref(params or parity_params()), int(reps), C.byref(best),
+                             C.byref(t), C.byref(d)))
+    return best, t.value, d.value
+
+
+def set_tuning(t: GpTuning | None = None):
+    check(load().gp_set_tuning(C.byref(t or GpTuning())))
 
 
 def preprocess_batch(dlps, params: GpParams | None = None) -> BatchSummary:

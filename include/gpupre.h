@@ -324,6 +324,29 @@ int gp_record_copy(const gp_summary* summary,
                    int32_t* types, gp_offset* index_starts, gp_offset* value_starts,
                    uint8_t* dual_modes, int32_t* indices, double* vals);
 
+/* ---- launch-schedule autotuner (no reference counterpart) ---------------- */
+
+/* Result-invariant launch schedule: lanes per row of the grouped sweeps
+ * (0, 2, 4, 8, 16 or 64; 64 / lanes rows per wavefront) and rows per
+ * workgroup of the row folds (0, 128, 256 or 512).  0 = derived from the
+ * matrix's mean row length.  Process-wide. */
+typedef struct gp_tuning {
+  int32_t lanes_candidates;
+  int32_t lanes_col_max;
+  int32_t lanes_compact;
+  int32_t lanes_live_stats;
+  int32_t fold_rows;
+} gp_tuning;
+
+int gp_get_tuning(gp_tuning* tuning);
+int gp_set_tuning(const gp_tuning* tuning);
+/* Measure each candidate schedule on `lp` (best of `reps` presolves each) and
+ * return the fastest per knob in *best -- or all zeros when the tuned
+ * schedule is not faster overall.  The current schedule is left unchanged.
+ * tuned_ms / default_ms (may be NULL): presolve core ms with either. */
+int gp_autotune(const gp_device_lp* lp, const gp_params* params, int32_t reps, gp_tuning* best,
+                double* tuned_ms, double* default_ms);
+
 /* ---- batched presolve (no reference counterpart) ------------------------- */
 
 /* Presolve `count` device LPs (caller-owned, unmodified) as one batch. */

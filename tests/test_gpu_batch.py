@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 def instances():
     out = [G.generate(m, m * 3 // 2, seed=40 + k, degree_mode=0, mean_degree=6.0)
-           for k, m in enumerate((1000, 2500, 1200, 4000, 1500))]
+           for k, m in enumerate((1000, 2500, 1200, 3000, 1500))]
     out.insert(2, G.generate_qap(4, seed=9))
     out.append(G.HostLP.from_dense(np.array([[1.0, 2.0]]), np.array([1.0, 1.0]), np.array([0.0, 0.0]),
                                    np.array([4.0, 4.0]), np.array([1.0]), np.array([3.0]), obj_constant=0.5))
