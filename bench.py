@@ -56,7 +56,7 @@ def pct(v, q):
     return round(float(np.percentile(np.asarray(v) * 1e3, q)), 3)
 
 
-def batched_leg(count, seed):
+def batched_leg(count, seed, serial=True):
     """Batched presolve of `count` LPs (rows log-uniform in [1e3, 1e5], columns
     1.5 x rows, uniform degree 6) resident in HBM: one gp_preprocess_batch
     against the plain path presolving the same LPs one after another."""
@@ -94,12 +94,15 @@ def batched_leg(count, seed):
     lat = []
     G.check(G.load().gp_synchronize())
     t0 = time.perf_counter()
-    for d in dl:
+    for d in (dl if serial else []):
         t = time.perf_counter()
         G.preprocess(d, params).free()
         lat.append(time.perf_counter() - t)
     serial_s = time.perf_counter() - t0
-    log(f"plain, one LP after another: {serial_s * 1e3:.1f} ms, p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms")
+    if serial:
+        log(f"plain, one LP after another: {serial_s * 1e3:.1f} ms, p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms")
+    else:
+        lat, serial_s = [float("nan")], float("nan")
     for d in dl:
         d.free()
     return {"lps": count, "nnz": nnz, "lps_per_s": round(count / batch_s, 2), "ms": round(batch_s * 1e3, 2),
@@ -107,8 +110,8 @@ def batched_leg(count, seed):
             "latency_ms_p99": round(batch_s * 1e3, 2),
             "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3]},
             "plain_serial": {"lps_per_s": round(count / serial_s, 2), "ms": round(serial_s * 1e3, 2),
-                             "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)},
-            "speedup_vs_plain": round(serial_s / batch_s, 2),
+                             "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)} if serial else None,
+            "speedup_vs_plain": round(serial_s / batch_s, 2) if serial else None,
             "tuning": tune.as_dict(), "tuned_presolve_ms": round(tuned_ms, 2),
             "untuned_presolve_ms": round(default_ms, 2),
             "note": "every instance of a batch completes with the batch, so its per-instance latency is the "
@@ -165,6 +168,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-presolve leg")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batched leg without the one-LP-after-another comparison (profiling runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
@@ -298,7 +303,7 @@ def main():
             restatement = None
 
     dlp.free()
-    batched = None if args.no_batched else batched_leg(args.batch_count, args.seed)
+    batched = None if args.no_batched else batched_leg(args.batch_count, args.seed, not args.no_serial)
     learned = None if args.no_learned else learned_leg(args.seed)
 
     value = lp.nnz / (ms_per_step * 1e-3) / 1e6

@@ -362,20 +362,41 @@ __global__ void k_col_hist(const int32_t* col, int64_t nnz, int32_t* cnt) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p < nnz) atomicAdd(cnt + col[p], 1);
 }
-__global__ void k_scatter_rows(const gp_offset* rp, const int32_t* col, const double* val, int32_t m,
-                               int32_t* cursor, int32_t* at_col, double* at_val) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (gp_offset p = rp[i]; p < rp[i + 1]; ++p) {
-    const int32_t q = atomicAdd(cursor + col[p], 1);
-    at_col[q] = static_cast<int32_t>(i);
-    at_val[q] = val[p];
+// G lanes per row: coalesced reads of the row, one cursor atomic per entry.
+// The order inside a column is restored by the segment sorts below.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_scatter_rows(const gp_offset* rp, const int32_t* col, const double* val,
+                                                          int32_t m, int32_t* cursor, int32_t* at_col,
+                                                          double* at_val) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
+    const gp_offset p1 = rp[i + 1];
+    for (gp_offset p = rp[i] + lane; p < p1; p += G) {
+      const int32_t q = atomicAdd(cursor + ld_stream(col + p), 1);
+      at_col[q] = static_cast<int32_t>(i);
+      at_val[q] = ld_stream(val + p);
+    }
   }
 }
-__global__ void k_sort_segments(const gp_offset* rp, int32_t n, int32_t* idx, double* val) {
+constexpr int kShortSegment = 32;
+constexpr int kLongSegment = 8192;
+// Segments of at most kShortSegment entries: one lane each, insertion sort.
+// Longer ones are listed for k_sort_long (up to kLongSegment entries) or for a
+// device radix sort of their own (longer still: dense columns).
+__global__ void k_sort_segments(const gp_offset* rp, int32_t n, int32_t* idx, double* val, int32_t* long_list,
+                                int32_t* counts) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const gp_offset a = rp[j], b = rp[j + 1];
+  if (b - a > kLongSegment) {
+    long_list[n - 1 - atomicAdd(counts + 1, 1)] = static_cast<int32_t>(j);  // huge: listed from the end
+    return;
+  }
+  if (b - a > kShortSegment) {
+    long_list[atomicAdd(counts, 1)] = static_cast<int32_t>(j);
+    return;
+  }
   for (gp_offset p = a + 1; p < b; ++p) {
     const int32_t key = idx[p];
     const double v = val[p];
@@ -388,6 +409,52 @@ __global__ void k_sort_segments(const gp_offset* rp, int32_t n, int32_t* idx, do
     idx[q + 1] = key;
     val[q + 1] = v;
   }
+}
+// One workgroup per long segment, in LDS chunks: every entry is placed by its
+// rank, the number of entries with a smaller key or an equal key before it.
+constexpr int kLongChunk = 2048;
+__global__ void __launch_bounds__(256) k_sort_long(const gp_offset* rp, const int32_t* long_list,
+                                                   const int32_t* long_count, int32_t* idx, double* val,
+                                                   int32_t* tmp_idx, double* tmp_val) {
+  __shared__ int32_t keys[kLongChunk];
+  const int32_t total = *long_count;
+  for (int32_t s = blockIdx.x; s < total; s += gridDim.x) {
+    const int32_t j = long_list[s];
+    const gp_offset a = rp[j], b = rp[j + 1];
+    const int64_t len = b - a;
+    // ranks against every chunk of the segment, accumulated per element
+    for (int64_t e0 = 0; e0 < len; e0 += 256) {
+      const int64_t e = e0 + threadIdx.x;
+      const int32_t mine = e < len ? idx[a + e] : 0;
+      int64_t rank = 0;
+      for (int64_t c0 = 0; c0 < len; c0 += kLongChunk) {
+        const int64_t c1 = c0 + kLongChunk < len ? c0 + kLongChunk : len;
+        __syncthreads();
+        for (int64_t k = c0 + threadIdx.x; k < c1; k += 256) keys[k - c0] = idx[a + k];
+        __syncthreads();
+        if (e < len)  // stable: equal keys keep their order
+          for (int64_t k = 0; k < c1 - c0; ++k) rank += (keys[k] < mine || (keys[k] == mine && c0 + k < e)) ? 1 : 0;
+      }
+      if (e < len) {
+        tmp_idx[a + rank] = mine;
+        tmp_val[a + rank] = val[a + e];
+      }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < len; e += 256) {
+      idx[a + e] = tmp_idx[a + e];
+      val[a + e] = tmp_val[a + e];
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_widen_keys(const int32_t* idx, int64_t len, uint64_t* keys) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < len) keys[k] = static_cast<uint32_t>(idx[k]);
+}
+__global__ void k_narrow_keys(const uint64_t* keys, int64_t len, int32_t* idx) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < len) idx[k] = static_cast<int32_t>(keys[k]);
 }
 __global__ void k_i32_to_offset_shift(const int32_t* src, int32_t n, gp_offset* dst) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -415,11 +482,43 @@ void transpose_csr(const gp_csr& A, gp_csr& AT) {
   hipLaunchKernelGGL(k_i32_to_offset_shift, dim3(grid_for(n + 1)), dim3(kBlock), 0, 0, cnt.data(), n + 1, AT.row_ptr);
   GP_LAUNCH_CHECK("row ptr");
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
-                       cnt.data(), AT.col_idx, AT.vals);
+    const unsigned grid = grid_capped(static_cast<int64_t>(m) * 8, kBlock, 16384);
+    switch (lanes_for_mean(m ? static_cast<double>(nnz) / m : 0.0)) {
+      case 2: hipLaunchKernelGGL(k_scatter_rows<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, cnt.data(), AT.col_idx, AT.vals); break;
+      case 4: hipLaunchKernelGGL(k_scatter_rows<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, cnt.data(), AT.col_idx, AT.vals); break;
+      case 8: hipLaunchKernelGGL(k_scatter_rows<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, cnt.data(), AT.col_idx, AT.vals); break;
+      case 16: hipLaunchKernelGGL(k_scatter_rows<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, cnt.data(), AT.col_idx, AT.vals); break;
+      default: hipLaunchKernelGGL(k_scatter_rows<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, cnt.data(), AT.col_idx, AT.vals); break;
+    }
     GP_LAUNCH_CHECK("scatter rows");
-    hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n)), dim3(kBlock), 0, 0, AT.row_ptr, n, AT.col_idx, AT.vals);
+    DVec<int32_t> long_list(static_cast<size_t>(n) + 1), counts(2);
+    GP_CHECK(hipMemsetAsync(counts.data(), 0, 2 * sizeof(int32_t), 0));
+    hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n)), dim3(kBlock), 0, 0, AT.row_ptr, n, AT.col_idx, AT.vals,
+                       long_list.data(), counts.data());
     GP_LAUNCH_CHECK("sort segments");
+    DVec<int32_t> tmp_idx(static_cast<size_t>(nnz));
+    DVec<double> tmp_val(static_cast<size_t>(nnz));
+    hipLaunchKernelGGL(k_sort_long, dim3(grid_capped(static_cast<int64_t>(n) * 256, 256, 2048)), dim3(256), 0, 0,
+                       AT.row_ptr, long_list.data(), counts.data(), AT.col_idx, AT.vals, tmp_idx.data(),
+                       tmp_val.data());
+    GP_LAUNCH_CHECK("sort long segments");
+    int32_t huge = 0;
+    d2h(&huge, counts.data() + 1, 1);
+    if (huge > 0) {
+      std::vector<int32_t> cols(static_cast<size_t>(huge));
+      d2h(cols.data(), long_list.data() + (n - huge), static_cast<size_t>(huge));
+      std::vector<gp_offset> rp(static_cast<size_t>(n) + 1);
+      d2h(rp.data(), AT.row_ptr, rp.size());
+      DVec<uint64_t> keys;
+      for (const int32_t j : cols) {
+        const int64_t a = rp[j], len = rp[j + 1] - rp[j];
+        keys.resize(static_cast<size_t>(len));
+        hipLaunchKernelGGL(k_widen_keys, dim3(grid_for(len)), dim3(kBlock), 0, 0, AT.col_idx + a, len, keys.data());
+        sort_pairs_u64_f64(keys.data(), AT.vals + a, len, 32);
+        hipLaunchKernelGGL(k_narrow_keys, dim3(grid_for(len)), dim3(kBlock), 0, 0, keys.data(), len, AT.col_idx + a);
+        GP_LAUNCH_CHECK("sort dense column");
+      }
+    }
   }
 }
 
@@ -587,6 +686,76 @@ void PackedMask::pack(const uint8_t* mask, int64_t n) {
     hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(nw)), dim3(kBlock), 0, 0, mask, n, words.data());
     GP_LAUNCH_CHECK("pack bits");
   }
+}
+
+// ---- rows touching marked columns ------------------------------------------------------------
+namespace {
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_t* keep_row, Bits mark,
+                                                           int32_t* list, int32_t* count) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < A.rows; i += groups) {
+    int hit = 0;
+    if (!keep_row || keep_row[i]) {
+      const gp_offset p1 = A.row_ptr[i + 1];
+      for (gp_offset p = A.row_ptr[i] + lane; p < p1 && !hit; p += G) hit = mark[ld_stream(A.col_idx + p)] ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, G);
+    if (lane == 0 && hit) list[atomicAdd(count, 1)] = static_cast<int32_t>(i);
+  }
+}
+
+template <int G>
+void launch_rows_touching(const gp_csr& A, const uint8_t* keep_row, Bits mark, int32_t* list, int32_t* count) {
+  const unsigned grid = grid_capped(static_cast<int64_t>(A.rows) * G, kBlock, 16384);
+  hipLaunchKernelGGL(k_rows_touching<G>, dim3(grid), dim3(kBlock), 0, 0, A, keep_row, mark, list, count);
+}
+
+__global__ void k_shift_listed(gp_csr A, const int32_t* list, const int32_t* count, const uint8_t* sel,
+                               const double* val, double* AL, double* AU) {
+  const int32_t total = *count;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = list[k];
+    RowAcc acc;
+    const gp_offset p0 = A.row_ptr[i];
+    for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
+      acc.pos(q - p0);
+      const int32_t j = A.col_idx[q];
+      if (sel[j]) acc.add(A.vals[q] * val[j]);
+    }
+    const double sum = acc.value();
+    AL[i] -= sum;
+    AU[i] -= sum;
+  }
+}
+}  // namespace
+
+void list_rows_touching(const gp_csr& A, const uint8_t* keep_row, const uint8_t* col_mark, int32_t* list,
+                        int32_t* count) {
+  if (A.rows <= 0) return;
+  const PackedMask mark(col_mark, A.cols);
+  switch (lanes_for_mean(static_cast<double>(A.nnz) / A.rows)) {
+    case 2: launch_rows_touching<2>(A, keep_row, mark.view(), list, count); break;
+    case 4: launch_rows_touching<4>(A, keep_row, mark.view(), list, count); break;
+    case 8: launch_rows_touching<8>(A, keep_row, mark.view(), list, count); break;
+    case 16: launch_rows_touching<16>(A, keep_row, mark.view(), list, count); break;
+    default: launch_rows_touching<64>(A, keep_row, mark.view(), list, count); break;
+  }
+  GP_LAUNCH_CHECK("rows touching columns");
+}
+
+void shift_rows_by_columns(const gp_csr& A, const uint8_t* keep_row, const uint8_t* sel, const double* val,
+                           double* AL, double* AU) {
+  if (A.rows <= 0) return;
+  DVec<int32_t> list(static_cast<size_t>(A.rows)), count(1);
+  GP_CHECK(hipMemsetAsync(count.data(), 0, sizeof(int32_t), 0));
+  list_rows_touching(A, keep_row, sel, list.data(), count.data());
+  hipLaunchKernelGGL(k_shift_listed, dim3(grid_capped(A.rows, kBlock, 4096)), dim3(kBlock), 0, 0, A, list.data(),
+                     count.data(), sel, val, AL, AU);
+  GP_LAUNCH_CHECK("shift listed rows");
 }
 
 // ---- live statistics ---------------------------------------------------------------------

@@ -80,22 +80,6 @@ __global__ void k_fixed_filter(gp_csr AT, const uint8_t* keep_row, const double*
 }
 
 // row_shift[i] = sum over entries with fixed[col] of a * val[col] (CSR order)
-__global__ void k_row_shift(gp_csr A, const uint8_t* keep_row, const uint8_t* fixed, const double* val,
-                            double* AL, double* AU) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= A.rows || !keep_row[i]) return;
-  RowAcc acc;
-  const gp_offset p0 = A.row_ptr[i];
-  for (gp_offset p = p0; p < A.row_ptr[i + 1]; ++p) {
-    acc.pos(p - p0);
-    const int32_t j = A.col_idx[p];
-    if (fixed[j]) acc.add(A.vals[p] * val[j]);
-  }
-  const double s = acc.value();
-  AL[i] -= s;
-  AU[i] -= s;
-}
-
 __global__ void k_fix_columns(int32_t n, const uint8_t* fixed, const double* val, double* l, double* u,
                               uint8_t* keep_col) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -135,8 +119,7 @@ void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params
   const double obj = fixed_variable_records(tape_on(plan, p) && p.record_postsolve_tape_cpu ? plan_tape(plan) : nullptr,
                                             fixed.data(), val.data(), plan.keep_row_mask, plan.new_c, lp.AT, n, true);
   if (m > 0) {
-    hipLaunchKernelGGL(k_row_shift, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, fixed.data(),
-                       val.data(), plan.new_AL, plan.new_AU);
+    shift_rows_by_columns(lp.A, plan.keep_row_mask, fixed.data(), val.data(), plan.new_AL, plan.new_AU);
     GP_LAUNCH_CHECK("fixed row shift");
   }
   hipLaunchKernelGGL(k_fix_columns, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, fixed.data(), val.data(), plan.new_l,

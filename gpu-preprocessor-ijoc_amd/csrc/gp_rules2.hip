@@ -34,24 +34,6 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
-// AL/AU -= sum over stored entries of kept row i whose column has sel[col] of
-// a * val[col] (blocked CSR order).
-__global__ void k_shift_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* sel, const double* val, double* AL,
-                             double* AU) {
-  const int64_t i = gtid();
-  if (i >= A.rows || !keep_row[i]) return;
-  RowAcc acc;
-  const gp_offset p0 = A.row_ptr[i];
-  for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
-    acc.pos(q - p0);
-    const int32_t j = A.col_idx[q];
-    if (sel[j]) acc.add(A.vals[q] * val[j]);
-  }
-  const double s = acc.value();
-  AL[i] -= s;
-  AU[i] -= s;
-}
-
 // ============================================================== singleton columns
 enum : uint8_t { SC_NONE = 0, SC_SET_ROW_SIDE = 1, SC_TIGHTEN_LHS = 2, SC_TIGHTEN_RHS = 3, SC_ELIM_FREE = 4,
                  SC_ELIM_ABOVE = 5, SC_ELIM_BELOW = 6 };
@@ -555,15 +537,22 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
   }
 }
 
-__global__ void k_df_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* ffix, const uint8_t* inf,
-                          const double* val, double* shift, uint8_t* row_remove, uint8_t* keep_after,
-                          int32_t* flags) {
-  const int64_t i = gtid();
-  if (i >= A.rows) return;
-  double s = 0.0;
-  uint8_t rem = 0;
-  if (keep_row[i]) {
+// The row pass only matters for rows that meet a fixed or infinite-bound
+// column, usually few: list_rows_touching lists them and only the listed rows
+// take the sequential fold.  Every other row keeps shift 0, no removal.
+__global__ void k_df_touch(int32_t n, const uint8_t* ffix, const uint8_t* inf, uint8_t* touched) {
+  const int64_t j = gtid();
+  if (j < n) touched[j] = (ffix[j] || inf[j]) ? 1 : 0;
+}
+
+__global__ void k_df_rows_list(gp_csr A, const int32_t* list, const int32_t* count, const uint8_t* ffix,
+                               const uint8_t* inf, const double* val, double* shift, uint8_t* row_remove,
+                               uint8_t* keep_after, int32_t* flags) {
+  const int32_t total = *count;
+  for (int64_t k = gtid(); k < total; k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = list[k];
     RowAcc acc;
+    uint8_t rem = 0;
     const gp_offset p0 = A.row_ptr[i];
     for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
       acc.pos(q - p0);
@@ -571,12 +560,11 @@ __global__ void k_df_rows(gp_csr A, const uint8_t* keep_row, const uint8_t* ffix
       if (ffix[j]) acc.add(A.vals[q] * val[j]);
       if (inf[j]) rem = 1;
     }
-    s = acc.value();
     if (rem) wave_flag(&flags[3], 1);
+    shift[i] = acc.value();
+    row_remove[i] = rem;
+    keep_after[i] = rem ? 0 : 1;
   }
-  shift[i] = s;
-  row_remove[i] = rem;
-  keep_after[i] = (keep_row[i] && !rem) ? 1 : 0;
 }
 
 __device__ inline int64_t entry_len(const gp_csr& A, int32_t r, const uint8_t* entry) {
@@ -1653,8 +1641,18 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
   DVec<double> shift(m ? m : 1);
   DVec<uint8_t> row_remove(m ? m : 1), keep_after(m ? m : 1);
   if (m > 0) {
-    hipLaunchKernelGGL(k_df_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, ffix.data(),
-                       inf.data(), val.data(), shift.data(), row_remove.data(), keep_after.data(), flags.d.data());
+    GP_CHECK(hipMemsetAsync(shift.data(), 0, sizeof(double) * static_cast<size_t>(m), 0));  // +0.0
+    fill_u8(row_remove.data(), 0, m);
+    d2d(keep_after.data(), plan.keep_row_mask, static_cast<size_t>(m));
+    DVec<uint8_t> touched(n);
+    hipLaunchKernelGGL(k_df_touch, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, ffix.data(), inf.data(),
+                       touched.data());
+    DVec<int32_t> list(m);
+    int32_t* count = flags.d.data() + 4;
+    list_rows_touching(lp.A, plan.keep_row_mask, touched.data(), list.data(), count);
+    hipLaunchKernelGGL(k_df_rows_list, dim3(grid_capped(m, kBlock, 4096)), dim3(kBlock), 0, 0, lp.A, list.data(),
+                       count, ffix.data(), inf.data(), val.data(), shift.data(), row_remove.data(), keep_after.data(),
+                       flags.d.data());
     GP_LAUNCH_CHECK("dual fix rows");
   }
   const bool row_changed = flags.read()[3] != 0;
@@ -1809,8 +1807,7 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
     obj = fixed_variable_records(tape_on(plan, p) ? plan_tape(plan) : nullptr, fixed.data(), fval.data(),
                                  plan.keep_row_mask, plan.new_c, lp.AT, n, true);
     if (m > 0) {
-      hipLaunchKernelGGL(k_shift_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, fixed.data(),
-                         fval.data(), plan.new_AL, plan.new_AU);
+      shift_rows_by_columns(lp.A, plan.keep_row_mask, fixed.data(), fval.data(), plan.new_AL, plan.new_AU);
       GP_LAUNCH_CHECK("parallel columns row shift");
     }
     plan.has_row_action = 1;

@@ -118,7 +118,12 @@ def test_learned_presolve_matches_restatement(name, lp):
     assert np.array_equal(red.vals.view(np.uint64), ref.reduced.vals.view(np.uint64))
 
 
-@pytest.mark.parametrize("name,lp", lps(), ids=[x[0] for x in lps()])
+def gap_lps():
+    return [("qap6", G.generate_qap(6, seed=105)),
+            ("gen2k", G.generate(2000, 3000, seed=107, degree_mode=0, mean_degree=6.0))]
+
+
+@pytest.mark.parametrize("name,lp", gap_lps(), ids=[x[0] for x in gap_lps()])
 def test_learned_postsolve_objective_gap(name, lp):
     """Reduced LP of the learned schedule, solved by HiGHS and postsolved on the
     GPU, matches HiGHS on the original within 1e-6 (relative), as the plain

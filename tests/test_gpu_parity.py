@@ -67,6 +67,34 @@ def test_transpose_zipf_medium():
     assert_same(h.at[2], va, "AT vals")
 
 
+def test_transpose_dense_and_long_columns():
+    """Columns of every length class of the transpose's segment sorts: short
+    (<= 32 entries), long (33..8192) and dense (> 8192, radix-sorted)."""
+    rng = np.random.default_rng(5)
+    m, n = 20000, 300
+    rows, cols = [np.arange(m)], [np.zeros(m, dtype=np.int64)]          # dense column 0
+    for j in range(1, 40):                                              # long columns
+        r = rng.choice(m, size=int(rng.integers(33, 3000)), replace=False)
+        rows.append(r)
+        cols.append(np.full(r.size, j))
+    for j in range(40, n):                                              # short columns
+        r = rng.choice(m, size=int(rng.integers(0, 33)), replace=False)
+        rows.append(r)
+        cols.append(np.full(r.size, j))
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((rng.normal(size=r.size), (r, c)), shape=(m, n))
+    A.sort_indices()
+    lp = G.HostLP(m, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy(), np.zeros(n),
+                  np.zeros(n), np.ones(n), np.full(m, -np.inf), np.full(m, np.inf))
+    with G.DeviceLP(lp) as d:
+        h = G.device_lp_to_host(d.lp)
+    rp, ci, va = lp.transpose()
+    assert_same(h.at[0], rp, "AT row_ptr")
+    assert_same(h.at[1], ci, "AT col_idx")
+    assert_same(h.at[2], va, "AT vals")
+
+
 def test_activity_dyadic_exact():
     """Mixed row lengths with dyadic data: sums are exact in any order
     (test_gpu_presolve_kernels.cpp:49-137)."""
