@@ -58,12 +58,13 @@ def pct(v, q):
 
 def batched_leg(count, seed, serial=True):
     """Batched presolve of `count` LPs (rows log-uniform in [1e3, 1e5], columns
-    1.5 x rows, uniform degree 6) resident in HBM: one gp_preprocess_batch
-    against the plain path presolving the same LPs one after another."""
+    1.5 x rows, uniform degree 6, integer-valued FP64 data as in the large
+    workload) resident in HBM: one gp_preprocess_batch against the plain path
+    presolving the same LPs one after another."""
     import gpupre as G
 
     t = time.time()
-    lps = G.batch_workload(count, seed)
+    lps = G.batch_workload(count, seed, integral=1)
     dl = [G.DeviceLP(lp) for lp in lps]
     nnz = sum(lp.nnz for lp in lps)
     log(f"batched workload: {count} LPs, {sum(lp.m for lp in lps)} rows, {nnz} nnz "
@@ -85,7 +86,8 @@ def batched_leg(count, seed, serial=True):
         b = G.preprocess_batch(dl, params)
         walls.append(time.perf_counter() - t)
         w = b.whole()
-        red = (w.reduced_rows, w.reduced_cols, w.reduced_nnz, w.iterations)
+        red = (w.reduced_rows, w.reduced_cols, w.reduced_nnz, w.iterations,
+               "infeasible" if w.has_infeasible else "unbounded" if w.has_unbounded else "ok")
         asm, pre = b.s.assemble_seconds, b.s.presolve_seconds
         b.free()
     batch_s = min(walls)
@@ -108,7 +110,7 @@ def batched_leg(count, seed, serial=True):
     return {"lps": count, "nnz": nnz, "lps_per_s": round(count / batch_s, 2), "ms": round(batch_s * 1e3, 2),
             "assembly_ms": round(asm * 1e3, 2), "latency_ms_p50": round(batch_s * 1e3, 2),
             "latency_ms_p99": round(batch_s * 1e3, 2),
-            "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3]},
+            "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3], "status": red[4]},
             "plain_serial": {"lps_per_s": round(count / serial_s, 2), "ms": round(serial_s * 1e3, 2),
                              "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)} if serial else None,
             "speedup_vs_plain": round(serial_s / batch_s, 2) if serial else None,

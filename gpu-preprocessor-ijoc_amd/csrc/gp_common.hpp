@@ -519,6 +519,20 @@ inline int lanes_for_mean(double mean) {
   if (mean <= 64.0) return 16;
   return 64;
 }
+// Launch K<G>::kernel with G lanes per row (G in {2, 4, 8, 16, 64}) over `rows`
+// rows in a grid-stride loop.
+template <template <int> class K, class... Args>
+void launch_grouped(int lanes, int64_t rows, Args... args) {
+  const unsigned grid = grid_capped(rows * lanes, kBlock, 16384);
+  switch (lanes) {
+    case 2: hipLaunchKernelGGL(K<2>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 4: hipLaunchKernelGGL(K<4>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 8: hipLaunchKernelGGL(K<8>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    case 16: hipLaunchKernelGGL(K<16>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+    default: hipLaunchKernelGGL(K<64>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
+  }
+}
+
 // the tuned lane count when one is set, else the mean-length rule
 inline int pick_lanes(int32_t knob, double mean) { return knob > 0 ? knob : lanes_for_mean(mean); }
 

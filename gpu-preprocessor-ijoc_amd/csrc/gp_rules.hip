@@ -634,17 +634,6 @@ __global__ void k_cm_refresh(gp_csr AT, Bits keep_row, const int32_t* list, int3
   vmax[j] = v;
 }
 
-template <template <int> class K, class... Args>
-void launch_grouped(int lanes, int64_t rows, Args... args) {
-  const unsigned grid = grid_capped(rows * lanes, kBlock, 16384);
-  switch (lanes) {
-    case 2: hipLaunchKernelGGL(K<2>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
-    case 4: hipLaunchKernelGGL(K<4>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
-    case 8: hipLaunchKernelGGL(K<8>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
-    case 16: hipLaunchKernelGGL(K<16>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
-    default: hipLaunchKernelGGL(K<64>::kernel, dim3(grid), dim3(kBlock), 0, 0, args...); break;
-  }
-}
 template <int G>
 struct BsCandidatesK {
   static constexpr auto kernel = k_bs_candidates<G>;
@@ -740,35 +729,52 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   fval[j] = fat;
 }
 
-__global__ void k_bs_support(gp_csr AT, Bits keep_row, const int32_t* row_nnz, const double* AL,
-                             const double* AU, const double* l, const double* u, const double* min_fin,
-                             const double* max_fin, const int32_t* min_ninf, const int32_t* max_pinf,
-                             const uint8_t* lch, const uint8_t* uch, const double* fl, const double* fu,
-                             double zero_tol, double tol, double support_tol, int32_t* sl_out, int32_t* su_out,
-                             int64_t* r, int64_t* ci, int64_t* cv) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= AT.rows) return;
-  int32_t sl = INT32_MAX, su = INT32_MAX;
-  const bool need_l = lch[j], need_u = uch[j];
-  if (need_l || need_u) {
-    const double lj = l[j], uj = u[j];
-    for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) {
-      const int32_t i = AT.col_idx[p];
-      if (!keep_row[i] || row_nnz[i] <= 1) continue;
-      const double a = AT.vals[p];
-      if (fabs(a) <= zero_tol) continue;
-      double il, iu;
-      implied_bounds(a, lj, uj, AL[i], AU[i], min_fin[i], min_ninf[i], max_fin[i], max_pinf[i], il, iu);
-      if (need_l && fl[j] > lj + tol && isfinite(il) && fabs(il - fl[j]) <= support_tol) sl = i < sl ? i : sl;
-      if (need_u && fu[j] < uj - tol && isfinite(iu) && fabs(iu - fu[j]) <= support_tol) su = i < su ? i : su;
+// Support row of each tightened bound: the smallest live row whose implied
+// bound matches.  G lanes per column take partial minima (order-free).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_bs_support(
+    gp_csr AT, Bits keep_row, const int32_t* row_nnz, const double* AL, const double* AU, const double* l,
+    const double* u, const double* min_fin, const double* max_fin, const int32_t* min_ninf, const int32_t* max_pinf,
+    const uint8_t* lch, const uint8_t* uch, const double* fl, const double* fu, double zero_tol, double tol,
+    double support_tol, int32_t* sl_out, int32_t* su_out, int64_t* r, int64_t* ci, int64_t* cv) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; j < AT.rows; j += groups) {
+    int32_t sl = INT32_MAX, su = INT32_MAX;
+    const bool need_l = lch[j], need_u = uch[j];
+    if (need_l || need_u) {
+      const double lj = l[j], uj = u[j], flj = fl[j], fuj = fu[j];
+      const gp_offset p1 = AT.row_ptr[j + 1];
+      for (gp_offset p = AT.row_ptr[j] + lane; p < p1; p += G) {
+        const int32_t i = AT.col_idx[p];
+        if (!keep_row[i] || row_nnz[i] <= 1) continue;
+        const double a = AT.vals[p];
+        if (fabs(a) <= zero_tol) continue;
+        double il, iu;
+        implied_bounds(a, lj, uj, AL[i], AU[i], min_fin[i], min_ninf[i], max_fin[i], max_pinf[i], il, iu);
+        if (need_l && flj > lj + tol && isfinite(il) && fabs(il - flj) <= support_tol) sl = i < sl ? i : sl;
+        if (need_u && fuj < uj - tol && isfinite(iu) && fabs(iu - fuj) <= support_tol) su = i < su ? i : su;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      const int32_t a = __shfl_xor(sl, o, G), b = __shfl_xor(su, o, G);
+      sl = a < sl ? a : sl;
+      su = b < su ? b : su;
+    }
+    if (lane == 0) {
+      sl_out[j] = sl;
+      su_out[j] = su;
+      r[j] = (need_l ? 1 : 0) + (need_u ? 1 : 0);
+      ci[j] = (need_l ? (sl == INT32_MAX ? 1 : 2) : 0) + (need_u ? (su == INT32_MAX ? 1 : 2) : 0);
+      cv[j] = 4 * r[j];
     }
   }
-  sl_out[j] = sl;
-  su_out[j] = su;
-  r[j] = (need_l ? 1 : 0) + (need_u ? 1 : 0);
-  ci[j] = (need_l ? (sl == INT32_MAX ? 1 : 2) : 0) + (need_u ? (su == INT32_MAX ? 1 : 2) : 0);
-  cv[j] = 4 * r[j];
 }
+template <int G>
+struct BsSupportK {
+  static constexpr auto kernel = k_bs_support<G>;
+};
 
 __global__ void k_bs_tape_pack(TapeView out, int32_t n, const uint8_t* lch, const uint8_t* uch, const int32_t* sl,
                                const int32_t* su, const double* l, const double* u, const double* fl,
@@ -934,10 +940,16 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     const double support_tol = cmax(10.0 * p.bound_tol, 1.0e-10);
     DVec<int32_t> sl(n), su(n);
     AppendCounts cnt(n);
-    hipLaunchKernelGGL(k_bs_support, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, rows_all.view(), s.row_nnz,
-                       plan.new_AL, plan.new_AU, plan.new_l, plan.new_u, min_fin.data(), max_fin.data(),
-                       min_ninf.data(), max_pinf.data(), lch.data(), uch.data(), fl.data(), fu.data(), p.zero_tol,
-                       p.bound_tol, support_tol, sl.data(), su.data(), cnt.r.data(), cnt.i.data(), cnt.v.data());
+    launch_grouped<BsSupportK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
+                               rows_all.view(), static_cast<const int32_t*>(s.row_nnz),
+                               static_cast<const double*>(plan.new_AL), static_cast<const double*>(plan.new_AU),
+                               static_cast<const double*>(plan.new_l), static_cast<const double*>(plan.new_u),
+                               static_cast<const double*>(min_fin.data()), static_cast<const double*>(max_fin.data()),
+                               static_cast<const int32_t*>(min_ninf.data()),
+                               static_cast<const int32_t*>(max_pinf.data()), static_cast<const uint8_t*>(lch.data()),
+                               static_cast<const uint8_t*>(uch.data()), static_cast<const double*>(fl.data()),
+                               static_cast<const double*>(fu.data()), p.zero_tol, p.bound_tol, support_tol, sl.data(),
+                               su.data(), cnt.r.data(), cnt.i.data(), cnt.v.data());
     GP_LAUNCH_CHECK("bound support rows");
     cnt.scan(n);
     if (cnt.R > 0) {

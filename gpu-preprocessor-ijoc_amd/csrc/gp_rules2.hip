@@ -1091,15 +1091,29 @@ __global__ void k_pc_apply(int32_t n, const uint8_t* col_delete, const uint8_t* 
 constexpr double kDblMaxRatioPivot = 1.0e3;
 constexpr int kDblMatchingRounds = 8;
 
-__global__ void k_db_colnnz(gp_csr AT, const uint8_t* keep_row, const uint8_t* keep_col, double zt, int32_t* nnz) {
-  const int64_t j = gtid();
-  if (j >= AT.rows) return;
-  int32_t c = 0;
-  if (keep_col[j])
-    for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q)
-      c += (keep_row[AT.col_idx[q]] && fabs(AT.vals[q]) > zt) ? 1 : 0;
-  nnz[j] = c;
+// live entries (|a| > zt, live row) per live column: G lanes per column, row
+// liveness from the packed row mask
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_db_colnnz(gp_csr AT, Bits keep_row, const uint8_t* keep_col, double zt,
+                                                       int32_t* nnz) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t j = gtid() / G; j < AT.rows; j += groups) {
+    int32_t c = 0;
+    if (keep_col[j]) {
+      const gp_offset p1 = AT.row_ptr[j + 1];
+      for (gp_offset q = AT.row_ptr[j] + lane; q < p1; q += G)
+        c += (keep_row[ld_stream(AT.col_idx + q)] && fabs(ld_stream(AT.vals + q)) > zt) ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
+    if (lane == 0) nnz[j] = c;
+  }
 }
+template <int G>
+struct DbColNnzK {
+  static constexpr auto kernel = k_db_colnnz<G>;
+};
 
 __device__ inline bool db_integral(double num, double den, double tol) {
   if (fabs(den) <= tol) return false;
@@ -1838,8 +1852,9 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   fill_u8(selected.data(), 0, m);
   fill_u8(blocked.data(), 0, n);
   DbCand cand{ce.data(), ck.data(), cev.data(), ckv.data()};
-  hipLaunchKernelGGL(k_db_colnnz, dim3(grid_for(n)), dim3(kBlock), 0, 0, AT, plan.keep_row_mask, plan.keep_col_mask,
-                     zt, col_nnz.data());
+  const PackedMask live_rows(plan.keep_row_mask, m);
+  launch_grouped<DbColNnzK>(lanes_for_mean(n ? static_cast<double>(AT.nnz) / n : 0.0), n, AT, live_rows.view(),
+                            static_cast<const uint8_t*>(plan.keep_col_mask), zt, col_nnz.data());
   GP_LAUNCH_CHECK("doubleton column counts");
   hipLaunchKernelGGL(k_db_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, AT, plan.keep_row_mask,
                      plan.keep_col_mask, plan.new_AL, plan.new_AU, col_nnz.data(), zt, bt, ratio_tol,

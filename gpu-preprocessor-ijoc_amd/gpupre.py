@@ -669,10 +669,11 @@ def _take_lpgen(g, raw) -> HostLP:
 
 
 def batch_workload(count: int = 512, seed: int = 0, min_rows: int = 1000, max_rows: int = 100_000,
-                   threads: int = 8) -> list:
+                   threads: int = 8, integral: int = 0) -> list:
     """The batched-presolve workload: `count` independent LPs of the synthetic
     generator (uniform column degree 6, planted structure), rows log-uniform in
-    [min_rows, max_rows], columns 1.5 x rows; instance k uses seed seed*100003+k."""
+    [min_rows, max_rows], columns 1.5 x rows; instance k uses seed seed*100003+k.
+    integral=1 draws integer coefficients, bounds and planted point."""
     import concurrent.futures as cf
 
     rng = np.random.default_rng(seed)
@@ -680,7 +681,7 @@ def batch_workload(count: int = 512, seed: int = 0, min_rows: int = 1000, max_ro
 
     def one(k):
         m = int(rows[k])
-        return generate(m, m * 3 // 2, seed * 100003 + k, degree_mode=0, mean_degree=6.0)
+        return generate(m, m * 3 // 2, seed * 100003 + k, degree_mode=0, mean_degree=6.0, integral=integral)
 
     with cf.ThreadPoolExecutor(max_workers=threads) as pool:
         return list(pool.map(one, range(count)))
