@@ -24,9 +24,9 @@ HBM_PEAK_GBPS = 8000.0
 
 # kernel-timing family -> the one kernel it brackets (rocprofv3 kernel name)
 KERNEL_OF_FAMILY = {
-    "activity": "k_row_fold<ActivityFold>",
-    "sc_activity": "k_row_fold<ScActivityFold>",
-    "redundant": "k_row_fold<RedundantFold>",
+    "activity": "k_row_fold<ActivityFold, 256, 2048>",
+    "sc_activity": "k_row_fold<ScActivityFold, 256, 2048>",
+    "redundant": "k_row_fold<RedundantFold, 256, 2048>",
     "bound_candidates": "k_bs_candidates",
     "col_max": "k_col_max_abs",
 }

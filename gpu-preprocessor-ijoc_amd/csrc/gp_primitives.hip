@@ -569,37 +569,77 @@ struct ColRank {
   }
 };
 
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_compact_copy(const gp_offset* rp, const int32_t* col, const double* val,
-                                                          int32_t m, const int32_t* row_o2n, ColRank col_rank,
-                                                          const int64_t* excl, int32_t* out_col, double* out_val) {
-  const int lane = threadIdx.x % G;
-  const int wl = static_cast<int>(__lane_id());
-  const unsigned long long group_bits = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (wl - lane);
-  const unsigned long long below = (1ull << wl) - 1ull;
-  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
-  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
-    if (row_o2n && row_o2n[i] < 0) continue;
-    int64_t q = excl[i];
-    const gp_offset p1 = rp[i + 1];
-    for (gp_offset base = rp[i]; base < p1; base += G) {
-      const gp_offset p = base + lane;
+// Entry-parallel stable compaction.  The output is the kept entries in input
+// order, so a kept entry's position is the number of kept entries before it:
+// each workgroup takes a tile of kCompactTile consecutive entries, starts from
+// the kept count before the tile (the row offset of the tile's first row plus
+// that row's kept entries before the tile) and adds a block prefix of the keep
+// flags.  Reads and writes are contiguous per pass; row ids come from LDS.
+constexpr int kCompactTile = 4096;
+__global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
+                                                       int32_t m, int64_t nnz, const uint8_t* keep_row,
+                                                       ColRank col_rank, const int64_t* excl, int32_t* out_col,
+                                                       double* out_val) {
+  __shared__ int32_t row_of[kCompactTile];
+  __shared__ int32_t s_rows[2];
+  __shared__ int32_t s_part[4];
+  __shared__ int32_t s_wave[4];
+  const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
+  for (int64_t t0 = static_cast<int64_t>(blockIdx.x) * kCompactTile; t0 < nnz;
+       t0 += static_cast<int64_t>(gridDim.x) * kCompactTile) {
+    const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
+    if (threadIdx.x < 2) {  // the rows holding entries t0 and t1 - 1: last i with rp[i] <= key
+      const int64_t key = threadIdx.x == 0 ? t0 : t1 - 1;
+      int32_t lo = 0, hi = m - 1;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (rp[mid] <= key) lo = mid;
+        else hi = mid - 1;
+      }
+      s_rows[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const int32_t i0 = s_rows[0], i1 = s_rows[1];
+    for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r <= i1; r += 256) {
+      const int64_t a = rp[r] > t0 ? rp[r] : t0, b = rp[r + 1] < t1 ? rp[r + 1] : t1;
+      for (int64_t p = a; p < b; ++p) row_of[p - t0] = r;
+    }
+    // kept entries of row i0 before the tile
+    int32_t before = 0;
+    if (!keep_row || keep_row[i0])
+      for (int64_t p = rp[i0] + threadIdx.x; p < t0; p += 256) before += col_rank(col[p]) >= 0 ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
+    if (lane == 0) s_part[wave] = before;
+    __syncthreads();
+    int64_t running = excl[i0] + s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    for (int k = 0; k < kCompactTile / 256; ++k) {
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
       int32_t j = -1;
-      double v = 0.0;
-      if (p < p1) {
-        j = col_rank(col[p]);
-        v = val[p];
+      if (p < t1) {
+        const int32_t r = row_of[p - t0];
+        const int32_t c = ld_stream(col + p);
+        if (!keep_row || keep_row[r]) j = col_rank(c);
       }
-      const unsigned long long keep = __ballot(j >= 0) & group_bits;
+      const unsigned long long bal = __ballot(j >= 0);
+      if (lane == 0) s_wave[wave] = __popcll(bal);
+      __syncthreads();
+      int32_t off = 0, total = 0;
+      for (int w = 0; w < 4; ++w) {
+        off += w < wave ? s_wave[w] : 0;
+        total += s_wave[w];
+      }
       if (j >= 0) {
-        const int64_t at = q + __popcll(keep & below);
+        const int64_t at = running + off + __popcll(bal & ((1ull << lane) - 1ull));
         out_col[at] = j;
-        out_val[at] = v;
+        out_val[at] = ld_stream(val + p);
       }
-      q += __popcll(keep);
+      running += total;
+      __syncthreads();
     }
   }
 }
+static_assert(kBlock == 256, "k_compact_tiles assumes 256-lane workgroups");
+
 __global__ void k_set_last(gp_offset* rp, int32_t at, gp_offset v) { rp[at] = v; }
 __global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
   const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -649,16 +689,13 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   if (m > 0) {
     hipLaunchKernelGGL(k_compact_rowptr, dim3(grid_for(m)), dim3(kBlock), 0, 0, cnt.data(), row_o2n, m, out.row_ptr);
     GP_LAUNCH_CHECK("compact rowptr");
-    const int lanes = pick_lanes(tuning().lanes_compact, static_cast<double>(A.nnz) / m);
-    const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
-    switch (lanes) {
-      case 2: hipLaunchKernelGGL(k_compact_copy<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
-      case 4: hipLaunchKernelGGL(k_compact_copy<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
-      case 8: hipLaunchKernelGGL(k_compact_copy<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
-      case 16: hipLaunchKernelGGL(k_compact_copy<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
-      default: hipLaunchKernelGGL(k_compact_copy<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, row_o2n, rank, cnt.data(), out.col_idx, out.vals); break;
+    if (A.nnz > 0) {
+      const int64_t tiles = (static_cast<int64_t>(A.nnz) + kCompactTile - 1) / kCompactTile;
+      hipLaunchKernelGGL(k_compact_tiles, dim3(static_cast<unsigned>(tiles < 16384 ? tiles : 16384)), dim3(256), 0, 0,
+                         A.row_ptr, A.col_idx, A.vals, m, static_cast<int64_t>(A.nnz), keep_row, rank, cnt.data(),
+                         out.col_idx, out.vals);
+      GP_LAUNCH_CHECK("compact copy");
     }
-    GP_LAUNCH_CHECK("compact copy");
   }
   hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, 0, out.row_ptr, m_new, static_cast<gp_offset>(total));
   GP_LAUNCH_CHECK("compact last");
@@ -703,7 +740,17 @@ __global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, G);
-    if (lane == 0 && hit) list[atomicAdd(count, 1)] = static_cast<int32_t>(i);
+    // one atomic per wavefront: the group leaders with a hit take consecutive slots
+    const bool put = lane == 0 && hit;
+    const unsigned long long bal = __ballot(put);
+    const int wl = static_cast<int>(__lane_id());
+    int32_t base = 0;
+    if (bal) {
+      const int first = __ffsll(bal) - 1;
+      if (wl == first) base = atomicAdd(count, static_cast<int32_t>(__popcll(bal)));
+      base = __shfl(base, first, 64);
+    }
+    if (put) list[base + __popcll(bal & ((1ull << wl) - 1ull))] = static_cast<int32_t>(i);
   }
 }
 
