@@ -1370,12 +1370,19 @@ __global__ void k_db_pair_of(DbPairs P, DbCand cand, int32_t* pair_of) {
 
 // Row shift of kept row i: a * beta for eliminated columns (other rows of the
 // pair) and a * fixed value for fixed keep columns, live entries, CSR order.
-__global__ void k_db_shift(gp_csr A, const uint8_t* keep_row, const int32_t* pair_of, DbPairs P, DbCand cand,
-                           const double* AU, const double* l, double zt, double* shift) {
-  const int64_t i = gtid();
-  if (i >= A.rows) return;
-  double s = 0.0;
-  if (keep_row[i]) {
+// Row shift of the substitution / fixing: only rows with an entry in a paired
+// column contribute (rows are listed by list_rows_touching; the rest keep
+// shift 0); each listed row is folded in CSR order.
+__global__ void k_db_pair_mark(int32_t n, const int32_t* pair_of, uint8_t* mark) {
+  const int64_t j = gtid();
+  if (j < n) mark[j] = pair_of[j] >= 0 ? 1 : 0;
+}
+
+__global__ void k_db_shift(gp_csr A, const int32_t* list, const int32_t* count, const int32_t* pair_of, DbPairs P,
+                           DbCand cand, const double* AU, const double* l, double zt, double* shift) {
+  const int32_t total = *count;
+  for (int64_t k = gtid(); k < total; k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = list[k];
     RowAcc acc;
     const gp_offset p0 = A.row_ptr[i];
     for (gp_offset q = p0; q < A.row_ptr[i + 1]; ++q) {
@@ -1391,9 +1398,8 @@ __global__ void k_db_shift(gp_csr A, const uint8_t* keep_row, const int32_t* pai
         acc.add(a * l[cand.keep[prow]]);
       }
     }
-    s = acc.value();
+    shift[i] = acc.value();
   }
-  shift[i] = s;
 }
 
 __global__ void k_db_entry_counts(DbPairs P, DbCand cand, gp_csr AT, const uint8_t* keep_row, double zt,
@@ -1436,35 +1442,85 @@ __global__ void k_db_delta_rows(int64_t total, const uint64_t* keys, int32_t n, 
 
 // Merge of a source row with its sorted deltas (rule_doubleton_equations.cu:1242-1335):
 // equal columns are summed (source entries first), |sum| <= zero_tol dropped.
-template <bool kWrite>
-__global__ void k_db_merge(gp_csr A, const int32_t* drp, const uint64_t* keys, const double* dvals, int32_t n,
-                           double zt, int32_t* counts, gp_offset* out_ptr, int32_t* out_col, double* out_val) {
-  const int64_t i = gtid();
-  if (i >= A.rows) return;
-  gp_offset pp = A.row_ptr[i];
-  const gp_offset pe = A.row_ptr[i + 1];
-  int64_t qq = drp[i];
-  const int64_t qe = drp[i + 1];
-  int64_t w = kWrite ? out_ptr[i] : 0;
-  int32_t c = 0;
-  while (pp < pe || qq < qe) {
-    const int32_t sc = pp < pe ? A.col_idx[pp] : INT_MAX;
-    const int32_t dc = qq < qe ? static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) : INT_MAX;
-    const int32_t next = sc < dc ? sc : dc;
-    double merged = 0.0;
-    while (pp < pe && A.col_idx[pp] == next) merged += A.vals[pp++];
-    while (qq < qe && static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) == next) merged += dvals[qq++];
-    if (fabs(merged) > zt) {
-      if (kWrite) {
-        out_col[w] = next;
-        out_val[w] = merged;
-        ++w;
-      } else {
-        ++c;
+// Row merge of A with the sorted delta entries.  G lanes per row: a row without
+// deltas is a stable |a| > zt filter (group ballot prefix); a row with deltas is
+// merged sequentially by the group's first lane (equal columns summed A first,
+// then the deltas in key order).
+template <bool kWrite, int G>
+__global__ void __launch_bounds__(kBlock) k_db_merge(gp_csr A, const int32_t* drp, const uint64_t* keys,
+                                                      const double* dvals, int32_t n, double zt, int32_t* counts,
+                                                      const gp_offset* out_ptr, int32_t* out_col, double* out_val) {
+  const int lane = threadIdx.x % G;
+  const int wl = static_cast<int>(__lane_id());
+  const unsigned long long group_bits = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (wl - lane);
+  const unsigned long long below = (1ull << wl) - 1ull;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = gtid() / G; i < A.rows; i += groups) {
+    gp_offset pp = A.row_ptr[i];
+    const gp_offset pe = A.row_ptr[i + 1];
+    int64_t qq = drp[i];
+    const int64_t qe = drp[i + 1];
+    if (qq == qe) {
+      int64_t w = kWrite ? out_ptr[i] : 0;
+      int32_t c = 0;
+      for (gp_offset base = pp; base < pe; base += G) {
+        const gp_offset p = base + lane;
+        int32_t col = 0;
+        double v = 0.0;
+        bool keep = false;
+        if (p < pe) {
+          col = A.col_idx[p];
+          v = A.vals[p];
+          keep = fabs(v) > zt;
+        }
+        const unsigned long long bits = __ballot(keep) & group_bits;
+        if (kWrite && keep) {
+          const int64_t at = w + __popcll(bits & below);
+          out_col[at] = col;
+          out_val[at] = v;
+        }
+        w += __popcll(bits);
+        c += __popcll(bits);
+      }
+      if (!kWrite && lane == 0) counts[i] = c;
+      continue;
+    }
+    if (lane != 0) continue;
+    int64_t w = kWrite ? out_ptr[i] : 0;
+    int32_t c = 0;
+    while (pp < pe || qq < qe) {
+      const int32_t sc = pp < pe ? A.col_idx[pp] : INT_MAX;
+      const int32_t dc = qq < qe ? static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) : INT_MAX;
+      const int32_t next = sc < dc ? sc : dc;
+      double merged = 0.0;
+      while (pp < pe && A.col_idx[pp] == next) merged += A.vals[pp++];
+      while (qq < qe && static_cast<int32_t>(keys[qq] % static_cast<uint64_t>(n)) == next) merged += dvals[qq++];
+      if (fabs(merged) > zt) {
+        if (kWrite) {
+          out_col[w] = next;
+          out_val[w] = merged;
+          ++w;
+        } else {
+          ++c;
+        }
       }
     }
+    if (!kWrite) counts[i] = c;
   }
-  if (!kWrite) counts[i] = c;
+}
+
+template <bool kWrite>
+void launch_db_merge(const gp_csr& A, const int32_t* drp, const uint64_t* keys, const double* dvals, int32_t n,
+                     double zt, int32_t* counts, const gp_offset* out_ptr, int32_t* out_col, double* out_val) {
+  const int lanes = lanes_for_mean(A.rows ? static_cast<double>(A.nnz) / A.rows : 0.0);
+  const unsigned grid = grid_capped(static_cast<int64_t>(A.rows) * lanes, kBlock, 16384);
+  switch (lanes) {
+    case 2: hipLaunchKernelGGL((k_db_merge<kWrite, 2>), dim3(grid), dim3(kBlock), 0, 0, A, drp, keys, dvals, n, zt, counts, out_ptr, out_col, out_val); break;
+    case 4: hipLaunchKernelGGL((k_db_merge<kWrite, 4>), dim3(grid), dim3(kBlock), 0, 0, A, drp, keys, dvals, n, zt, counts, out_ptr, out_col, out_val); break;
+    case 8: hipLaunchKernelGGL((k_db_merge<kWrite, 8>), dim3(grid), dim3(kBlock), 0, 0, A, drp, keys, dvals, n, zt, counts, out_ptr, out_col, out_val); break;
+    case 16: hipLaunchKernelGGL((k_db_merge<kWrite, 16>), dim3(grid), dim3(kBlock), 0, 0, A, drp, keys, dvals, n, zt, counts, out_ptr, out_col, out_val); break;
+    default: hipLaunchKernelGGL((k_db_merge<kWrite, 64>), dim3(grid), dim3(kBlock), 0, 0, A, drp, keys, dvals, n, zt, counts, out_ptr, out_col, out_val); break;
+  }
 }
 
 __global__ void k_i32_to_offset(const int32_t* src, int64_t n, gp_offset* dst) {
@@ -1910,9 +1966,17 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   hipLaunchKernelGGL(k_db_pair_of, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, pair_of.data());
   GP_LAUNCH_CHECK("doubleton pair map");
   DVec<double> shift(m);
-  hipLaunchKernelGGL(k_db_shift, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, plan.keep_row_mask, pair_of.data(), P, cand,
-                     plan.new_AU, plan.new_l, zt, shift.data());
-  GP_LAUNCH_CHECK("doubleton row shift");
+  {
+    GP_CHECK(hipMemsetAsync(shift.data(), 0, sizeof(double) * static_cast<size_t>(m), 0));  // +0.0
+    DVec<uint8_t> mark(n);
+    hipLaunchKernelGGL(k_db_pair_mark, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, pair_of.data(), mark.data());
+    DVec<int32_t> list(m), count(1);
+    GP_CHECK(hipMemsetAsync(count.data(), 0, sizeof(int32_t), 0));
+    list_rows_touching(A, plan.keep_row_mask, mark.data(), list.data(), count.data());
+    hipLaunchKernelGGL(k_db_shift, dim3(grid_capped(m, kBlock, 4096)), dim3(kBlock), 0, 0, A, list.data(), count.data(),
+                       pair_of.data(), P, cand, plan.new_AU, plan.new_l, zt, shift.data());
+    GP_LAUNCH_CHECK("doubleton row shift");
+  }
   if (subst_count > 0) {
     DVec<int64_t> start(static_cast<size_t>(k) + 1);
     hipLaunchKernelGGL(k_db_entry_counts, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, AT, plan.keep_row_mask, zt,
@@ -1932,8 +1996,7 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
       hipLaunchKernelGGL(k_db_delta_rows, dim3(grid_for(total)), dim3(kBlock), 0, 0, total, keys.data(), n, drp.data());
       GP_LAUNCH_CHECK("doubleton delta rows");
       exclusive_scan_i32_total(drp.data(), static_cast<int64_t>(m) + 1);
-      hipLaunchKernelGGL(k_db_merge<false>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, drp.data(), keys.data(),
-                         dvals.data(), n, zt, cnt.data(), nullptr, nullptr, nullptr);
+      launch_db_merge<false>(A, drp.data(), keys.data(), dvals.data(), n, zt, cnt.data(), nullptr, nullptr, nullptr);
       GP_LAUNCH_CHECK("doubleton merge counts");
       GP_CHECK(hipMemsetAsync(cnt.data() + m, 0, sizeof(int32_t), 0));
       const int64_t nnz = exclusive_scan_i32_total(cnt.data(), static_cast<int64_t>(m) + 1);
@@ -1947,8 +2010,8 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
       out.vals = pool_alloc<double>(static_cast<size_t>(nnz));
       hipLaunchKernelGGL(k_i32_to_offset, dim3(grid_for(m + 1)), dim3(kBlock), 0, 0, cnt.data(),
                          static_cast<int64_t>(m) + 1, out.row_ptr);
-      hipLaunchKernelGGL(k_db_merge<true>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, drp.data(), keys.data(),
-                         dvals.data(), n, zt, nullptr, out.row_ptr, out.col_idx, out.vals);
+      launch_db_merge<true>(A, drp.data(), keys.data(), dvals.data(), n, zt, nullptr, out.row_ptr, out.col_idx,
+                            out.vals);
       GP_LAUNCH_CHECK("doubleton merge rows");
       if (plan.has_new_A) {
         GP_CHECK(hipStreamSynchronize(0));
