@@ -438,6 +438,9 @@ int32_t count_u8(const uint8_t* mask, int32_t n);
 
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
+// the histogram / atomic-scatter / segment-sort transpose (also for shapes whose
+// (column, row) key exceeds 64 bits)
+void transpose_csr_scatter(const gp_csr& A, gp_csr& AT);
 // Stable compaction by keep masks (null = keep all); row_old_to_new is the
 // exclusive prefix of keep_row (null with keep_row null).  Column indices are
 // renumbered to their rank among the kept columns.
