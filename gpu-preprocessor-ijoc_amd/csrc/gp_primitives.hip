@@ -222,6 +222,33 @@ void scan_impl(T* data, int64_t n, bool inclusive) {
   GP_LAUNCH_CHECK("tile scan");
 }
 
+// The total of an exclusive scan (last output + last input) is formed on the
+// device, so a scan costs at most one host round trip, when the caller reads it.
+template <class T>
+__global__ void k_save_last(const T* data, int64_t n, T* slot) {
+  *slot = data[n - 1];
+}
+template <class T>
+__global__ void k_scan_total(const T* data, int64_t n, const T* saved, int64_t* out) {
+  *out = static_cast<int64_t>(data[n - 1]) + static_cast<int64_t>(*saved);
+}
+__global__ void k_zero_i64(int64_t* out) { *out = 0; }
+
+template <class T>
+void exclusive_scan_dev_total(T* data, int64_t n, int64_t* dev_total) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(1), 0, 0, dev_total);
+    GP_LAUNCH_CHECK("scan total");
+    return;
+  }
+  DVec<T> saved(1);
+  hipLaunchKernelGGL(k_save_last<T>, dim3(1), dim3(1), 0, 0, static_cast<const T*>(data), n, saved.data());
+  scan_impl<T>(data, n, false);
+  hipLaunchKernelGGL(k_scan_total<T>, dim3(1), dim3(1), 0, 0, static_cast<const T*>(data), n,
+                     static_cast<const T*>(saved.data()), dev_total);
+  GP_LAUNCH_CHECK("scan total");
+}
+
 __global__ void k_mask_to_i32(const uint8_t* keep, int32_t n, int32_t* out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = keep[i] != 0;
@@ -240,17 +267,24 @@ void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total) {
     if (total) *total = 0;
     return;
   }
-  int64_t last = 0;
-  if (total) last = read1(data + n - 1);
-  scan_impl<int64_t>(data, n, false);
-  if (total) *total = read1(data + n - 1) + last;
+  if (!total) {
+    scan_impl<int64_t>(data, n, false);
+    return;
+  }
+  DVec<int64_t> t(1);
+  exclusive_scan_dev_total<int64_t>(data, n, t.data());
+  *total = read1(t.data());
+}
+
+void exclusive_scan_i64_device_total(int64_t* data, int64_t n, int64_t* dev_total) {
+  exclusive_scan_dev_total<int64_t>(data, n, dev_total);
 }
 
 int64_t exclusive_scan_i32_total(int32_t* data, int64_t n) {
   if (n <= 0) return 0;
-  const int32_t last = read1(data + n - 1);
-  scan_impl<int32_t>(data, n, false);
-  return static_cast<int64_t>(read1(data + n - 1)) + last;
+  DVec<int64_t> t(1);
+  exclusive_scan_dev_total<int32_t>(data, n, t.data());
+  return read1(t.data());
 }
 
 int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new) {
@@ -803,7 +837,7 @@ __global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_
     int hit = 0;
     if (!keep_row || keep_row[i]) {
       const gp_offset p1 = A.row_ptr[i + 1];
-      for (gp_offset p = A.row_ptr[i] + lane; p < p1 && !hit; p += G) hit = mark[ld_stream(A.col_idx + p)] ? 1 : 0;
+      for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) hit |= mark[A.col_idx[p]] ? 1 : 0;
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, G);

@@ -42,9 +42,15 @@ AppendCounts::AppendCounts(int64_t n)
     : r(static_cast<size_t>(n ? n : 1)), i(static_cast<size_t>(n ? n : 1)), v(static_cast<size_t>(n ? n : 1)) {}
 
 void AppendCounts::scan(int64_t n) {
-  exclusive_scan_i64(r.data(), n, &R);
-  exclusive_scan_i64(i.data(), n, &I);
-  exclusive_scan_i64(v.data(), n, &V);
+  DVec<int64_t> totals(3);
+  exclusive_scan_i64_device_total(r.data(), n, totals.data());
+  exclusive_scan_i64_device_total(i.data(), n, totals.data() + 1);
+  exclusive_scan_i64_device_total(v.data(), n, totals.data() + 2);
+  int64_t h[3];
+  d2h(h, totals.data(), 3);  // one round trip for the three totals
+  R = h[0];
+  I = h[1];
+  V = h[2];
 }
 
 namespace {
