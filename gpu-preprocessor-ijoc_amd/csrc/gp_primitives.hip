@@ -828,32 +828,44 @@ void PackedMask::pack(const uint8_t* mask, int64_t n) {
 
 // ---- rows touching marked columns ------------------------------------------------------------
 namespace {
+// Listed rows are staged in LDS and flushed with one global atomic per
+// workgroup flush, so the list counter is not a hot spot.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_t* keep_row, Bits mark,
                                                            int32_t* list, int32_t* count) {
-  const int lane = threadIdx.x % G;
-  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
-  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < A.rows; i += groups) {
+  constexpr int kRowsPerIter = kBlock / G;
+  constexpr int kStage = 2048;
+  __shared__ int32_t s_rows[kStage];
+  __shared__ int32_t s_n, s_base;
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kRowsPerIter; b0 < A.rows;
+       b0 += static_cast<int64_t>(gridDim.x) * kRowsPerIter) {
+    const int64_t i = b0 + grp;
     int hit = 0;
-    if (!keep_row || keep_row[i]) {
+    if (i < A.rows && (!keep_row || keep_row[i])) {
       const gp_offset p1 = A.row_ptr[i + 1];
       for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) hit |= mark[A.col_idx[p]] ? 1 : 0;
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, G);
-    // one atomic per wavefront: the group leaders with a hit take consecutive slots
-    const bool put = lane == 0 && hit;
-    const unsigned long long bal = __ballot(put);
-    const int wl = static_cast<int>(__lane_id());
-    int32_t base = 0;
-    if (bal) {
-      const int first = __ffsll(bal) - 1;
-      if (wl == first) base = atomicAdd(count, static_cast<int32_t>(__popcll(bal)));
-      base = __shfl(base, first, 64);
+    if (lane == 0 && hit) s_rows[atomicAdd(&s_n, 1)] = static_cast<int32_t>(i);
+    __syncthreads();
+    const int32_t staged = s_n;  // every thread reads the same value before anyone changes it
+    __syncthreads();
+    const bool last = b0 + static_cast<int64_t>(gridDim.x) * kRowsPerIter >= A.rows;
+    if (staged > kStage - kRowsPerIter || (last && staged > 0)) {  // flush (uniform across the block)
+      if (threadIdx.x == 0) s_base = atomicAdd(count, staged);
+      __syncthreads();
+      for (int k = threadIdx.x; k < staged; k += kBlock) list[s_base + k] = s_rows[k];
+      __syncthreads();
+      if (threadIdx.x == 0) s_n = 0;
+      __syncthreads();
     }
-    if (put) list[base + __popcll(bal & ((1ull << wl) - 1ull))] = static_cast<int32_t>(i);
   }
 }
+static_assert(kBlock % 64 == 0, "rows-touching staging assumes whole wavefronts");
 
 template <int G>
 void launch_rows_touching(const gp_csr& A, const uint8_t* keep_row, Bits mark, int32_t* list, int32_t* count) {
