@@ -636,27 +636,6 @@ void csr_pool_free(gp_csr& A) {
 
 // ---- compaction ------------------------------------------------------------------------------
 namespace {
-// G lanes per row (a stable filter: the survivors of a row keep their order).
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_compact_count(const gp_offset* rp, const int32_t* col, int32_t m,
-                                                           const uint8_t* keep_row, Bits keep_col, int64_t* cnt) {
-  const int lane = threadIdx.x % G;
-  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
-  for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
-    int32_t c = 0;
-    if (!keep_row || keep_row[i])
-      for (gp_offset p = rp[i] + lane; p < rp[i + 1]; p += G) c += keep_col[col[p]] ? 1 : 0;
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
-    if (lane == 0) cnt[i] = c;
-  }
-}
-__global__ void k_compact_rowptr(const int64_t* excl, const int32_t* row_o2n, int32_t m, gp_offset* out_rp) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const int32_t r = row_o2n ? row_o2n[i] : static_cast<int32_t>(i);
-  if (r >= 0) out_rp[r] = static_cast<gp_offset>(excl[i]);
-}
 // New index of column j: its rank among the kept columns, from the packed
 // keep bits and a per-word prefix (both L2-resident), instead of an HBM
 // gather into an n-entry old-to-new map.  -1 for a dropped column.
@@ -677,42 +656,71 @@ struct ColRank {
 // that row's kept entries before the tile) and adds a block prefix of the keep
 // flags.  Reads and writes are contiguous per pass; row ids come from LDS.
 constexpr int kCompactTile = 4096;
-__global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
-                                                       int32_t m, int64_t nnz, const uint8_t* keep_row,
-                                                       ColRank col_rank, const int64_t* excl, int32_t* out_col,
-                                                       double* out_val) {
+// tile_row[t] = the row holding entry t * kCompactTile (one lane per row)
+__global__ void k_tile_rows(const gp_offset* rp, int32_t m, int32_t* tile_row) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int64_t a = rp[r], b = rp[r + 1];
+  if (a >= b) return;
+  for (int64_t t = (a + kCompactTile - 1) / kCompactTile; t * kCompactTile < b; ++t)
+    tile_row[t] = static_cast<int32_t>(r);
+}
+// Row ids of the tile's entries into LDS: rows from i0 (the row holding t0) on
+// that start before t1.
+__device__ inline void fill_row_of(const gp_offset* rp, int32_t m, int32_t i0, int64_t t0, int64_t t1,
+                                   int32_t* row_of) {
+  for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && rp[r] < t1; r += 256) {
+    const int64_t a = rp[r] > t0 ? rp[r] : t0, b = rp[r + 1] < t1 ? rp[r + 1] : t1;
+    for (int64_t p = a; p < b; ++p) row_of[p - t0] = r;
+  }
+}
+
+// kept entries per tile
+__global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp, const int32_t* col, int32_t m,
+                                                             int64_t nnz, const int32_t* tile_row,
+                                                             const uint8_t* keep_row, ColRank col_rank,
+                                                             int64_t* tile_cnt) {
   __shared__ int32_t row_of[kCompactTile];
-  __shared__ int32_t s_rows[2];
   __shared__ int32_t s_part[4];
+  const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
+  for (int64_t tile = blockIdx.x; tile * kCompactTile < nnz; tile += gridDim.x) {
+    const int64_t t0 = tile * kCompactTile;
+    const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
+    fill_row_of(rp, m, tile_row[tile], t0, t1, row_of);
+    __syncthreads();
+    int32_t c = 0;
+    for (int k = 0; k < kCompactTile / 256; ++k) {
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      if (p < t1 && (!keep_row || keep_row[row_of[p - t0]])) c += col_rank(ld_stream(col + p)) >= 0 ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) s_part[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[tile] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    __syncthreads();
+  }
+}
+
+// Kept entries written at tile_base[tile] + their prefix in the tile; the
+// output row pointer of every kept row that starts in the tile is the prefix
+// at its first position.
+__global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
+                                                       int32_t m, int64_t nnz, const int32_t* tile_row,
+                                                       const uint8_t* keep_row, const int32_t* row_o2n,
+                                                       ColRank col_rank, const int64_t* tile_base, gp_offset* out_rp,
+                                                       int32_t* out_col, double* out_val) {
+  __shared__ int32_t row_of[kCompactTile];
+  __shared__ int32_t pre[kCompactTile];
   __shared__ int32_t s_wave[4];
   const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
-  for (int64_t t0 = static_cast<int64_t>(blockIdx.x) * kCompactTile; t0 < nnz;
-       t0 += static_cast<int64_t>(gridDim.x) * kCompactTile) {
+  for (int64_t tile = blockIdx.x; tile * kCompactTile < nnz; tile += gridDim.x) {
+    const int64_t t0 = tile * kCompactTile;
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
-    if (threadIdx.x < 2) {  // the rows holding entries t0 and t1 - 1: last i with rp[i] <= key
-      const int64_t key = threadIdx.x == 0 ? t0 : t1 - 1;
-      int32_t lo = 0, hi = m - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (rp[mid] <= key) lo = mid;
-        else hi = mid - 1;
-      }
-      s_rows[threadIdx.x] = lo;
-    }
+    const int32_t i0 = tile_row[tile];
+    fill_row_of(rp, m, i0, t0, t1, row_of);
     __syncthreads();
-    const int32_t i0 = s_rows[0], i1 = s_rows[1];
-    for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r <= i1; r += 256) {
-      const int64_t a = rp[r] > t0 ? rp[r] : t0, b = rp[r + 1] < t1 ? rp[r + 1] : t1;
-      for (int64_t p = a; p < b; ++p) row_of[p - t0] = r;
-    }
-    // kept entries of row i0 before the tile
-    int32_t before = 0;
-    if (!keep_row || keep_row[i0])
-      for (int64_t p = rp[i0] + threadIdx.x; p < t0; p += 256) before += col_rank(col[p]) >= 0 ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
-    if (lane == 0) s_part[wave] = before;
-    __syncthreads();
-    int64_t running = excl[i0] + s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    const int64_t base = tile_base[tile];
+    int32_t running = 0;
     for (int k = 0; k < kCompactTile / 256; ++k) {
       const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
       int32_t j = -1;
@@ -729,14 +737,28 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
         off += w < wave ? s_wave[w] : 0;
         total += s_wave[w];
       }
+      const int32_t at = running + off + __popcll(bal & ((1ull << lane) - 1ull));
+      if (p < t1) pre[p - t0] = at;
       if (j >= 0) {
-        const int64_t at = running + off + __popcll(bal & ((1ull << lane) - 1ull));
-        out_col[at] = j;
-        out_val[at] = ld_stream(val + p);
+        out_col[base + at] = j;
+        out_val[base + at] = ld_stream(val + p);
       }
       running += total;
       __syncthreads();
     }
+    // row pointers of the kept rows starting in this tile (trailing empty rows
+    // at nnz belong to the last tile; empty rows just before i0 start at t0)
+    const bool last = t1 == nnz;
+    for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && (rp[r] < t1 || (last && rp[r] == nnz));
+         r += 256) {
+      const int64_t a = rp[r];
+      if (a < t0 || (keep_row && !keep_row[r])) continue;
+      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : running));
+    }
+    if (threadIdx.x == 0)
+      for (int32_t r = i0 - 1; r >= 0 && rp[r] == t0; --r)
+        if (!keep_row || keep_row[r]) out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base);
+    __syncthreads();
   }
 }
 static_assert(kBlock == 256, "k_compact_tiles assumes 256-lane workgroups");
@@ -751,7 +773,6 @@ __global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
 void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_o2n,
                  int32_t m_new, int32_t n_new, gp_csr& out) {
   const int32_t m = A.rows;
-  DVec<int64_t> cnt(static_cast<size_t>(m) + 1);
   int64_t total = 0;
   const PackedMask cols(keep_col, A.cols);
   // column ranks: exclusive prefix of the kept bits per 32-column word
@@ -763,21 +784,20 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
     exclusive_scan_i32_total(word_prefix.data(), nw);
   }
   const ColRank rank{keep_col ? cols.words.data() : nullptr, word_prefix.data()};
+  const int64_t tiles = (static_cast<int64_t>(A.nnz) + kCompactTile - 1) / kCompactTile;
+  const unsigned grid = static_cast<unsigned>(tiles < 16384 ? (tiles > 0 ? tiles : 1) : 16384);
+  DVec<int32_t> tile_row(static_cast<size_t>(tiles > 0 ? tiles : 1));
+  DVec<int64_t> tile_cnt(static_cast<size_t>(tiles > 0 ? tiles : 1));
   {
     TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
-    if (m > 0) {
-      const int lanes = pick_lanes(tuning().lanes_compact, static_cast<double>(A.nnz) / m);
-      const unsigned grid = grid_capped(static_cast<int64_t>(m) * lanes, kBlock, 16384);
-      switch (lanes) {
-        case 2: hipLaunchKernelGGL(k_compact_count<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
-        case 4: hipLaunchKernelGGL(k_compact_count<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
-        case 8: hipLaunchKernelGGL(k_compact_count<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
-        case 16: hipLaunchKernelGGL(k_compact_count<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
-        default: hipLaunchKernelGGL(k_compact_count<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, keep_row, cols.view(), cnt.data()); break;
-      }
+    if (tiles > 0) {
+      hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
+      hipLaunchKernelGGL(k_compact_tile_counts, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, m,
+                         static_cast<int64_t>(A.nnz), static_cast<const int32_t*>(tile_row.data()), keep_row, rank,
+                         tile_cnt.data());
       GP_LAUNCH_CHECK("compact count");
+      exclusive_scan_i64(tile_cnt.data(), tiles, &total);
     }
-    exclusive_scan_i64(cnt.data(), m, &total);
   }
   if (total > INT32_MAX) throw std::length_error("compacted nnz exceeds int32 offsets");
   out.rows = m_new;
@@ -787,16 +807,13 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(total));
   out.vals = pool_alloc<double>(static_cast<size_t>(total));
   TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + 12.0 * total + 4.0 * (m_new + 1));
-  if (m > 0) {
-    hipLaunchKernelGGL(k_compact_rowptr, dim3(grid_for(m)), dim3(kBlock), 0, 0, cnt.data(), row_o2n, m, out.row_ptr);
-    GP_LAUNCH_CHECK("compact rowptr");
-    if (A.nnz > 0) {
-      const int64_t tiles = (static_cast<int64_t>(A.nnz) + kCompactTile - 1) / kCompactTile;
-      hipLaunchKernelGGL(k_compact_tiles, dim3(static_cast<unsigned>(tiles < 16384 ? tiles : 16384)), dim3(256), 0, 0,
-                         A.row_ptr, A.col_idx, A.vals, m, static_cast<int64_t>(A.nnz), keep_row, rank, cnt.data(),
-                         out.col_idx, out.vals);
-      GP_LAUNCH_CHECK("compact copy");
-    }
+  if (tiles > 0) {
+    hipLaunchKernelGGL(k_compact_tiles, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
+                       static_cast<int64_t>(A.nnz), static_cast<const int32_t*>(tile_row.data()), keep_row, row_o2n,
+                       rank, static_cast<const int64_t*>(tile_cnt.data()), out.row_ptr, out.col_idx, out.vals);
+    GP_LAUNCH_CHECK("compact copy");
+  } else {
+    GP_CHECK(hipMemsetAsync(out.row_ptr, 0, sizeof(gp_offset) * (static_cast<size_t>(m_new) + 1), 0));
   }
   hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, 0, out.row_ptr, m_new, static_cast<gp_offset>(total));
   GP_LAUNCH_CHECK("compact last");

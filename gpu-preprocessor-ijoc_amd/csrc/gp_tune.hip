@@ -1,7 +1,7 @@
 // gp_tune.hip -- launch-schedule autotuner.
 //
-// The grouped sweeps (bound candidates, column max, compaction, live
-// statistics) run G lanes per row, i.e. 64 / G rows per wavefront, and the row
+// The grouped sweeps (bound candidates, column max, live statistics) run G
+// lanes per row, i.e. 64 / G rows per wavefront, and the row
 // folds stage R rows per workgroup.  Every choice computes the same results
 // (the grouped sweeps are order-free, a row fold's order is its CSR order), so
 // the schedule can be picked by measurement.  gp_autotune presolves the given
@@ -115,7 +115,6 @@ int gp_autotune(const gp_device_lp* lp, const gp_params* params, int32_t reps, g
       const Sample s = best_of(lp, p, reps);
       consider(0, kLanes[i], s.family[GP_KT_BOUND_CANDIDATES]);
       consider(1, kLanes[i], s.family[GP_KT_COL_MAX]);
-      consider(2, kLanes[i], s.family[GP_KT_COMPACT]);
       consider(3, kLanes[i], s.family[GP_KT_LIVE_STATS]);
       if (i < 3)
         consider(4, kRows[i], s.family[GP_KT_ACTIVITY] + s.family[GP_KT_SC_ACTIVITY] + s.family[GP_KT_REDUNDANT]);

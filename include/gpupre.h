@@ -333,7 +333,7 @@ int gp_record_copy(const gp_summary* summary,
 typedef struct gp_tuning {
   int32_t lanes_candidates;
   int32_t lanes_col_max;
-  int32_t lanes_compact;
+  int32_t lanes_compact;   /* accepted, no effect: compaction is entry-parallel */
   int32_t lanes_live_stats;
   int32_t fold_rows;
 } gp_tuning;
