@@ -675,6 +675,11 @@ __device__ inline void fill_row_of(const gp_offset* rp, int32_t m, int32_t i0, i
   }
 }
 
+// Each lane owns kCompactPer consecutive entries of the tile (vector loads,
+// independent loads in flight); its first row comes from LDS and the row is
+// advanced along the row pointers.
+constexpr int kCompactPer = kCompactTile / 256;
+
 // kept entries per tile
 __global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp, const int32_t* col, int32_t m,
                                                              int64_t nnz, const int32_t* tile_row,
@@ -688,10 +693,24 @@ __global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
     fill_row_of(rp, m, tile_row[tile], t0, t1, row_of);
     __syncthreads();
+    const int64_t q0 = t0 + static_cast<int64_t>(threadIdx.x) * kCompactPer;
     int32_t c = 0;
-    for (int k = 0; k < kCompactTile / 256; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      if (p < t1 && (!keep_row || keep_row[row_of[p - t0]])) c += col_rank(ld_stream(col + p)) >= 0 ? 1 : 0;
+    if (q0 < t1) {
+      int32_t r = row_of[q0 - t0];
+      gp_offset rend = rp[r + 1];
+      bool rk = !keep_row || keep_row[r];
+#pragma unroll
+      for (int k = 0; k < kCompactPer; ++k) {
+        const int64_t p = q0 + k;
+        if (p < t1) {
+          while (p >= rend) {
+            ++r;
+            rend = rp[r + 1];
+            rk = !keep_row || keep_row[r];
+          }
+          if (rk && col_rank(ld_stream(col + p)) >= 0) ++c;
+        }
+      }
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if (lane == 0) s_part[wave] = c;
@@ -720,32 +739,57 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
     fill_row_of(rp, m, i0, t0, t1, row_of);
     __syncthreads();
     const int64_t base = tile_base[tile];
-    int32_t running = 0;
-    for (int k = 0; k < kCompactTile / 256; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      int32_t j = -1;
-      if (p < t1) {
-        const int32_t r = row_of[p - t0];
-        const int32_t c = ld_stream(col + p);
-        if (!keep_row || keep_row[r]) j = col_rank(c);
+    const int64_t q0 = t0 + static_cast<int64_t>(threadIdx.x) * kCompactPer;
+    int32_t jv[kCompactPer];
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < kCompactPer; ++k) jv[k] = -1;
+    if (q0 < t1) {
+      int32_t r = row_of[q0 - t0];
+      gp_offset rend = rp[r + 1];
+      bool rk = !keep_row || keep_row[r];
+#pragma unroll
+      for (int k = 0; k < kCompactPer; ++k) {
+        const int64_t p = q0 + k;
+        if (p < t1) {
+          while (p >= rend) {
+            ++r;
+            rend = rp[r + 1];
+            rk = !keep_row || keep_row[r];
+          }
+          const int32_t cj = ld_stream(col + p);
+          jv[k] = rk ? col_rank(cj) : -1;
+          c += jv[k] >= 0 ? 1 : 0;
+        }
       }
-      const unsigned long long bal = __ballot(j >= 0);
-      if (lane == 0) s_wave[wave] = __popcll(bal);
-      __syncthreads();
-      int32_t off = 0, total = 0;
-      for (int w = 0; w < 4; ++w) {
-        off += w < wave ? s_wave[w] : 0;
-        total += s_wave[w];
-      }
-      const int32_t at = running + off + __popcll(bal & ((1ull << lane) - 1ull));
-      if (p < t1) pre[p - t0] = at;
-      if (j >= 0) {
-        out_col[base + at] = j;
-        out_val[base + at] = ld_stream(val + p);
-      }
-      running += total;
-      __syncthreads();
     }
+    // exclusive prefix of the lanes' counts over the workgroup
+    int32_t incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t off = incl - c, total = 0;
+    for (int w = 0; w < 4; ++w) {
+      off += w < wave ? s_wave[w] : 0;
+      total += s_wave[w];
+    }
+    int32_t at = off;
+#pragma unroll
+    for (int k = 0; k < kCompactPer; ++k) {
+      const int64_t p = q0 + k;
+      if (p < t1) {
+        pre[p - t0] = at;
+        if (jv[k] >= 0) {
+          out_col[base + at] = jv[k];
+          out_val[base + at] = ld_stream(val + p);
+          ++at;
+        }
+      }
+    }
+    __syncthreads();
     // row pointers of the kept rows starting in this tile (trailing empty rows
     // at nnz belong to the last tile; empty rows just before i0 start at t0)
     const bool last = t1 == nnz;
@@ -753,7 +797,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
          r += 256) {
       const int64_t a = rp[r];
       if (a < t0 || (keep_row && !keep_row[r])) continue;
-      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : running));
+      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : total));
     }
     if (threadIdx.x == 0)
       for (int32_t r = i0 - 1; r >= 0 && rp[r] == t0; --r)

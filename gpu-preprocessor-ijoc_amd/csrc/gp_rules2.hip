@@ -653,44 +653,180 @@ __device__ inline uint64_t dbits(double v) { return static_cast<uint64_t>(__doub
 __device__ inline uint64_t as_u64(int64_t v) { return static_cast<uint64_t>(v); }
 
 // ============================================================== parallel rows
-__global__ void k_pr_hash(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_col, double tol, int use_xor,
-                          uint64_t* hash, int32_t* ids) {
-  const int64_t i = gtid();
-  if (i >= A.rows) return;
-  ids[i] = static_cast<int32_t>(i);
-  if (!keep_row[i]) {
-    hash[i] = as_u64(i);
-    return;
-  }
-  const gp_offset p1 = A.row_ptr[i + 1];
-  gp_offset first = A.row_ptr[i];
-  while (keep_col && first < p1 && !keep_col[A.col_idx[first]]) ++first;
-  int64_t len = 0;
-  for (gp_offset q = first; q < p1; ++q)
-    if (!keep_col || keep_col[A.col_idx[q]]) ++len;
-  if (len <= 0) {
-    hash[i] = fnv_mix(kFnvOffset, as_u64(i));
-    return;
-  }
-  const double pivot = A.vals[first];
-  if (fabs(pivot) <= tol) {
-    hash[i] = fnv_mix(kFnvOffset, as_u64(i));
-    return;
-  }
-  uint64_t h = fnv_mix(kFnvOffset, as_u64(len)), agg = 0;
-  for (gp_offset q = first; q < p1; ++q) {
-    const int32_t col = A.col_idx[q];
-    if (keep_col && !keep_col[col]) continue;
-    const uint64_t nb = dbits(A.vals[q] / pivot);
-    if (use_xor) {
-      agg ^= fnv_mix(fnv_mix(kFnvOffset, as_u64(col)), nb);
-    } else {
-      h = fnv_mix(h, as_u64(col));
-      h = fnv_mix(h, nb);
+// ---- staged hashing ------------------------------------------------------------------------------
+// The row / column hashes are sequential FNV chains over each line's live
+// entries, so they cannot be split across lanes.  Two passes instead:
+// (1) G lanes per line count the live entries and find the first one
+// (order-free); (2) k_line_stream stages the lines' (index, value) pairs
+// through LDS with coalesced loads and each lane runs its own line's chain
+// from LDS, in order.  The results equal the one-lane-per-line hash.
+struct LivePr {  // parallel rows: entry live when its column is live
+  const uint8_t* keep;
+  __device__ bool operator()(int32_t idx, double) const { return !keep || keep[idx]; }
+};
+struct LivePc {  // parallel columns: live row and |a| above the tolerance
+  const uint8_t* keep;
+  double tol;
+  __device__ bool operator()(int32_t idx, double v) const { return keep[idx] && fabs(v) > tol; }
+};
+
+template <int G, class Live>
+__global__ void __launch_bounds__(kBlock) k_first_live(gp_csr M, Live live, int32_t* len, gp_offset* first) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t i = gtid() / G; i < M.rows; i += groups) {
+    int32_t c = 0;
+    gp_offset f = INT_MAX;
+    const gp_offset p1 = M.row_ptr[i + 1];
+    for (gp_offset p = M.row_ptr[i] + lane; p < p1; p += G)
+      if (live(M.col_idx[p], M.vals[p])) {
+        ++c;
+        f = p < f ? p : f;
+      }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      c += __shfl_xor(c, o, G);
+      const gp_offset g = __shfl_xor(f, o, G);
+      f = g < f ? g : f;
+    }
+    if (lane == 0) {
+      len[i] = c;
+      first[i] = f;
     }
   }
-  hash[i] = use_xor ? fnv_mix(h, agg) : h;
 }
+
+template <class Live>
+void first_live(const gp_csr& M, Live live, int32_t* len, gp_offset* first) {
+  if (M.rows <= 0) return;
+  const int lanes = lanes_for_mean(static_cast<double>(M.nnz) / M.rows);
+  const unsigned grid = grid_capped(static_cast<int64_t>(M.rows) * lanes, kBlock, 16384);
+  switch (lanes) {
+    case 2: hipLaunchKernelGGL((k_first_live<2, Live>), dim3(grid), dim3(kBlock), 0, 0, M, live, len, first); break;
+    case 4: hipLaunchKernelGGL((k_first_live<4, Live>), dim3(grid), dim3(kBlock), 0, 0, M, live, len, first); break;
+    case 8: hipLaunchKernelGGL((k_first_live<8, Live>), dim3(grid), dim3(kBlock), 0, 0, M, live, len, first); break;
+    case 16: hipLaunchKernelGGL((k_first_live<16, Live>), dim3(grid), dim3(kBlock), 0, 0, M, live, len, first); break;
+    default: hipLaunchKernelGGL((k_first_live<64, Live>), dim3(grid), dim3(kBlock), 0, 0, M, live, len, first); break;
+  }
+  GP_LAUNCH_CHECK("first live entries");
+}
+
+constexpr int kStreamRows = 256;
+constexpr int kStreamChunk = 2048;
+// Policy P: State; void init(State&, line); void add(State&, index, value); void finish(line, const State&)
+template <class P>
+__global__ void __launch_bounds__(kStreamRows) k_line_stream(gp_csr M, P pol) {
+  __shared__ int32_t s_idx[kStreamChunk];
+  __shared__ double s_val[kStreamChunk];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kStreamRows;
+  const int64_t r1 = r0 + kStreamRows < M.rows ? r0 + kStreamRows : M.rows;
+  const int64_t line = r0 + threadIdx.x;
+  const bool mine = line < r1 && pol.active(line);
+  typename P::State st;
+  if (mine) pol.init(st, line);
+  if (__syncthreads_or(mine)) {
+    const gp_offset e0 = M.row_ptr[r0], e1 = M.row_ptr[r1];
+    const gp_offset p0 = mine ? M.row_ptr[line] : 0, p1 = mine ? M.row_ptr[line + 1] : 0;
+    for (gp_offset c0 = e0; c0 < e1; c0 += kStreamChunk) {
+      const gp_offset c1 = e1 - c0 > kStreamChunk ? c0 + kStreamChunk : e1;
+      for (gp_offset q = c0 + threadIdx.x; q < c1; q += kStreamRows) {
+        s_idx[q - c0] = ld_stream(M.col_idx + q);
+        s_val[q - c0] = ld_stream(M.vals + q);
+      }
+      __syncthreads();
+      if (mine) {
+        const gp_offset a0 = p0 > c0 ? p0 : c0, a1 = p1 < c1 ? p1 : c1;
+        for (gp_offset q = a0; q < a1; ++q) pol.add(st, s_idx[q - c0], s_val[q - c0]);
+      }
+      __syncthreads();
+    }
+  }
+  if (line < r1) pol.finish(line, st, mine);
+}
+
+template <class P>
+void launch_line_stream(const gp_csr& M, const P& pol) {
+  if (M.rows <= 0) return;
+  const int64_t blocks = (static_cast<int64_t>(M.rows) + kStreamRows - 1) / kStreamRows;
+  hipLaunchKernelGGL(k_line_stream<P>, dim3(static_cast<unsigned>(blocks)), dim3(kStreamRows), 0, 0, M, pol);
+  GP_LAUNCH_CHECK("line stream");
+}
+
+// hash of a row over its live columns (rule_parallel_rows.cu:132-192)
+struct PrHash {
+  const uint8_t* keep_row;
+  LivePr live;
+  const int32_t* len;
+  const gp_offset* first;
+  const double* vals;
+  double tol;
+  int use_xor;
+  uint64_t* hash;
+  int32_t* ids;
+  struct State {
+    uint64_t h, agg;
+    double pivot;
+    bool ok;
+  };
+  __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
+  __device__ void init(State& st, int64_t i) const {
+    st.ok = len[i] > 0;
+    st.pivot = st.ok ? vals[first[i]] : 0.0;
+    if (st.ok && fabs(st.pivot) <= tol) st.ok = false;
+    st.h = fnv_mix(kFnvOffset, as_u64(len[i]));
+    st.agg = 0;
+  }
+  __device__ void add(State& st, int32_t col, double v) const {
+    if (!st.ok || !live(col, v)) return;
+    const uint64_t nb = dbits(v / st.pivot);
+    if (use_xor) {
+      st.agg ^= fnv_mix(fnv_mix(kFnvOffset, as_u64(col)), nb);
+    } else {
+      st.h = fnv_mix(st.h, as_u64(col));
+      st.h = fnv_mix(st.h, nb);
+    }
+  }
+  __device__ void finish(int64_t i, const State& st, bool mine) const {
+    ids[i] = static_cast<int32_t>(i);
+    if (!mine) hash[i] = as_u64(i);
+    else if (!st.ok) hash[i] = fnv_mix(kFnvOffset, as_u64(i));
+    else hash[i] = use_xor ? fnv_mix(st.h, st.agg) : st.h;
+  }
+};
+
+// hash of a column over its live rows (rule_parallel_columns.cu:189-240)
+struct PcHash {
+  const uint8_t* keep_col;
+  LivePc live;
+  const int32_t* len;
+  const gp_offset* first;
+  const double* vals;
+  uint64_t* hash;
+  int32_t* ids;
+  struct State {
+    uint64_t h;
+    double pivot;
+    bool ok;
+  };
+  __device__ bool active(int64_t j) const { return keep_col[j] != 0; }
+  __device__ void init(State& st, int64_t j) const {
+    st.ok = len[j] > 0;
+    st.pivot = st.ok ? vals[first[j]] : 0.0;
+    st.h = fnv_mix(kFnvOffset, as_u64(len[j]));
+  }
+  __device__ void add(State& st, int32_t row, double v) const {
+    if (!st.ok || !live(row, v)) return;
+    st.h = fnv_mix(st.h, as_u64(row));
+    st.h = fnv_mix(st.h, dbits(v / st.pivot));
+  }
+  __device__ void finish(int64_t j, const State& st, bool mine) const {
+    ids[j] = static_cast<int32_t>(j);
+    if (!mine) hash[j] = as_u64(j);
+    else if (!st.ok) hash[j] = fnv_mix(kFnvOffset, as_u64(j));
+    else hash[j] = st.h;
+  }
+};
+
 
 __device__ bool pr_ratio(const gp_csr& A, int32_t i, int32_t k, const uint8_t* keep_col, double tol, double* out) {
   gp_offset li = A.row_ptr[i], le = A.row_ptr[i + 1], ri = A.row_ptr[k], re = A.row_ptr[k + 1];
@@ -847,38 +983,6 @@ __device__ inline gp_offset pc_next(const gp_csr& AT, gp_offset q, gp_offset sto
   return q;
 }
 
-__global__ void k_pc_hash(gp_csr AT, const uint8_t* keep_col, const uint8_t* keep_row, double tol, uint64_t* hash,
-                          int32_t* ids) {
-  const int64_t j = gtid();
-  if (j >= AT.rows) return;
-  ids[j] = static_cast<int32_t>(j);
-  if (!keep_col[j]) {
-    hash[j] = as_u64(j);
-    return;
-  }
-  int32_t len = 0;
-  double pivot = 0.0;
-  bool set = false;
-  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
-    if (!pc_live(AT, q, keep_row, tol)) continue;
-    ++len;
-    if (!set) {
-      pivot = AT.vals[q];
-      set = true;
-    }
-  }
-  if (!set || len == 0) {
-    hash[j] = fnv_mix(kFnvOffset, as_u64(j));
-    return;
-  }
-  uint64_t h = fnv_mix(kFnvOffset, as_u64(len));
-  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
-    if (!pc_live(AT, q, keep_row, tol)) continue;
-    h = fnv_mix(h, as_u64(AT.col_idx[q]));
-    h = fnv_mix(h, dbits(AT.vals[q] / pivot));
-  }
-  hash[j] = h;
-}
 
 __device__ bool pc_ratio(const gp_csr& AT, int32_t j, int32_t k, const uint8_t* keep_row, double zt, double ctol,
                          double* out) {
@@ -1768,9 +1872,11 @@ void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
   DVec<int32_t> ids(m);
   {
     TimedScope ts(GP_KT_HASH_SORT, 0.0);
-    hipLaunchKernelGGL(k_pr_hash, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, plan.keep_row_mask, keep_col,
-                       p.zero_tol, use_xor, hash.data(), ids.data());
-    GP_LAUNCH_CHECK("parallel rows hash");
+    DVec<int32_t> len(m);
+    DVec<gp_offset> first(m);
+    first_live(lp.A, LivePr{keep_col}, len.data(), first.data());
+    launch_line_stream(lp.A, PrHash{plan.keep_row_mask, LivePr{keep_col}, len.data(), first.data(), lp.A.vals,
+                                    p.zero_tol, use_xor, hash.data(), ids.data()});
     sort_pairs_u64_i32(hash.data(), ids.data(), m);
   }
   TimedScope ts(GP_KT_RULES_OTHER, 0.0);
@@ -1824,9 +1930,11 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
   DVec<int32_t> ids(n);
   {
     TimedScope ts(GP_KT_HASH_SORT, 0.0);
-    hipLaunchKernelGGL(k_pc_hash, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask, plan.keep_row_mask,
-                       zt, hash.data(), ids.data());
-    GP_LAUNCH_CHECK("parallel columns hash");
+    DVec<int32_t> len(n);
+    DVec<gp_offset> first(n);
+    first_live(lp.AT, LivePc{plan.keep_row_mask, zt}, len.data(), first.data());
+    launch_line_stream(lp.AT, PcHash{plan.keep_col_mask, LivePc{plan.keep_row_mask, zt}, len.data(), first.data(),
+                                     lp.AT.vals, hash.data(), ids.data()});
     sort_pairs_u64_i32(hash.data(), ids.data(), n);
   }
   TimedScope ts(GP_KT_RULES_OTHER, 0.0);
