@@ -675,10 +675,40 @@ __device__ inline void fill_row_of(const gp_offset* rp, int32_t m, int32_t i0, i
   }
 }
 
-// Each lane owns kCompactPer consecutive entries of the tile (vector loads,
-// independent loads in flight); its first row comes from LDS and the row is
-// advanced along the row pointers.
-constexpr int kCompactPer = kCompactTile / 256;
+// Entry t0 + k * 256 + threadIdx.x of a tile is the lane's k-th entry, so
+// every load stays coalesced; all kPer loads of a lane are issued before any
+// is used (branch-free: out-of-tile lanes read index 0 and are masked after).
+constexpr int kPer = kCompactTile / 256;
+struct TileEntries {
+  int32_t j[kPer];  // new column index, -1 when the entry is dropped or past the tile
+  __device__ void load(const int32_t* row_of, const int32_t* col, const uint8_t* keep_row, ColRank rank, int64_t t0,
+                       int64_t t1) {
+    int32_t c[kPer], r[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      const bool in = p < t1;
+      c[k] = ld_stream(col + (in ? p : t0));
+      r[k] = row_of[in ? static_cast<int32_t>(p - t0) : 0];
+    }
+    uint32_t w[kPer];
+    int32_t pre[kPer];
+    uint8_t kr[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      kr[k] = keep_row ? keep_row[r[k]] : 1;
+      w[k] = rank.words ? rank.words[c[k] >> 5] : ~0u;
+      pre[k] = rank.words ? rank.prefix[c[k] >> 5] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      const uint32_t bit = 1u << (c[k] & 31);
+      const int32_t nj = rank.words ? pre[k] + __popc(w[k] & (bit - 1u)) : c[k];
+      j[k] = (p < t1 && kr[k] && (w[k] & bit)) ? nj : -1;
+    }
+  }
+};
 
 // kept entries per tile
 __global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp, const int32_t* col, int32_t m,
@@ -693,25 +723,11 @@ __global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
     fill_row_of(rp, m, tile_row[tile], t0, t1, row_of);
     __syncthreads();
-    const int64_t q0 = t0 + static_cast<int64_t>(threadIdx.x) * kCompactPer;
+    TileEntries e;
+    e.load(row_of, col, keep_row, col_rank, t0, t1);
     int32_t c = 0;
-    if (q0 < t1) {
-      int32_t r = row_of[q0 - t0];
-      gp_offset rend = rp[r + 1];
-      bool rk = !keep_row || keep_row[r];
 #pragma unroll
-      for (int k = 0; k < kCompactPer; ++k) {
-        const int64_t p = q0 + k;
-        if (p < t1) {
-          while (p >= rend) {
-            ++r;
-            rend = rp[r + 1];
-            rk = !keep_row || keep_row[r];
-          }
-          if (rk && col_rank(ld_stream(col + p)) >= 0) ++c;
-        }
-      }
-    }
+    for (int k = 0; k < kPer; ++k) c += e.j[k] >= 0 ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if (lane == 0) s_part[wave] = c;
     __syncthreads();
@@ -722,7 +738,8 @@ __global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp
 
 // Kept entries written at tile_base[tile] + their prefix in the tile; the
 // output row pointer of every kept row that starts in the tile is the prefix
-// at its first position.
+// at its first position.  The tile prefix orders entries by position: the
+// count of (k', w') before (k, w) is all of k' < k plus waves w' < w of k.
 __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
                                                        int32_t m, int64_t nnz, const int32_t* tile_row,
                                                        const uint8_t* keep_row, const int32_t* row_o2n,
@@ -730,63 +747,47 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
                                                        int32_t* out_col, double* out_val) {
   __shared__ int32_t row_of[kCompactTile];
   __shared__ int32_t pre[kCompactTile];
-  __shared__ int32_t s_wave[4];
+  __shared__ int32_t s_cnt[kPer * 4];
   const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
+  const unsigned long long lt = (1ull << lane) - 1ull;
   for (int64_t tile = blockIdx.x; tile * kCompactTile < nnz; tile += gridDim.x) {
     const int64_t t0 = tile * kCompactTile;
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
     const int32_t i0 = tile_row[tile];
     fill_row_of(rp, m, i0, t0, t1, row_of);
     __syncthreads();
-    const int64_t base = tile_base[tile];
-    const int64_t q0 = t0 + static_cast<int64_t>(threadIdx.x) * kCompactPer;
-    int32_t jv[kCompactPer];
-    int32_t c = 0;
+    TileEntries e;
+    e.load(row_of, col, keep_row, col_rank, t0, t1);
+    double v[kPer];
 #pragma unroll
-    for (int k = 0; k < kCompactPer; ++k) jv[k] = -1;
-    if (q0 < t1) {
-      int32_t r = row_of[q0 - t0];
-      gp_offset rend = rp[r + 1];
-      bool rk = !keep_row || keep_row[r];
+    for (int k = 0; k < kPer; ++k) {
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      v[k] = ld_stream(val + (p < t1 ? p : t0));
+    }
+    unsigned long long bal[kPer];
 #pragma unroll
-      for (int k = 0; k < kCompactPer; ++k) {
-        const int64_t p = q0 + k;
-        if (p < t1) {
-          while (p >= rend) {
-            ++r;
-            rend = rp[r + 1];
-            rk = !keep_row || keep_row[r];
-          }
-          const int32_t cj = ld_stream(col + p);
-          jv[k] = rk ? col_rank(cj) : -1;
-          c += jv[k] >= 0 ? 1 : 0;
-        }
-      }
+    for (int k = 0; k < kPer; ++k) {
+      bal[k] = __ballot(e.j[k] >= 0);
+      if (lane == 0) s_cnt[k * 4 + wave] = __popcll(bal[k]);
     }
-    // exclusive prefix of the lanes' counts over the workgroup
-    int32_t incl = c;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    int32_t off = incl - c, total = 0;
-    for (int w = 0; w < 4; ++w) {
-      off += w < wave ? s_wave[w] : 0;
-      total += s_wave[w];
-    }
-    int32_t at = off;
+    const int64_t base = tile_base[tile];
+    int32_t running = 0;
 #pragma unroll
-    for (int k = 0; k < kCompactPer; ++k) {
-      const int64_t p = q0 + k;
-      if (p < t1) {
-        pre[p - t0] = at;
-        if (jv[k] >= 0) {
-          out_col[base + at] = jv[k];
-          out_val[base + at] = ld_stream(val + p);
-          ++at;
-        }
+    for (int k = 0; k < kPer; ++k) {
+      int32_t off = running;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int32_t cnt = s_cnt[k * 4 + w];
+        off += w < wave ? cnt : 0;
+        running += cnt;
+      }
+      const int32_t at = off + __popcll(bal[k] & lt);
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      if (p < t1) pre[p - t0] = at;
+      if (e.j[k] >= 0) {
+        out_col[base + at] = e.j[k];
+        out_val[base + at] = v[k];
       }
     }
     __syncthreads();
@@ -797,7 +798,7 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
          r += 256) {
       const int64_t a = rp[r];
       if (a < t0 || (keep_row && !keep_row[r])) continue;
-      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : total));
+      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : running));
     }
     if (threadIdx.x == 0)
       for (int32_t r = i0 - 1; r >= 0 && rp[r] == t0; --r)
