@@ -710,47 +710,62 @@ struct TileEntries {
   }
 };
 
-// kept entries per tile
-__global__ void __launch_bounds__(256) k_compact_tile_counts(const gp_offset* rp, const int32_t* col, int32_t m,
-                                                             int64_t nnz, const int32_t* tile_row,
-                                                             const uint8_t* keep_row, ColRank col_rank,
-                                                             int64_t* tile_cnt) {
-  __shared__ int32_t row_of[kCompactTile];
-  __shared__ int32_t s_part[4];
-  const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
-  for (int64_t tile = blockIdx.x; tile * kCompactTile < nnz; tile += gridDim.x) {
-    const int64_t t0 = tile * kCompactTile;
-    const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
-    fill_row_of(rp, m, tile_row[tile], t0, t1, row_of);
-    __syncthreads();
-    TileEntries e;
-    e.load(row_of, col, keep_row, col_rank, t0, t1);
-    int32_t c = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) c += e.j[k] >= 0 ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (lane == 0) s_part[wave] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) tile_cnt[tile] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
-    __syncthreads();
+// Single pass with decoupled look-back.  Workgroups take tiles in ticket
+// order from an atomic counter, so every tile a workgroup waits for belongs to
+// a workgroup that is already running and publishes its count before it waits
+// on anything.  A tile's status word holds a flag in the top two bits (1: the
+// tile's own count, 2: the inclusive count of tiles 0..t) and the count below.
+// A look-back that spins past kSpinLimit polls gives up, marks the error word
+// and lets the kernel finish; the host then raises an error.
+constexpr unsigned long long kStatAgg = 1ull << 62, kStatIncl = 2ull << 62, kStatMask = (1ull << 62) - 1;
+constexpr int64_t kSpinLimit = int64_t{1} << 26;
+__device__ inline void publish(unsigned long long* st, unsigned long long v) {
+  __hip_atomic_store(st, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// kept entries before tile t (thread-local; called by one lane)
+__device__ inline int64_t look_back(unsigned long long* status, int64_t t, int32_t* err) {
+  int64_t excl = 0, spins = 0;
+  for (int64_t q = t - 1; q >= 0;) {
+    const unsigned long long v = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((v >> 62) == 0) {
+      if (++spins > kSpinLimit) {
+        atomicOr(err, 1);
+        return 0;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += static_cast<int64_t>(v & kStatMask);
+    if ((v >> 62) == 2) break;
+    --q;
   }
+  return excl;
 }
 
-// Kept entries written at tile_base[tile] + their prefix in the tile; the
+// Kept entries written at the tile's base + their prefix in the tile; the
 // output row pointer of every kept row that starts in the tile is the prefix
 // at its first position.  The tile prefix orders entries by position: the
 // count of (k', w') before (k, w) is all of k' < k plus waves w' < w of k.
+// ctl[0]: tile ticket, ctl[1]: error word, ctl[2]: total kept entries,
+// ctl[3..]: tile status words; all zero at launch.
 __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
-                                                       int32_t m, int64_t nnz, const int32_t* tile_row,
+                                                       int32_t m, int64_t nnz, int64_t tiles, const int32_t* tile_row,
                                                        const uint8_t* keep_row, const int32_t* row_o2n,
-                                                       ColRank col_rank, const int64_t* tile_base, gp_offset* out_rp,
-                                                       int32_t* out_col, double* out_val) {
+                                                       ColRank col_rank, int32_t m_new, unsigned long long* ctl,
+                                                       gp_offset* out_rp, int32_t* out_col, double* out_val) {
   __shared__ int32_t row_of[kCompactTile];
   __shared__ int32_t pre[kCompactTile];
   __shared__ int32_t s_cnt[kPer * 4];
+  __shared__ int64_t s_tile, s_base;
   const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  for (int64_t tile = blockIdx.x; tile * kCompactTile < nnz; tile += gridDim.x) {
+  unsigned long long* status = ctl + 3;
+  int32_t* err = reinterpret_cast<int32_t*>(ctl + 1);
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = static_cast<int64_t>(atomicAdd(ctl, 1ull));
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= tiles) break;
     const int64_t t0 = tile * kCompactTile;
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
     const int32_t i0 = tile_row[tile];
@@ -771,7 +786,25 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
       if (lane == 0) s_cnt[k * 4 + wave] = __popcll(bal[k]);
     }
     __syncthreads();
-    const int64_t base = tile_base[tile];
+    if (threadIdx.x == 0) {
+      int64_t own = 0;
+      for (int q = 0; q < kPer * 4; ++q) own += s_cnt[q];
+      int64_t excl = 0;
+      if (tile == 0) {
+        publish(status, kStatIncl | static_cast<unsigned long long>(own));
+      } else {
+        publish(status + tile, kStatAgg | static_cast<unsigned long long>(own));
+        excl = look_back(status, tile, err);
+        publish(status + tile, kStatIncl | static_cast<unsigned long long>(excl + own));
+      }
+      if (tile == tiles - 1) {
+        ctl[2] = static_cast<unsigned long long>(excl + own);
+        out_rp[m_new] = static_cast<gp_offset>(excl + own);
+      }
+      s_base = excl;
+    }
+    __syncthreads();
+    const int64_t base = s_base;
     int32_t running = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -808,7 +841,6 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
 }
 static_assert(kBlock == 256, "k_compact_tiles assumes 256-lane workgroups");
 
-__global__ void k_set_last(gp_offset* rp, int32_t at, gp_offset v) { rp[at] = v; }
 __global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
   const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w < nw) out[w] = __popc(words[w]);
@@ -818,7 +850,6 @@ __global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
 void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_o2n,
                  int32_t m_new, int32_t n_new, gp_csr& out) {
   const int32_t m = A.rows;
-  int64_t total = 0;
   const PackedMask cols(keep_col, A.cols);
   // column ranks: exclusive prefix of the kept bits per 32-column word
   const int64_t nw = (static_cast<int64_t>(A.cols) + 31) / 32;
@@ -830,38 +861,36 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   }
   const ColRank rank{keep_col ? cols.words.data() : nullptr, word_prefix.data()};
   const int64_t tiles = (static_cast<int64_t>(A.nnz) + kCompactTile - 1) / kCompactTile;
-  const unsigned grid = static_cast<unsigned>(tiles < 16384 ? (tiles > 0 ? tiles : 1) : 16384);
-  DVec<int32_t> tile_row(static_cast<size_t>(tiles > 0 ? tiles : 1));
-  DVec<int64_t> tile_cnt(static_cast<size_t>(tiles > 0 ? tiles : 1));
-  {
-    TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 4.0 * A.nnz + (double)m + (double)A.cols);
-    if (tiles > 0) {
-      hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
-      hipLaunchKernelGGL(k_compact_tile_counts, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, m,
-                         static_cast<int64_t>(A.nnz), static_cast<const int32_t*>(tile_row.data()), keep_row, rank,
-                         tile_cnt.data());
-      GP_LAUNCH_CHECK("compact count");
-      exclusive_scan_i64(tile_cnt.data(), tiles, &total);
-    }
-  }
-  if (total > INT32_MAX) throw std::length_error("compacted nnz exceeds int32 offsets");
   out.rows = m_new;
   out.cols = n_new;
-  out.nnz = static_cast<gp_offset>(total);
   out.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(m_new) + 1);
-  out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(total));
-  out.vals = pool_alloc<double>(static_cast<size_t>(total));
-  TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + 12.0 * total + 4.0 * (m_new + 1));
-  if (tiles > 0) {
-    hipLaunchKernelGGL(k_compact_tiles, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
-                       static_cast<int64_t>(A.nnz), static_cast<const int32_t*>(tile_row.data()), keep_row, row_o2n,
-                       rank, static_cast<const int64_t*>(tile_cnt.data()), out.row_ptr, out.col_idx, out.vals);
-    GP_LAUNCH_CHECK("compact copy");
-  } else {
+  if (tiles == 0) {
+    out.nnz = 0;
+    out.col_idx = pool_alloc<int32_t>(0);
+    out.vals = pool_alloc<double>(0);
     GP_CHECK(hipMemsetAsync(out.row_ptr, 0, sizeof(gp_offset) * (static_cast<size_t>(m_new) + 1), 0));
+    return;
   }
-  hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, 0, out.row_ptr, m_new, static_cast<gp_offset>(total));
-  GP_LAUNCH_CHECK("compact last");
+  // the kept count is known only after the pass: room for every entry
+  out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(A.nnz));
+  out.vals = pool_alloc<double>(static_cast<size_t>(A.nnz));
+  DVec<int32_t> tile_row(static_cast<size_t>(tiles));
+  DVec<unsigned long long> ctl(static_cast<size_t>(tiles) + 3);
+  unsigned long long res[3] = {};
+  {
+    TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + 12.0 * A.nnz + 4.0 * (m_new + 1));
+    GP_CHECK(hipMemsetAsync(ctl.data(), 0, sizeof(unsigned long long) * (static_cast<size_t>(tiles) + 3), 0));
+    hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
+    const unsigned grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
+    hipLaunchKernelGGL(k_compact_tiles, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
+                       static_cast<int64_t>(A.nnz), tiles, static_cast<const int32_t*>(tile_row.data()), keep_row,
+                       row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
+    GP_LAUNCH_CHECK("compact");
+    d2h(res, ctl.data(), 3);
+  }
+  if (res[1] != 0) throw std::runtime_error("compaction look-back did not complete");
+  if (res[2] > static_cast<unsigned long long>(INT32_MAX)) throw std::length_error("compacted nnz exceeds int32 offsets");
+  out.nnz = static_cast<gp_offset>(res[2]);
 }
 
 // ---- packed masks ----------------------------------------------------------------------------
