@@ -275,14 +275,26 @@ def main():
         import pslp
 
         if pslp.available():
-            # the reference's bundled CPU presolver (PSLP 0.0.11) on the full LP
-            r = pslp.presolve(lp)
+            # the reference's bundled CPU presolver (PSLP 0.0.11) on the full LP;
+            # its C-level prints go to stderr so stdout keeps the one JSON line
+            sys.stdout.flush()
+            saved = os.dup(1)
+            os.dup2(2, 1)
+            try:
+                r = pslp.presolve(lp)
+            finally:
+                os.dup2(saved, 1)
+                os.close(saved)
+            st = r["status"]
+            outcome = ("declared the LP infeasible (a false positive: the generator's planted point satisfies "
+                       "every row and bound)" if st >= 0 and st & 2 else
+                       f"reduced it to {r['rows']}x{r['cols']} nnz={r['nnz']}")
             cpu = {"value": round(lp.nnz / r["wall_seconds"] / 1e6, 3), "unit": "Mnnz/s", "cores": 2,
                    "kind": "reference",
                    "sample": f"PSLP 0.0.11 (the reference's bundled CPU presolver, default settings) on the full "
                              f"{lp.m}x{lp.n} LP: new_presolver+run_presolver {r['wall_seconds']:.2f} s "
-                             f"(time_presolve {r['time_presolve']:.2f} s), reduced to {r['rows']}x{r['cols']} "
-                             f"nnz={r['nnz']}; one thread plus one helper thread during init",
+                             f"(time_presolve {r['time_presolve']:.2f} s); PSLP {outcome}; one thread plus one "
+                             f"helper thread during init",
                    "ms": round(r["wall_seconds"] * 1e3, 1)}
             log(f"cpu baseline (PSLP): {cpu}")
         import oracle as O

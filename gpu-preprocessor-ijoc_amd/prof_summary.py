@@ -20,6 +20,13 @@ import sys
 
 def short_name(name: str) -> str:
     n = name
+    if "rocprim::" in n:  # library kernels: the algorithm step and the key / value types
+        step = re.search(r"detail::(radix_sort_onesweep_iteration|radix_sort_onesweep_global_offsets"
+                         r"|radix_sort_block_sort|merge_sort_block_merge_impl|merge_\w+?_config_selector"
+                         r"|transform_impl)", n)
+        types = re.search(r"_config<[^,<>]*default_config, ([\w ]+), ([\w ]+)>", n)
+        out = "rocprim " + (step.group(1) if step else "kernel")
+        return out + (f"<{types.group(1)}, {types.group(2)}>" if types else "")
     if n.startswith("void "):
         n = n[5:]
     depth, cut = 0, len(n)
