@@ -438,6 +438,9 @@ void gather_compact(double* dst, const double* src, const int32_t* old_to_new, i
 void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* old_to_new, int32_t n_old);
 int32_t count_u8(const uint8_t* mask, int32_t n);
 
+// vmax[j] = max |a| over the entries of row j of AT whose row of A is live
+// (0 when there is none); entry-parallel tiles, order-free.
+void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax);
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
 // the histogram / atomic-scatter / segment-sort transpose (also for shapes whose

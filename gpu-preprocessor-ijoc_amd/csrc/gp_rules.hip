@@ -844,6 +844,53 @@ __global__ void k_bs_fixed_rows(gp_csr A, const uint8_t* fixed, const double* fv
     else wave_flag(flags + 3, 1);
   }
 }
+// The same shift and removal as k_bs_fixed_rows through the LDS row fold:
+// each column's bound pair is (fval, fval) and the fold's column mask is the
+// fixed bits, so the staged term of a fixed entry is a * fval in CSR order
+// with the same block positions.  A row's live count after the rule is its
+// live count at rule start minus its fixed entries (fixed columns were live
+// and are the only columns the rule removes).  Rows with no fixed entry keep
+// AL / AU as they are (x - (+0) == x).
+struct FixedRowsFold {
+  const uint8_t* keep_row;
+  const int32_t* row_nnz;
+  double ft;
+  uint8_t* keep_row_out;
+  double* AL;
+  double* AU;
+  int32_t* flags;
+  struct State {
+    RowAcc acc;
+    int32_t fixed;
+  };
+  __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
+  __device__ bool use(double) const { return true; }
+  __device__ void init(State& s) const {
+    s.acc = RowAcc{};
+    s.fixed = 0;
+  }
+  __device__ void add(State& s, int64_t k, double lo, double, bool skip) const {
+    s.acc.pos(k);
+    if (skip) return;
+    s.acc.add(lo);
+    ++s.fixed;
+  }
+  __device__ void finish(int64_t i, const State& s, bool active) const {
+    if (!active) return;
+    double al = AL[i], au = AU[i];
+    if (s.fixed > 0) {
+      const double sh = s.acc.value();
+      al -= sh;
+      au -= sh;
+      AL[i] = al;
+      AU[i] = au;
+    }
+    if (row_nnz[i] - s.fixed == 0) {
+      if (al <= ft && au >= -ft) keep_row_out[i] = 0;
+      else wave_flag(flags + 3, 1);
+    }
+  }
+};
 }  // namespace
 
 void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
@@ -912,9 +959,12 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     if (full) {
       // AT row_ptr, col_idx, vals, a mask bit per row, 8 bytes out per column
       TimedScope ts(GP_KT_COL_MAX, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + m / 8.0 + 8.0 * n);
-      launch_grouped<ColMaxAbsK>(pick_lanes(tuning().lanes_col_max, n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
-                                 live_rows.view(), cm.vmax.data());
-      GP_LAUNCH_CHECK("col max abs");
+      if (tuning().lanes_col_max == 0) {
+        col_max_abs_tiles(lp.AT, live_rows.view(), cm.vmax.data());
+      } else {
+        launch_grouped<ColMaxAbsK>(tuning().lanes_col_max, n, lp.AT, live_rows.view(), cm.vmax.data());
+        GP_LAUNCH_CHECK("col max abs");
+      }
     }
     if (masked) d2d(cm.keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
     cm.valid = true;
@@ -968,9 +1018,17 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   GP_LAUNCH_CHECK("bound apply");
   if (f2) {
     Flags f3;
-    hipLaunchKernelGGL(k_bs_fixed_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, fixed.data(), fval.data(),
-                       plan.keep_col_mask, p.feasibility_tol, plan.keep_row_mask, plan.new_AL, plan.new_AU,
-                       f3.d.data());
+    if (s.live_masks && s.row_nnz) {
+      const PackedBounds fv(fval.data(), fval.data(), n);
+      const PackedMask fixed_bits(fixed.data(), n);
+      launch_row_fold(lp.A, fv.data(), fixed_bits.view(),
+                      FixedRowsFold{plan.keep_row_mask, s.row_nnz, p.feasibility_tol, plan.keep_row_mask,
+                                    plan.new_AL, plan.new_AU, f3.d.data()});
+    } else {
+      hipLaunchKernelGGL(k_bs_fixed_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, fixed.data(), fval.data(),
+                         plan.keep_col_mask, p.feasibility_tol, plan.keep_row_mask, plan.new_AL, plan.new_AU,
+                         f3.d.data());
+    }
     GP_LAUNCH_CHECK("bound fixed rows");
     if (f3.read()[3]) {
       plan.has_infeasible = 1;

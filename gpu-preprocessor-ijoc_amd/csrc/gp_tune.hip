@@ -109,12 +109,15 @@ int gp_autotune(const gp_device_lp* lp, const gp_params* params, int32_t reps, g
     };
     for (int i = 0; i < 5; ++i) {
       Tuning t;
-      t.lanes_candidates = t.lanes_col_max = t.lanes_compact = t.lanes_live_stats = kLanes[i];
+      t.lanes_candidates = t.lanes_compact = t.lanes_live_stats = kLanes[i];
+      // column max: grouped lanes in the first passes, entry-parallel tiles (0) in the last
+      const int32_t col_max = i + 1 < 5 ? kLanes[i] : 0;
+      t.lanes_col_max = col_max;
       t.fold_rows = i < 3 ? kRows[i] : 0;
       tuning() = t;
       const Sample s = best_of(lp, p, reps);
       consider(0, kLanes[i], s.family[GP_KT_BOUND_CANDIDATES]);
-      consider(1, kLanes[i], s.family[GP_KT_COL_MAX]);
+      consider(1, col_max, s.family[GP_KT_COL_MAX]);
       consider(3, kLanes[i], s.family[GP_KT_LIVE_STATS]);
       if (i < 3)
         consider(4, kRows[i], s.family[GP_KT_ACTIVITY] + s.family[GP_KT_SC_ACTIVITY] + s.family[GP_KT_REDUNDANT]);

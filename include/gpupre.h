@@ -329,7 +329,8 @@ int gp_record_copy(const gp_summary* summary,
 /* Result-invariant launch schedule: lanes per row of the grouped sweeps
  * (0, 2, 4, 8, 16 or 64; 64 / lanes rows per wavefront) and rows per
  * workgroup of the row folds (0, 128, 256 or 512).  0 = derived from the
- * matrix's mean row length.  Process-wide. */
+ * matrix's mean row length; for lanes_col_max, 0 = entry-parallel tiles
+ * (no lanes per column).  Process-wide. */
 typedef struct gp_tuning {
   int32_t lanes_candidates;
   int32_t lanes_col_max;
