@@ -438,8 +438,105 @@ void gather_compact(double* dst, const double* src, const int32_t* old_to_new, i
 void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* old_to_new, int32_t n_old);
 int32_t count_u8(const uint8_t* mask, int32_t n);
 
+// ---- entry-parallel segment reductions ------------------------------------------
+// An order-free reduction over every row ("segment") of a CSR matrix M --
+// typically AT, whose rows are the columns of A -- that reads M entry-parallel
+// instead of giving each segment its own lanes, so short and long segments
+// cost the same per entry.  Tiles of kSegTile consecutive entries: each entry
+// maps to its segment's slot (the tile position of the segment's first entry
+// there), lanes combine Op::value(segment, inner index, value) into the slots with
+// LDS atomics, then a segment wholly inside the tile is stored and one that
+// crosses a tile edge is combined into the output with a global atomic.  The
+// output starts zero-filled, so 0 must be the identity of Op::combine and the
+// result of a segment with no entries.  Op supplies
+//   using Slot (a 32- or 64-bit unsigned integer)
+//   Slot value(int32_t segment, int32_t inner, double a)
+//   static void combine(Slot* s, Slot v)   -- atomic, for LDS and global memory
+constexpr int kSegTile = 2048;
+void seg_tile_first(const gp_csr& M, int32_t* tile_seg);  // tile -> segment of its first entry
+
+// Per tile the entry loads are issued before the slot map is built, so the
+// two global round trips overlap; whether a segment lies wholly in the tile
+// is recorded while the map is built, so the flush reads only LDS.
+template <class Op>
+__global__ void __launch_bounds__(256) k_seg_reduce_tiles(gp_csr M, const int32_t* tile_seg, Op op,
+                                                          typename Op::Slot* out) {
+  using Slot = typename Op::Slot;
+  __shared__ int32_t slot_of[kSegTile];
+  __shared__ int32_t seg_at[kSegTile];  // segment whose first entry in the tile is here, or -1
+  __shared__ uint8_t whole[kSegTile];   // that segment lies wholly in the tile
+  __shared__ Slot acc[kSegTile];
+  constexpr int kPerLane = kSegTile / 256;
+  const gp_offset* rp = M.row_ptr;
+  const int64_t nnz = M.nnz;
+  for (int q = threadIdx.x; q < kSegTile; q += 256) {
+    seg_at[q] = -1;
+    acc[q] = Slot(0);
+  }
+  __syncthreads();
+  for (int64_t tile = blockIdx.x; tile * kSegTile < nnz; tile += gridDim.x) {
+    const int64_t t0 = tile * kSegTile;
+    const int64_t t1 = t0 + kSegTile < nnz ? t0 + kSegTile : nnz;
+    int32_t inner[kPerLane];
+    double val[kPerLane];
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int64_t q = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      const int64_t at = q < t1 ? q : t0;
+      inner[k] = ld_stream(M.col_idx + at);
+      val[k] = ld_stream(M.vals + at);
+    }
+    for (int32_t r = tile_seg[tile] + static_cast<int32_t>(threadIdx.x); r < M.rows; r += 256) {
+      const int64_t ra = rp[r];
+      if (ra >= t1) break;
+      const int64_t rb = rp[r + 1];
+      const int64_t a = ra > t0 ? ra : t0, b = rb < t1 ? rb : t1;
+      if (a >= b) continue;
+      const int32_t slot = static_cast<int32_t>(a - t0);
+      seg_at[slot] = r;
+      whole[slot] = ra >= t0 && rb <= t1 ? 1 : 0;
+      for (int64_t q = a; q < b; ++q) slot_of[q - t0] = slot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int64_t q = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      if (q >= t1) continue;
+      const int32_t slot = slot_of[q - t0];
+      const Slot v = op.value(seg_at[slot], inner[k], val[k]);
+      if (v) Op::combine(&acc[slot], v);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kSegTile; q += 256) {
+      const int32_t r = seg_at[q];
+      if (r < 0) continue;
+      const Slot v = acc[q];
+      if (whole[q]) out[r] = v;
+      else if (v) Op::combine(out + r, v);
+      seg_at[q] = -1;
+      acc[q] = Slot(0);
+    }
+    __syncthreads();
+  }
+}
+
+// out[r] (M.rows slots) = the reduction of row r of M; zero-filled first.
+template <class Op>
+void seg_reduce_tiles(const gp_csr& M, const Op& op, typename Op::Slot* out) {
+  if (M.rows <= 0) return;
+  GP_CHECK(hipMemsetAsync(out, 0, sizeof(typename Op::Slot) * static_cast<size_t>(M.rows), 0));
+  const int64_t tiles = (static_cast<int64_t>(M.nnz) + kSegTile - 1) / kSegTile;
+  if (tiles == 0) return;
+  DVec<int32_t> tile_seg(static_cast<size_t>(tiles));
+  seg_tile_first(M, tile_seg.data());
+  const unsigned grid = static_cast<unsigned>(tiles < 16384 ? tiles : 16384);
+  hipLaunchKernelGGL(k_seg_reduce_tiles<Op>, dim3(grid), dim3(256), 0, 0, M,
+                     static_cast<const int32_t*>(tile_seg.data()), op, out);
+  GP_LAUNCH_CHECK("segment reduce tiles");
+}
+
 // vmax[j] = max |a| over the entries of row j of AT whose row of A is live
-// (0 when there is none); entry-parallel tiles, order-free.
+// (0 when there is none).
 void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax);
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool

@@ -680,31 +680,32 @@ __device__ inline void fill_row_of(const gp_offset* rp, int32_t m, int32_t i0, i
 // is used (branch-free: out-of-tile lanes read index 0 and are masked after).
 constexpr int kPer = kCompactTile / 256;
 struct TileEntries {
-  int32_t j[kPer];  // new column index, -1 when the entry is dropped or past the tile
-  __device__ void load(const int32_t* row_of, const int32_t* col, const uint8_t* keep_row, ColRank rank, int64_t t0,
-                       int64_t t1) {
-    int32_t c[kPer], r[kPer];
+  int32_t j[kPer];  // column index, then its new index (-1 when dropped or past the tile)
+  // column indices of the lane's entries (issued before the row map is built)
+  __device__ void load_cols(const int32_t* col, int64_t t0, int64_t t1) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      const bool in = p < t1;
-      c[k] = ld_stream(col + (in ? p : t0));
-      r[k] = row_of[in ? static_cast<int32_t>(p - t0) : 0];
+      j[k] = ld_stream(col + (p < t1 ? p : t0));
     }
+  }
+  __device__ void resolve(const int32_t* row_of, const uint8_t* keep_row, ColRank rank, int64_t t0, int64_t t1) {
     uint32_t w[kPer];
     int32_t pre[kPer];
     uint8_t kr[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      kr[k] = keep_row ? keep_row[r[k]] : 1;
-      w[k] = rank.words ? rank.words[c[k] >> 5] : ~0u;
-      pre[k] = rank.words ? rank.prefix[c[k] >> 5] : 0;
+      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
+      const int32_t r = row_of[p < t1 ? static_cast<int32_t>(p - t0) : 0];
+      kr[k] = keep_row ? keep_row[r] : 1;
+      w[k] = rank.words ? rank.words[j[k] >> 5] : ~0u;
+      pre[k] = rank.words ? rank.prefix[j[k] >> 5] : 0;
     }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      const uint32_t bit = 1u << (c[k] & 31);
-      const int32_t nj = rank.words ? pre[k] + __popc(w[k] & (bit - 1u)) : c[k];
+      const uint32_t bit = 1u << (j[k] & 31);
+      const int32_t nj = rank.words ? pre[k] + __popc(w[k] & (bit - 1u)) : j[k];
       j[k] = (p < t1 && kr[k] && (w[k] & bit)) ? nj : -1;
     }
   }
@@ -768,17 +769,18 @@ __global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, cons
     if (tile >= tiles) break;
     const int64_t t0 = tile * kCompactTile;
     const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
-    const int32_t i0 = tile_row[tile];
-    fill_row_of(rp, m, i0, t0, t1, row_of);
-    __syncthreads();
     TileEntries e;
-    e.load(row_of, col, keep_row, col_rank, t0, t1);
+    e.load_cols(col, t0, t1);
     double v[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
       v[k] = ld_stream(val + (p < t1 ? p : t0));
     }
+    const int32_t i0 = tile_row[tile];
+    fill_row_of(rp, m, i0, t0, t1, row_of);
+    __syncthreads();
+    e.resolve(row_of, keep_row, col_rank, t0, t1);
     unsigned long long bal[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -893,15 +895,8 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   out.nnz = static_cast<gp_offset>(res[2]);
 }
 
-// ---- entry-parallel column maxima ------------------------------------------------------------
+// ---- entry-parallel segment reductions -------------------------------------------------------
 namespace {
-// Tiles of kSegTile consecutive entries of AT (a row of AT is a column of A).
-// Each entry maps to the slot of its column in the tile (the tile position of
-// the column's first entry there); lanes combine |a| into the slots with LDS
-// 64-bit max, which orders nonnegative doubles as their bit patterns.  A
-// column wholly inside the tile is stored directly; one that crosses a tile
-// edge is combined into the zero-filled output with a global atomic max.
-constexpr int kSegTile = 2048;
 __global__ void k_seg_tile_first(const gp_offset* rp, int32_t m, int32_t* tile_seg) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= m) return;
@@ -910,68 +905,26 @@ __global__ void k_seg_tile_first(const gp_offset* rp, int32_t m, int32_t* tile_s
   for (int64_t t = (a + kSegTile - 1) / kSegTile; t * kSegTile < b; ++t) tile_seg[t] = static_cast<int32_t>(r);
 }
 
-__global__ void __launch_bounds__(256) k_col_max_tiles(gp_csr AT, const int32_t* tile_seg, Bits keep_row,
-                                                       unsigned long long* out) {
-  __shared__ int32_t slot_of[kSegTile];
-  __shared__ int32_t seg_at[kSegTile];
-  __shared__ unsigned long long best[kSegTile];
-  const gp_offset* rp = AT.row_ptr;
-  const int64_t nnz = AT.nnz;
-  for (int64_t tile = blockIdx.x; tile * kSegTile < nnz; tile += gridDim.x) {
-    const int64_t t0 = tile * kSegTile;
-    const int64_t t1 = t0 + kSegTile < nnz ? t0 + kSegTile : nnz;
-    for (int q = threadIdx.x; q < kSegTile; q += 256) {
-      seg_at[q] = -1;
-      best[q] = 0ull;
-    }
-    __syncthreads();
-    for (int32_t r = tile_seg[tile] + static_cast<int32_t>(threadIdx.x); r < AT.rows && rp[r] < t1; r += 256) {
-      const int64_t a = rp[r] > t0 ? rp[r] : t0, b = rp[r + 1] < t1 ? rp[r + 1] : t1;
-      if (a >= b) continue;
-      const int32_t slot = static_cast<int32_t>(a - t0);
-      seg_at[slot] = r;
-      for (int64_t q = a; q < b; ++q) slot_of[q - t0] = slot;
-    }
-    __syncthreads();
-    int32_t row[kSegTile / 256], slot[kSegTile / 256];
-    double val[kSegTile / 256];
-#pragma unroll
-    for (int k = 0; k < kSegTile / 256; ++k) {
-      const int64_t q = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      const bool in = q < t1;
-      row[k] = ld_stream(AT.col_idx + (in ? q : t0));
-      val[k] = ld_stream(AT.vals + (in ? q : t0));
-      slot[k] = in ? slot_of[q - t0] : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < kSegTile / 256; ++k)
-      if (slot[k] >= 0 && keep_row[row[k]])
-        atomicMax(&best[slot[k]], static_cast<unsigned long long>(__double_as_longlong(fabs(val[k]))));
-    __syncthreads();
-    for (int q = threadIdx.x; q < kSegTile; q += 256) {
-      const int32_t r = seg_at[q];
-      if (r < 0) continue;
-      const unsigned long long v = best[q];
-      if (rp[r] >= t0 && rp[r + 1] <= t1) out[r] = v;
-      else if (v) atomicMax(out + r, v);
-    }
-    __syncthreads();
+// max |a| over the entries whose row of A is live; nonnegative doubles order
+// as their bit patterns
+struct AbsMaxOp {
+  Bits keep_row;
+  using Slot = unsigned long long;
+  __device__ Slot value(int32_t, int32_t row, double a) const {
+    return keep_row[row] ? static_cast<Slot>(__double_as_longlong(fabs(a))) : 0ull;
   }
-}
+  __device__ static void combine(Slot* s, Slot v) { atomicMax(s, v); }
+};
 }  // namespace
 
+void seg_tile_first(const gp_csr& M, int32_t* tile_seg) {
+  if (M.rows <= 0) return;
+  hipLaunchKernelGGL(k_seg_tile_first, dim3(grid_for(M.rows)), dim3(kBlock), 0, 0, M.row_ptr, M.rows, tile_seg);
+  GP_LAUNCH_CHECK("segment tiles");
+}
+
 void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax) {
-  if (AT.rows <= 0) return;
-  GP_CHECK(hipMemsetAsync(vmax, 0, sizeof(double) * static_cast<size_t>(AT.rows), 0));
-  const int64_t tiles = (static_cast<int64_t>(AT.nnz) + kSegTile - 1) / kSegTile;
-  if (tiles == 0) return;
-  DVec<int32_t> tile_seg(static_cast<size_t>(tiles));
-  hipLaunchKernelGGL(k_seg_tile_first, dim3(grid_for(AT.rows)), dim3(kBlock), 0, 0, AT.row_ptr, AT.rows,
-                     tile_seg.data());
-  const unsigned grid = static_cast<unsigned>(tiles < 16384 ? tiles : 16384);
-  hipLaunchKernelGGL(k_col_max_tiles, dim3(grid), dim3(256), 0, 0, AT, static_cast<const int32_t*>(tile_seg.data()),
-                     keep_row, reinterpret_cast<unsigned long long*>(vmax));
-  GP_LAUNCH_CHECK("col max tiles");
+  seg_reduce_tiles(AT, AbsMaxOp{keep_row}, reinterpret_cast<unsigned long long*>(vmax));
 }
 
 // ---- packed masks ----------------------------------------------------------------------------

@@ -452,10 +452,37 @@ __global__ void k_sc_commit(int32_t m, int32_t n, const uint8_t* row_delete, con
 }
 
 // ============================================================== dual fix
-__global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_t* keep_row, const double* c,
-                                const double* l, const double* u, const double* AL, const double* AU, double zt,
-                                double bt, int32_t maxdeg, const int32_t* live_col_nnz, uint8_t* fixed, uint8_t* inf,
-                                uint8_t* ffix, double* val, double* contrib, int32_t* flags) {
+// Row sides for the dual fix: 4 for a live row, plus 1 when AL is finite and
+// 2 when AU is finite.
+__global__ void k_df_row_sides(int32_t m, const uint8_t* keep_row, const double* AL, const double* AU,
+                               uint8_t* side) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  side[i] = keep_row[i] ? static_cast<uint8_t>(4 | (isfinite(AL[i]) ? 1 : 0) | (isfinite(AU[i]) ? 2 : 0)) : 0;
+}
+
+// Per column, OR over its live entries: bit 0 ("down") when a finite row side
+// limits decreasing the column -- AL for a > zt, AU for a < -zt -- and bit 1
+// ("up") for the opposite side.  Order-free, so the column's entries are
+// reduced entry-parallel (seg_reduce_tiles over AT).
+struct DfSidesOp {
+  const uint8_t* side;
+  double zt;
+  using Slot = uint32_t;
+  __device__ Slot value(int32_t, int32_t r, double a) const {
+    const uint32_t f = side[r];
+    if (!(f & 4u)) return 0u;
+    if (a > zt) return f & 3u;
+    if (a < -zt) return ((f & 2u) >> 1) | ((f & 1u) << 1);
+    return 0u;
+  }
+  __device__ static void combine(Slot* s, Slot v) { atomicOr(s, v); }
+};
+
+__global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint32_t* sides, const double* c,
+                                const double* l, const double* u, double zt, double bt, int32_t maxdeg,
+                                const int32_t* live_col_nnz, uint8_t* fixed, uint8_t* inf, uint8_t* ffix,
+                                double* val, double* contrib, int32_t* flags) {
   const int64_t j = gtid();
   if (j >= AT.rows) return;
   fixed[j] = 0;
@@ -472,20 +499,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint8_
   const gp_offset a0 = AT.row_ptr[j], a1 = AT.row_ptr[j + 1];
   const int64_t degree = live_col_nnz ? static_cast<int64_t>(live_col_nnz[j]) : static_cast<int64_t>(a1 - a0);
   if (fabs(c[j]) > zt && degree > maxdeg) return;
-  bool down = false, up = false;
-  for (gp_offset q = a0; q < a1; ++q) {
-    const int32_t r = AT.col_idx[q];
-    if (!keep_row[r]) continue;
-    const double a = AT.vals[q];
-    if (a > zt) {
-      down = down || isfinite(AL[r]);
-      up = up || isfinite(AU[r]);
-    } else if (a < -zt) {
-      down = down || isfinite(AU[r]);
-      up = up || isfinite(AL[r]);
-    }
-    if (down && up) break;
-  }
+  const bool down = (sides[j] & 1u) != 0, up = (sides[j] & 2u) != 0;
   const double cj = c[j];
   bool fix = false, infinite = false;
   double v = 0.0;
@@ -1195,28 +1209,17 @@ __global__ void k_pc_apply(int32_t n, const uint8_t* col_delete, const uint8_t* 
 constexpr double kDblMaxRatioPivot = 1.0e3;
 constexpr int kDblMatchingRounds = 8;
 
-// live entries (|a| > zt, live row) per live column: G lanes per column, row
-// liveness from the packed row mask
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_db_colnnz(gp_csr AT, Bits keep_row, const uint8_t* keep_col, double zt,
-                                                       int32_t* nnz) {
-  const int lane = threadIdx.x % G;
-  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
-  for (int64_t j = gtid() / G; j < AT.rows; j += groups) {
-    int32_t c = 0;
-    if (keep_col[j]) {
-      const gp_offset p1 = AT.row_ptr[j + 1];
-      for (gp_offset q = AT.row_ptr[j] + lane; q < p1; q += G)
-        c += (keep_row[ld_stream(AT.col_idx + q)] && fabs(ld_stream(AT.vals + q)) > zt) ? 1 : 0;
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
-    if (lane == 0) nnz[j] = c;
+// live entries (|a| > zt, live row) per live column, entry-parallel
+// (seg_reduce_tiles over AT)
+struct DbColNnzOp {
+  Bits keep_row;
+  const uint8_t* keep_col;
+  double zt;
+  using Slot = uint32_t;
+  __device__ Slot value(int32_t j, int32_t r, double a) const {
+    return (keep_col[j] && keep_row[r] && fabs(a) > zt) ? 1u : 0u;
   }
-}
-template <int G>
-struct DbColNnzK {
-  static constexpr auto kernel = k_db_colnnz<G>;
+  __device__ static void combine(Slot* s, Slot v) { atomicAdd(s, v); }
 };
 
 __device__ inline bool db_integral(double num, double den, double tol) {
@@ -1796,10 +1799,20 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
   Flags flags;
   DVec<uint8_t> fixed(n), inf(n), ffix(n);
   DVec<double> val(n), contrib(n);
-  hipLaunchKernelGGL(k_df_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
-                     plan.keep_row_mask, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
-                     p.bound_tol, p.dual_fix_max_nonzero_cost_col_degree, s.live_masks ? s.col_nnz : nullptr,
-                     fixed.data(), inf.data(), ffix.data(), val.data(), contrib.data(), flags.d.data());
+  {
+    DVec<uint8_t> side(m ? m : 1);
+    DVec<uint32_t> sides(n);
+    if (m > 0) {
+      hipLaunchKernelGGL(k_df_row_sides, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, plan.keep_row_mask, plan.new_AL,
+                         plan.new_AU, side.data());
+      GP_LAUNCH_CHECK("dual fix row sides");
+    }
+    seg_reduce_tiles(lp.AT, DfSidesOp{side.data(), p.zero_tol}, sides.data());
+    hipLaunchKernelGGL(k_df_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
+                       static_cast<const uint32_t*>(sides.data()), plan.new_c, plan.new_l, plan.new_u, p.zero_tol,
+                       p.bound_tol, p.dual_fix_max_nonzero_cost_col_degree, s.live_masks ? s.col_nnz : nullptr,
+                       fixed.data(), inf.data(), ffix.data(), val.data(), contrib.data(), flags.d.data());
+  }
   GP_LAUNCH_CHECK("dual fix candidates");
   const int32_t* f = flags.read();
   if (f[0] == 1) {
@@ -2017,9 +2030,9 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   fill_u8(blocked.data(), 0, n);
   DbCand cand{ce.data(), ck.data(), cev.data(), ckv.data()};
   const PackedMask live_rows(plan.keep_row_mask, m);
-  launch_grouped<DbColNnzK>(lanes_for_mean(n ? static_cast<double>(AT.nnz) / n : 0.0), n, AT, live_rows.view(),
-                            static_cast<const uint8_t*>(plan.keep_col_mask), zt, col_nnz.data());
-  GP_LAUNCH_CHECK("doubleton column counts");
+  static_assert(sizeof(*col_nnz.data()) == sizeof(uint32_t), "column counts are 32-bit");
+  seg_reduce_tiles(AT, DbColNnzOp{live_rows.view(), plan.keep_col_mask, zt},
+                   reinterpret_cast<uint32_t*>(col_nnz.data()));
   hipLaunchKernelGGL(k_db_candidates, dim3(grid_for(m)), dim3(kBlock), 0, 0, A, AT, plan.keep_row_mask,
                      plan.keep_col_mask, plan.new_AL, plan.new_AU, col_nnz.data(), zt, bt, ratio_tol,
                      p.doubleton_equations_max_fill_in_proxy, cand, active.data(), flags.d.data());
