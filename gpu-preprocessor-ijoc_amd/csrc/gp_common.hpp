@@ -442,37 +442,44 @@ int32_t count_u8(const uint8_t* mask, int32_t n);
 // An order-free reduction over every row ("segment") of a CSR matrix M --
 // typically AT, whose rows are the columns of A -- that reads M entry-parallel
 // instead of giving each segment its own lanes, so short and long segments
-// cost the same per entry.  Tiles of kSegTile consecutive entries: each entry
-// maps to its segment's slot (the tile position of the segment's first entry
-// there), lanes combine Op::value(segment, inner index, value) into the slots with
-// LDS atomics, then a segment wholly inside the tile is stored and one that
-// crosses a tile edge is combined into the output with a global atomic.  The
-// output starts zero-filled, so 0 must be the identity of Op::combine and the
-// result of a segment with no entries.  Op supplies
+// cost the same per entry.  Tiles of kSegTile consecutive entries:
+//  * each segment met by the tile marks its first tile position in a
+//    kSegTile-bit head mask (one LDS OR per segment) and records whether it
+//    lies wholly in the tile;
+//  * an entry's slot is the last head at or before it: the highest set bit
+//    of its mask word up to it, else the carry of the words before (one
+//    wavefront scans the 64 words);
+//  * each wavefront reduces runs of equal slots with shuffles (its 64 lanes
+//    hold 64 consecutive entries), and the first lane of each run combines
+//    the run into the slot with an LDS atomic;
+//  * a segment wholly in the tile is stored, one that crosses a tile edge is
+//    combined into the output with a global atomic.
+// The output starts zero-filled, so 0 must be the identity of the reduction
+// and the result of a segment with no entries.  Op supplies
 //   using Slot (a 32- or 64-bit unsigned integer)
 //   Slot value(int32_t segment, int32_t inner, double a)
-//   static void combine(Slot* s, Slot v)   -- atomic, for LDS and global memory
+//   static Slot merge(Slot a, Slot b)      -- the reduction of two values
+//   static void combine(Slot* s, Slot v)   -- atomic merge, LDS or global memory
 constexpr int kSegTile = 2048;
 void seg_tile_first(const gp_csr& M, int32_t* tile_seg);  // tile -> segment of its first entry
 
-// Per tile the entry loads are issued before the slot map is built, so the
-// two global round trips overlap; whether a segment lies wholly in the tile
-// is recorded while the map is built, so the flush reads only LDS.
 template <class Op>
 __global__ void __launch_bounds__(256) k_seg_reduce_tiles(gp_csr M, const int32_t* tile_seg, Op op,
                                                           typename Op::Slot* out) {
   using Slot = typename Op::Slot;
-  __shared__ int32_t slot_of[kSegTile];
-  __shared__ int32_t seg_at[kSegTile];  // segment whose first entry in the tile is here, or -1
+  constexpr int kWords = kSegTile / 32;
+  static_assert(kWords == 64, "one wavefront scans the head words");
+  __shared__ int32_t seg_at[kSegTile];  // segment whose first entry in the tile is here
   __shared__ uint8_t whole[kSegTile];   // that segment lies wholly in the tile
   __shared__ Slot acc[kSegTile];
+  __shared__ uint32_t head[kWords];
+  __shared__ int32_t carry[kWords];     // last head before each word, -1 for none
   constexpr int kPerLane = kSegTile / 256;
+  const int lane = static_cast<int>(__lane_id());
   const gp_offset* rp = M.row_ptr;
   const int64_t nnz = M.nnz;
-  for (int q = threadIdx.x; q < kSegTile; q += 256) {
-    seg_at[q] = -1;
-    acc[q] = Slot(0);
-  }
+  for (int q = threadIdx.x; q < kSegTile; q += 256) acc[q] = Slot(0);
+  if (threadIdx.x < kWords) head[threadIdx.x] = 0u;
   __syncthreads();
   for (int64_t tile = blockIdx.x; tile * kSegTile < nnz; tile += gridDim.x) {
     const int64_t t0 = tile * kSegTile;
@@ -495,27 +502,49 @@ __global__ void __launch_bounds__(256) k_seg_reduce_tiles(gp_csr M, const int32_
       const int32_t slot = static_cast<int32_t>(a - t0);
       seg_at[slot] = r;
       whole[slot] = ra >= t0 && rb <= t1 ? 1 : 0;
-      for (int64_t q = a; q < b; ++q) slot_of[q - t0] = slot;
+      atomicOr(&head[slot >> 5], 1u << (slot & 31));
+    }
+    __syncthreads();
+    if (threadIdx.x < kWords) {
+      const uint32_t w = head[threadIdx.x];
+      int32_t last = w ? static_cast<int32_t>(threadIdx.x) * 32 + 31 - __clz(static_cast<int>(w)) : -1;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int32_t up = __shfl_up(last, o, 64);
+        if (lane >= o && up > last) last = up;
+      }
+      const int32_t before = __shfl_up(last, 1, 64);
+      carry[threadIdx.x] = lane == 0 ? -1 : before;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPerLane; ++k) {
-      const int64_t q = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      if (q >= t1) continue;
-      const int32_t slot = slot_of[q - t0];
-      const Slot v = op.value(seg_at[slot], inner[k], val[k]);
-      if (v) Op::combine(&acc[slot], v);
+      const int32_t off = k * 256 + static_cast<int32_t>(threadIdx.x);
+      const bool in = t0 + off < t1;
+      const uint32_t upto = head[off >> 5] & (0xffffffffu >> (31 - (off & 31)));
+      const int32_t slot = !in ? -1 - lane : (upto ? (off & ~31) + 31 - __clz(static_cast<int>(upto)) : carry[off >> 5]);
+      Slot v = op.value(seg_at[in ? slot : 0], inner[k], val[k]);  // slot 0 always holds a head
+      if (!in) v = Slot(0);
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const Slot y = __shfl_down(v, o, 64);
+        const int32_t ys = __shfl_down(slot, o, 64);
+        if (lane + o < 64 && ys == slot) v = Op::merge(v, y);
+      }
+      const int32_t prev = __shfl_up(slot, 1, 64);
+      if (in && (lane == 0 || prev != slot) && v) Op::combine(&acc[slot], v);
     }
     __syncthreads();
     for (int q = threadIdx.x; q < kSegTile; q += 256) {
+      if (!((head[q >> 5] >> (q & 31)) & 1u)) continue;
       const int32_t r = seg_at[q];
-      if (r < 0) continue;
       const Slot v = acc[q];
       if (whole[q]) out[r] = v;
       else if (v) Op::combine(out + r, v);
-      seg_at[q] = -1;
       acc[q] = Slot(0);
     }
+    __syncthreads();
+    if (threadIdx.x < kWords) head[threadIdx.x] = 0u;
     __syncthreads();
   }
 }

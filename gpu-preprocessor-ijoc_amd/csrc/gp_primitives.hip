@@ -913,6 +913,7 @@ struct AbsMaxOp {
   __device__ Slot value(int32_t, int32_t row, double a) const {
     return keep_row[row] ? static_cast<Slot>(__double_as_longlong(fabs(a))) : 0ull;
   }
+  __device__ static Slot merge(Slot a, Slot b) { return a > b ? a : b; }
   __device__ static void combine(Slot* s, Slot v) { atomicMax(s, v); }
 };
 }  // namespace

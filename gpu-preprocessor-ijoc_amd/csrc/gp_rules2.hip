@@ -476,6 +476,7 @@ struct DfSidesOp {
     if (a < -zt) return ((f & 2u) >> 1) | ((f & 1u) << 1);
     return 0u;
   }
+  __device__ static Slot merge(Slot a, Slot b) { return a | b; }
   __device__ static void combine(Slot* s, Slot v) { atomicOr(s, v); }
 };
 
@@ -1219,6 +1220,7 @@ struct DbColNnzOp {
   __device__ Slot value(int32_t j, int32_t r, double a) const {
     return (keep_col[j] && keep_row[r] && fabs(a) > zt) ? 1u : 0u;
   }
+  __device__ static Slot merge(Slot a, Slot b) { return a + b; }
   __device__ static void combine(Slot* s, Slot v) { atomicAdd(s, v); }
 };
 
