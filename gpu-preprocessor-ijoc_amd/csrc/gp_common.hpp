@@ -600,15 +600,18 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
 
 // Kept rows of A with an entry in a column whose col_mark byte is set, appended
 // to list (any order); *count (device, caller-zeroed) receives their number.
-// G lanes per row test one bit per entry of a packed copy of col_mark.
-void list_rows_touching(const gp_csr& A, const uint8_t* keep_row, const uint8_t* col_mark, int32_t* list,
-                        int32_t* count);
+// With AT (the transpose of A, or null), when the marked columns hold less
+// than a quarter of the entries, their entries are walked in AT; otherwise G
+// lanes per row of A test one bit per entry of a packed copy of col_mark.  The
+// choice is made on the device.
+void list_rows_touching(const gp_csr& A, const gp_csr* AT, const uint8_t* keep_row, const uint8_t* col_mark,
+                        int32_t* list, int32_t* count);
 // AL/AU -= the blocked CSR-order sum of a * val[col] over the entries of each
 // kept row whose column has sel[col] set.  Rows without such an entry are left
 // as they are (x - (+0) == x), so only the rows listed by list_rows_touching
 // are folded.
-void shift_rows_by_columns(const gp_csr& A, const uint8_t* keep_row, const uint8_t* sel, const double* val,
-                           double* AL, double* AU);
+void shift_rows_by_columns(const gp_csr& A, const gp_csr* AT, const uint8_t* keep_row, const uint8_t* sel,
+                           const double* val, double* AL, double* AU);
 
 // ---- batched presolve (gp_batch.hip) --------------------------------------------------------
 // A batch is presolved as the block-diagonal direct sum of its instances.  The

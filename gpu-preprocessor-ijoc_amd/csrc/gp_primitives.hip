@@ -954,11 +954,76 @@ void PackedMask::pack(const uint8_t* mask, int64_t n) {
 
 // ---- rows touching marked columns ------------------------------------------------------------
 namespace {
+// Two ways to list the live rows that have an entry in a marked column, chosen
+// on the device without a host read: k_marked_degree sums the entries of the
+// marked columns; below a quarter of nnz the marked columns' entries are
+// walked in AT (k_mark_rows_at flags the rows, k_list_rows lists the flags),
+// otherwise k_list_rows sweeps A testing one mask bit per entry.  Each kernel
+// returns at once when the other way was chosen.  The listed set is the same
+// either way; its order is not defined (listed rows are processed
+// independently).
+__global__ void k_marked_degree(gp_csr AT, const uint8_t* col_mark, unsigned long long* total) {
+  unsigned long long acc = 0;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < AT.rows;
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    if (col_mark[j]) acc += static_cast<unsigned long long>(AT.row_ptr[j + 1] - AT.row_ptr[j]);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (__lane_id() == 0 && acc) atomicAdd(total, acc);
+}
+
+// gate: run only when (*walk < limit) == want_walk
+__device__ inline bool gate_open(const unsigned long long* walk, unsigned long long limit, bool want_walk) {
+  return !walk || ((*walk < limit) == want_walk);
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_mark_rows_at(gp_csr AT, const uint8_t* col_mark, const uint8_t* keep_row,
+                                                          const unsigned long long* walk, unsigned long long limit,
+                                                          uint8_t* row_flag) {
+  if (!gate_open(walk, limit, true)) return;
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
+  for (int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; j < AT.rows; j += groups) {
+    if (!col_mark[j]) continue;
+    const gp_offset p1 = AT.row_ptr[j + 1];
+    for (gp_offset p = AT.row_ptr[j] + lane; p < p1; p += G) {
+      const int32_t r = AT.col_idx[p];
+      if (!keep_row || keep_row[r]) row_flag[r] = 1;
+    }
+  }
+}
+
+// row hit: some entry of a live row lies in a marked column (G lanes per row)
+struct HitMarkedEntry {
+  gp_csr A;
+  const uint8_t* keep_row;
+  Bits mark;
+  template <int G>
+  __device__ int eval(int64_t i, int lane) const {
+    int hit = 0;
+    if (!keep_row || keep_row[i]) {
+      const gp_offset p1 = A.row_ptr[i + 1];
+      for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) hit |= mark[A.col_idx[p]] ? 1 : 0;
+    }
+    return hit;
+  }
+};
+// row hit: the row's flag is set
+struct HitFlag {
+  const uint8_t* flag;
+  template <int G>
+  __device__ int eval(int64_t i, int lane) const {
+    return lane == 0 && flag[i] ? 1 : 0;
+  }
+};
+
 // Listed rows are staged in LDS and flushed with one global atomic per
 // workgroup flush, so the list counter is not a hot spot.
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_t* keep_row, Bits mark,
-                                                           int32_t* list, int32_t* count) {
+template <int G, class Hit>
+__global__ void __launch_bounds__(kBlock) k_list_rows(int64_t rows, Hit hit_of, const unsigned long long* walk,
+                                                       unsigned long long limit, bool want_walk, int32_t* list,
+                                                       int32_t* count) {
+  if (!gate_open(walk, limit, want_walk)) return;
   constexpr int kRowsPerIter = kBlock / G;
   constexpr int kStage = 2048;
   __shared__ int32_t s_rows[kStage];
@@ -966,21 +1031,17 @@ __global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
-  for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kRowsPerIter; b0 < A.rows;
+  for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kRowsPerIter; b0 < rows;
        b0 += static_cast<int64_t>(gridDim.x) * kRowsPerIter) {
     const int64_t i = b0 + grp;
-    int hit = 0;
-    if (i < A.rows && (!keep_row || keep_row[i])) {
-      const gp_offset p1 = A.row_ptr[i + 1];
-      for (gp_offset p = A.row_ptr[i] + lane; p < p1; p += G) hit |= mark[A.col_idx[p]] ? 1 : 0;
-    }
+    int hit = i < rows ? hit_of.template eval<G>(i, lane) : 0;
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, G);
     if (lane == 0 && hit) s_rows[atomicAdd(&s_n, 1)] = static_cast<int32_t>(i);
     __syncthreads();
     const int32_t staged = s_n;  // every thread reads the same value before anyone changes it
     __syncthreads();
-    const bool last = b0 + static_cast<int64_t>(gridDim.x) * kRowsPerIter >= A.rows;
+    const bool last = b0 + static_cast<int64_t>(gridDim.x) * kRowsPerIter >= rows;
     if (staged > kStage - kRowsPerIter || (last && staged > 0)) {  // flush (uniform across the block)
       if (threadIdx.x == 0) s_base = atomicAdd(count, staged);
       __syncthreads();
@@ -991,12 +1052,14 @@ __global__ void __launch_bounds__(kBlock) k_rows_touching(gp_csr A, const uint8_
     }
   }
 }
-static_assert(kBlock % 64 == 0, "rows-touching staging assumes whole wavefronts");
+static_assert(kBlock % 64 == 0, "row-list staging assumes whole wavefronts");
 
 template <int G>
-void launch_rows_touching(const gp_csr& A, const uint8_t* keep_row, Bits mark, int32_t* list, int32_t* count) {
+void launch_rows_touching(const gp_csr& A, const uint8_t* keep_row, Bits mark, const unsigned long long* walk,
+                          unsigned long long limit, int32_t* list, int32_t* count) {
   const unsigned grid = grid_capped(static_cast<int64_t>(A.rows) * G, kBlock, 16384);
-  hipLaunchKernelGGL(k_rows_touching<G>, dim3(grid), dim3(kBlock), 0, 0, A, keep_row, mark, list, count);
+  hipLaunchKernelGGL((k_list_rows<G, HitMarkedEntry>), dim3(grid), dim3(kBlock), 0, 0, static_cast<int64_t>(A.rows),
+                     HitMarkedEntry{A, keep_row, mark}, walk, limit, false, list, count);
 }
 
 __global__ void k_shift_listed(gp_csr A, const int32_t* list, const int32_t* count, const uint8_t* sel,
@@ -1019,26 +1082,46 @@ __global__ void k_shift_listed(gp_csr A, const int32_t* list, const int32_t* cou
 }
 }  // namespace
 
-void list_rows_touching(const gp_csr& A, const uint8_t* keep_row, const uint8_t* col_mark, int32_t* list,
-                        int32_t* count) {
+void list_rows_touching(const gp_csr& A, const gp_csr* AT, const uint8_t* keep_row, const uint8_t* col_mark,
+                        int32_t* list, int32_t* count) {
   if (A.rows <= 0) return;
   const PackedMask mark(col_mark, A.cols);
+  DVec<unsigned long long> walk;
+  DVec<uint8_t> row_flag;
+  const unsigned long long limit = static_cast<unsigned long long>(A.nnz) / 4;
+  const bool can_walk = AT && AT->rows == A.cols && col_mark;
+  if (can_walk) {
+    walk.resize(1);
+    row_flag.resize(static_cast<size_t>(A.rows));
+    GP_CHECK(hipMemsetAsync(walk.data(), 0, sizeof(unsigned long long), 0));
+    GP_CHECK(hipMemsetAsync(row_flag.data(), 0, static_cast<size_t>(A.rows), 0));
+    hipLaunchKernelGGL(k_marked_degree, dim3(grid_capped(AT->rows, kBlock, 1024)), dim3(kBlock), 0, 0, *AT, col_mark,
+                       walk.data());
+    hipLaunchKernelGGL(k_mark_rows_at<4>, dim3(grid_capped(static_cast<int64_t>(AT->rows) * 4, kBlock, 16384)),
+                       dim3(kBlock), 0, 0, *AT, col_mark, keep_row, static_cast<const unsigned long long*>(walk.data()),
+                       limit, row_flag.data());
+    hipLaunchKernelGGL((k_list_rows<1, HitFlag>), dim3(grid_capped(A.rows, kBlock, 2048)), dim3(kBlock), 0, 0,
+                       static_cast<int64_t>(A.rows), HitFlag{row_flag.data()},
+                       static_cast<const unsigned long long*>(walk.data()), limit, true, list, count);
+    GP_LAUNCH_CHECK("rows touching columns (walk)");
+  }
+  const unsigned long long* gate = can_walk ? walk.data() : nullptr;
   switch (lanes_for_mean(static_cast<double>(A.nnz) / A.rows)) {
-    case 2: launch_rows_touching<2>(A, keep_row, mark.view(), list, count); break;
-    case 4: launch_rows_touching<4>(A, keep_row, mark.view(), list, count); break;
-    case 8: launch_rows_touching<8>(A, keep_row, mark.view(), list, count); break;
-    case 16: launch_rows_touching<16>(A, keep_row, mark.view(), list, count); break;
-    default: launch_rows_touching<64>(A, keep_row, mark.view(), list, count); break;
+    case 2: launch_rows_touching<2>(A, keep_row, mark.view(), gate, limit, list, count); break;
+    case 4: launch_rows_touching<4>(A, keep_row, mark.view(), gate, limit, list, count); break;
+    case 8: launch_rows_touching<8>(A, keep_row, mark.view(), gate, limit, list, count); break;
+    case 16: launch_rows_touching<16>(A, keep_row, mark.view(), gate, limit, list, count); break;
+    default: launch_rows_touching<64>(A, keep_row, mark.view(), gate, limit, list, count); break;
   }
   GP_LAUNCH_CHECK("rows touching columns");
 }
 
-void shift_rows_by_columns(const gp_csr& A, const uint8_t* keep_row, const uint8_t* sel, const double* val,
-                           double* AL, double* AU) {
+void shift_rows_by_columns(const gp_csr& A, const gp_csr* AT, const uint8_t* keep_row, const uint8_t* sel,
+                           const double* val, double* AL, double* AU) {
   if (A.rows <= 0) return;
   DVec<int32_t> list(static_cast<size_t>(A.rows)), count(1);
   GP_CHECK(hipMemsetAsync(count.data(), 0, sizeof(int32_t), 0));
-  list_rows_touching(A, keep_row, sel, list.data(), count.data());
+  list_rows_touching(A, AT, keep_row, sel, list.data(), count.data());
   hipLaunchKernelGGL(k_shift_listed, dim3(grid_capped(A.rows, kBlock, 4096)), dim3(kBlock), 0, 0, A, list.data(),
                      count.data(), sel, val, AL, AU);
   GP_LAUNCH_CHECK("shift listed rows");

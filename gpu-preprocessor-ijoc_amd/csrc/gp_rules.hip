@@ -119,7 +119,8 @@ void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params
   const double obj = fixed_variable_records(tape_on(plan, p) && p.record_postsolve_tape_cpu ? plan_tape(plan) : nullptr,
                                             fixed.data(), val.data(), plan.keep_row_mask, plan.new_c, lp.AT, n, true);
   if (m > 0) {
-    shift_rows_by_columns(lp.A, plan.keep_row_mask, fixed.data(), val.data(), plan.new_AL, plan.new_AU);
+    shift_rows_by_columns(lp.A, &lp.AT, plan.keep_row_mask, fixed.data(), val.data(), plan.new_AL,
+                          plan.new_AU);
     GP_LAUNCH_CHECK("fixed row shift");
   }
   hipLaunchKernelGGL(k_fix_columns, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, fixed.data(), val.data(), plan.new_l,

@@ -1838,7 +1838,7 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
                        touched.data());
     DVec<int32_t> list(m);
     int32_t* count = flags.d.data() + 4;
-    list_rows_touching(lp.A, plan.keep_row_mask, touched.data(), list.data(), count);
+    list_rows_touching(lp.A, &lp.AT, plan.keep_row_mask, touched.data(), list.data(), count);
     hipLaunchKernelGGL(k_df_rows_list, dim3(grid_capped(m, kBlock, 4096)), dim3(kBlock), 0, 0, lp.A, list.data(),
                        count, ffix.data(), inf.data(), val.data(), shift.data(), row_remove.data(), keep_after.data(),
                        flags.d.data());
@@ -2000,7 +2000,8 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
     obj = fixed_variable_records(tape_on(plan, p) ? plan_tape(plan) : nullptr, fixed.data(), fval.data(),
                                  plan.keep_row_mask, plan.new_c, lp.AT, n, true);
     if (m > 0) {
-      shift_rows_by_columns(lp.A, plan.keep_row_mask, fixed.data(), fval.data(), plan.new_AL, plan.new_AU);
+      shift_rows_by_columns(lp.A, &lp.AT, plan.keep_row_mask, fixed.data(), fval.data(), plan.new_AL,
+                            plan.new_AU);
       GP_LAUNCH_CHECK("parallel columns row shift");
     }
     plan.has_row_action = 1;
@@ -2095,7 +2096,7 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
     hipLaunchKernelGGL(k_db_pair_mark, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, pair_of.data(), mark.data());
     DVec<int32_t> list(m), count(1);
     GP_CHECK(hipMemsetAsync(count.data(), 0, sizeof(int32_t), 0));
-    list_rows_touching(A, plan.keep_row_mask, mark.data(), list.data(), count.data());
+    list_rows_touching(A, &AT, plan.keep_row_mask, mark.data(), list.data(), count.data());
     hipLaunchKernelGGL(k_db_shift, dim3(grid_capped(m, kBlock, 4096)), dim3(kBlock), 0, 0, A, list.data(), count.data(),
                        pair_of.data(), P, cand, plan.new_AU, plan.new_l, zt, shift.data());
     GP_LAUNCH_CHECK("doubleton row shift");
