@@ -348,16 +348,15 @@ __global__ void k_gather_compact(T* dst, const T* src, const int32_t* o2n, int32
     if (k >= 0) dst[k] = src[i];
   }
 }
+// grid-stride, one atomic per wavefront (a block per 256 bytes would hit the
+// counter from tens of thousands of blocks)
 __global__ void k_count_u8(const uint8_t* mask, int32_t n, int32_t* out) {
-  __shared__ int32_t lds[kBlock];
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  lds[threadIdx.x] = (i < n && mask[i]) ? 1 : 0;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && lds[0]) atomicAdd(out, lds[0]);
+  int32_t c = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    c += mask[i] ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (__lane_id() == 0 && c) atomicAdd(out, c);
 }
 }  // namespace
 
@@ -383,7 +382,7 @@ int32_t count_u8(const uint8_t* mask, int32_t n) {
   if (n <= 0) return 0;
   DVec<int32_t> c(1);
   GP_CHECK(hipMemsetAsync(c.data(), 0, sizeof(int32_t), 0));
-  hipLaunchKernelGGL(k_count_u8, dim3(grid_for(n)), dim3(kBlock), 0, 0, mask, n, c.data());
+  hipLaunchKernelGGL(k_count_u8, dim3(grid_capped(n, kBlock, 1024)), dim3(kBlock), 0, 0, mask, n, c.data());
   GP_LAUNCH_CHECK("count u8");
   return read1(c.data());
 }
@@ -930,14 +929,12 @@ void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax) {
 
 // ---- packed masks ----------------------------------------------------------------------------
 namespace {
+// one byte per lane; a wavefront's ballot is two 32-bit words
 __global__ void k_pack_bits(const uint8_t* mask, int64_t n, uint32_t* words) {
-  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t base = w * 32;
-  if (base >= n) return;
-  uint32_t bits = 0;
-  const int64_t lim = n - base < 32 ? n - base : 32;
-  for (int64_t k = 0; k < lim; ++k) bits |= (mask[base + k] ? 1u : 0u) << k;
-  words[w] = bits;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const unsigned long long b = __ballot(i < n && mask[i] != 0);
+  const int lane = static_cast<int>(__lane_id());
+  if ((lane == 0 || lane == 32) && i < n) words[i >> 5] = static_cast<uint32_t>(lane == 0 ? b : b >> 32);
 }
 }  // namespace
 
@@ -947,7 +944,7 @@ void PackedMask::pack(const uint8_t* mask, int64_t n) {
   const int64_t nw = (n + 31) / 32;
   words.resize(static_cast<size_t>(nw ? nw : 1));
   if (nw > 0) {
-    hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(nw)), dim3(kBlock), 0, 0, mask, n, words.data());
+    hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(nw * 32)), dim3(kBlock), 0, 0, mask, n, words.data());
     GP_LAUNCH_CHECK("pack bits");
   }
 }
