@@ -434,6 +434,13 @@ struct PackedBounds {
 void sort_keys_u64(uint64_t* keys, int64_t n, int end_bit = 64);
 void sort_pairs_u64_i32(uint64_t* keys, int32_t* vals, int64_t n, int end_bit = 64);
 void sort_pairs_u64_f64(uint64_t* keys, double* vals, int64_t n, int end_bit = 64);
+// (row, value) payload of the transpose sort
+struct RowVal {
+  int32_t row;
+  int32_t pad;
+  double val;
+};
+void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit = 32);
 void gather_compact(double* dst, const double* src, const int32_t* old_to_new, int32_t n_old);
 void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* old_to_new, int32_t n_old);
 int32_t count_u8(const uint8_t* mask, int32_t n);
@@ -569,8 +576,8 @@ void seg_reduce_tiles(const gp_csr& M, const Op& op, typename Op::Slot* out) {
 void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax);
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
-// the histogram / atomic-scatter / segment-sort transpose (also for shapes whose
-// (column, row) key exceeds 64 bits)
+// the histogram / atomic-scatter / segment-sort transpose (for matrices with more
+// entries than one radix sort takes, 2^31 - 1)
 void transpose_csr_scatter(const gp_csr& A, gp_csr& AT);
 // Stable compaction by keep masks (null = keep all); row_old_to_new is the
 // exclusive prefix of keep_row (null with keep_row null).  Column indices are

@@ -499,26 +499,31 @@ __global__ void k_i32_to_offset_shift(const int32_t* src, int32_t n, gp_offset* 
 // LSD radix sort of (key, value) pairs, then row indices and the column
 // starts read off the sorted keys.  Entries of a column come out in row
 // order, so the result is the unique sorted CSR of A^T.
+// Key = column, payload = (row, value): the entries arrive in CSR order and
+// the radix sort is stable, so within a column they stay in increasing row
+// order -- the unique CSR of A^T -- with only the column bits sorted.
 template <int G>
-__global__ void __launch_bounds__(kBlock) k_transpose_keys(const gp_offset* rp, const int32_t* col, int32_t m,
-                                                            int row_bits, uint64_t* keys) {
+__global__ void __launch_bounds__(kBlock) k_transpose_keys(const gp_offset* rp, const int32_t* col, const double* val,
+                                                            int32_t m, uint32_t* keys, RowVal* rv) {
   const int lane = threadIdx.x % G;
   const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
   for (int64_t i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; i < m; i += groups) {
     const gp_offset p1 = rp[i + 1];
-    for (gp_offset p = rp[i] + lane; p < p1; p += G)
-      keys[p] = (static_cast<uint64_t>(static_cast<uint32_t>(ld_stream(col + p))) << row_bits) |
-                static_cast<uint64_t>(i);
+    for (gp_offset p = rp[i] + lane; p < p1; p += G) {
+      keys[p] = static_cast<uint32_t>(ld_stream(col + p));
+      rv[p] = RowVal{static_cast<int32_t>(i), 0, ld_stream(val + p)};
+    }
   }
 }
-__global__ void k_transpose_out(const uint64_t* keys, int64_t nnz, int32_t n, int row_bits, int32_t* at_col,
-                                gp_offset* at_rp) {
+__global__ void k_transpose_out(const uint32_t* keys, const RowVal* rv, int64_t nnz, int32_t n, int32_t* at_col,
+                                double* at_val, gp_offset* at_rp) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= nnz) return;
-  const uint64_t k = keys[p];
-  const int64_t c = static_cast<int64_t>(k >> row_bits);
-  at_col[p] = static_cast<int32_t>(k & ((1ull << row_bits) - 1ull));
-  const int64_t prev = p > 0 ? static_cast<int64_t>(keys[p - 1] >> row_bits) : -1;
+  const int64_t c = static_cast<int64_t>(keys[p]);
+  const RowVal e = rv[p];
+  at_col[p] = e.row;
+  at_val[p] = e.val;
+  const int64_t prev = p > 0 ? static_cast<int64_t>(keys[p - 1]) : -1;
   for (int64_t j = prev + 1; j <= c; ++j) at_rp[j] = static_cast<gp_offset>(p);  // columns starting here
   if (p == nnz - 1)
     for (int64_t j = c + 1; j <= n; ++j) at_rp[j] = static_cast<gp_offset>(nnz);  // trailing empty columns
@@ -532,8 +537,8 @@ int bits_for(int64_t v) {  // bits needed for values 0..v-1 (at least 1)
 
 void transpose_csr(const gp_csr& A, gp_csr& AT) {
   const int32_t m = A.rows, n = A.cols;
-  const int rb = bits_for(m), cb = bits_for(n);
-  if (A.nnz > 0 && rb + cb <= 64) {
+  if (A.nnz > 0 && static_cast<int64_t>(A.nnz) <= INT32_MAX) {
+    // CSR in, keys + payload (4 + 16 B per entry) out and back, CSR of A^T out
     TimedScope ts(GP_KT_TRANSPOSE, 2.0 * (12.0 * A.nnz) + 4.0 * (A.rows + 1) + 4.0 * (A.cols + 1));
     const int64_t nnz = A.nnz;
     AT.rows = n;
@@ -542,20 +547,20 @@ void transpose_csr(const gp_csr& A, gp_csr& AT) {
     AT.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(n) + 1);
     AT.col_idx = pool_alloc<int32_t>(static_cast<size_t>(nnz));
     AT.vals = pool_alloc<double>(static_cast<size_t>(nnz));
-    DVec<uint64_t> keys(static_cast<size_t>(nnz));
+    DVec<uint32_t> keys(static_cast<size_t>(nnz));
+    DVec<RowVal> rv(static_cast<size_t>(nnz));
     const unsigned grid = grid_capped(static_cast<int64_t>(m) * 8, kBlock, 16384);
     switch (lanes_for_mean(m ? static_cast<double>(nnz) / m : 0.0)) {
-      case 2: hipLaunchKernelGGL(k_transpose_keys<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, rb, keys.data()); break;
-      case 4: hipLaunchKernelGGL(k_transpose_keys<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, rb, keys.data()); break;
-      case 8: hipLaunchKernelGGL(k_transpose_keys<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, rb, keys.data()); break;
-      case 16: hipLaunchKernelGGL(k_transpose_keys<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, rb, keys.data()); break;
-      default: hipLaunchKernelGGL(k_transpose_keys<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, m, rb, keys.data()); break;
+      case 2: hipLaunchKernelGGL(k_transpose_keys<2>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
+      case 4: hipLaunchKernelGGL(k_transpose_keys<4>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
+      case 8: hipLaunchKernelGGL(k_transpose_keys<8>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
+      case 16: hipLaunchKernelGGL(k_transpose_keys<16>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
+      default: hipLaunchKernelGGL(k_transpose_keys<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
     }
     GP_LAUNCH_CHECK("transpose keys");
-    d2d(AT.vals, A.vals, static_cast<size_t>(nnz));
-    sort_pairs_u64_f64(keys.data(), AT.vals, nnz, rb + cb);
-    hipLaunchKernelGGL(k_transpose_out, dim3(grid_for(nnz)), dim3(kBlock), 0, 0, keys.data(), nnz, n, rb, AT.col_idx,
-                       AT.row_ptr);
+    sort_pairs_u32_rowval(keys.data(), rv.data(), nnz, bits_for(n));
+    hipLaunchKernelGGL(k_transpose_out, dim3(grid_for(nnz)), dim3(kBlock), 0, 0, keys.data(),
+                       static_cast<const RowVal*>(rv.data()), nnz, n, AT.col_idx, AT.vals, AT.row_ptr);
     GP_LAUNCH_CHECK("transpose columns");
     return;
   }

@@ -45,4 +45,8 @@ void sort_pairs_u64_f64(uint64_t* keys, double* vals, int64_t n, int end_bit) {
   sort_pairs<uint64_t, double>(keys, vals, n, end_bit);
 }
 
+void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit) {
+  sort_pairs<uint32_t, RowVal>(keys, vals, n, end_bit);
+}
+
 }  // namespace gp
