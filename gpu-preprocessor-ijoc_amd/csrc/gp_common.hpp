@@ -320,6 +320,27 @@ __device__ inline void wave_count(int32_t* counter) {
   if (__lane_id() == static_cast<unsigned>(__ffsll(active) - 1)) atomicAdd(counter, static_cast<int32_t>(__popcll(active)));
 }
 
+// ---- flag words -------------------------------------------------------------------
+// Eight zeroed int32 device words for a rule's flags and counters.  Slots come
+// from two pre-zeroed buffers of kFlagSlots slots used in turn; a buffer is
+// zeroed again (one memset, stream-ordered) when its turn comes back, so a slot
+// stays valid while fewer than kFlagSlots further slots are taken -- a rule
+// takes a handful.  Saves a memset launch per rule call.
+constexpr int kFlagSlots = 4096;
+int32_t* flag_slot();
+struct Flags {
+  struct Slot {
+    int32_t* p;
+    int32_t* data() const { return p; }
+  } d;
+  int32_t h[8] = {};
+  Flags() : d{flag_slot()} {}
+  const int32_t* read() {
+    d2h(h, d.p, 8);
+    return h;
+  }
+};
+
 // ---- primitives (gp_primitives.hip) ---------------------------------------------
 // In-place inclusive / exclusive scans; `total` receives the sum when non-null.
 void inclusive_scan_i32(int32_t* data, int64_t n);

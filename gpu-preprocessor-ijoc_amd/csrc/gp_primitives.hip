@@ -129,6 +129,29 @@ void KernelClock::flush() {
 
 KernelClock::~KernelClock() {}
 
+// ---- flag words ------------------------------------------------------------------------
+namespace {
+struct FlagBuffers {
+  int32_t* buf[2] = {nullptr, nullptr};
+  int cur = 0, next = kFlagSlots;  // next slot of buf[cur]; kFlagSlots = switch buffers
+};
+std::map<int, FlagBuffers> g_flag_buffers;
+}  // namespace
+
+int32_t* flag_slot() {
+  int dev = 0;
+  GP_CHECK(hipGetDevice(&dev));
+  FlagBuffers& f = g_flag_buffers[dev];
+  const size_t bytes = sizeof(int32_t) * 8 * kFlagSlots;
+  if (f.next == kFlagSlots) {
+    f.cur ^= 1;
+    if (!f.buf[f.cur]) GP_CHECK(hipMalloc(&f.buf[f.cur], bytes));
+    GP_CHECK(hipMemsetAsync(f.buf[f.cur], 0, bytes, 0));
+    f.next = 0;
+  }
+  return f.buf[f.cur] + 8 * (f.next++);
+}
+
 // ---- scans -------------------------------------------------------------------------
 namespace {
 constexpr int kScanThreads = 256;

@@ -14,16 +14,6 @@ namespace gp {
 namespace {
 inline bool tape_on(const gp_plan& plan, const gp_params& p) { return plan.tape && p.record_postsolve_tape; }
 
-struct Flags {
-  DVec<int32_t> d;
-  int32_t h[8] = {};
-  explicit Flags(int k = 8) : d(8) { GP_CHECK(hipMemsetAsync(d.data(), 0, sizeof(int32_t) * 8, 0)); }
-  const int32_t* read() {
-    d2h(h, d.data(), 8);
-    return h;
-  }
-};
-
 // ---------------------------------------------------------------- fixed variables
 __global__ void k_fixed_candidates(int32_t n, const uint8_t* keep_col, const double* l, const double* u,
                                    const double* c, double bound_tol, uint8_t* fixed, double* val,

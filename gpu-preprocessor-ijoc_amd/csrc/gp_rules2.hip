@@ -17,16 +17,6 @@ namespace gp {
 namespace {
 inline bool tape_on(const gp_plan& plan, const gp_params& p) { return plan.tape && p.record_postsolve_tape; }
 
-struct Flags {
-  DVec<int32_t> d;
-  int32_t h[8] = {};
-  Flags() : d(8) { GP_CHECK(hipMemsetAsync(d.data(), 0, sizeof(int32_t) * 8, 0)); }
-  const int32_t* read() {
-    d2h(h, d.data(), 8);
-    return h;
-  }
-};
-
 __device__ inline int64_t gtid() { return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
