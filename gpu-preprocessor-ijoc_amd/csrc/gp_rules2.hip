@@ -333,20 +333,45 @@ __global__ void k_sc_touch(gp_csr A, int32_t n, const int32_t* owner, const uint
 }
 
 // _kernel_apply_singleton_objective_updates_deterministic: plain sequential sum
-// in AT order starting from c (untouched columns keep c exactly).
-__global__ void k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col_delete, const uint8_t* touched,
-                          const int32_t* owner, const double* sval, double* c) {
-  const int64_t j = gtid();
-  if (j >= AT.rows || !touched[j] || !keep_col[j] || col_delete[j]) return;
+// in AT order starting from c (untouched columns keep c exactly).  G lanes per
+// column load G consecutive entries and form their terms at once; every lane
+// then adds the valid terms in entry order, so the additions are those of the
+// one-lane loop (invalid entries are skipped, not added as zero).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_sc_cost(gp_csr AT, const uint8_t* keep_col, const uint8_t* col_delete,
+                                                     const uint8_t* touched, const int32_t* owner, const double* sval,
+                                                     double* c) {
+  const int lane = threadIdx.x % G;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
   const int32_t n = AT.rows;
-  double updated = c[j];
-  for (gp_offset q = AT.row_ptr[j]; q < AT.row_ptr[j + 1]; ++q) {
-    const int32_t o = owner[AT.col_idx[q]];
-    if (o < 0 || o >= n || !col_delete[o]) continue;
-    updated += -(c[o] * AT.vals[q] / sval[o]);
+  for (int64_t j = gtid() / G; j < n; j += groups) {
+    if (!touched[j] || !keep_col[j] || col_delete[j]) continue;  // uniform over the group
+    double updated = c[j];
+    const gp_offset p1 = AT.row_ptr[j + 1];
+    for (gp_offset b = AT.row_ptr[j]; b < p1; b += G) {
+      const gp_offset q = b + lane;
+      double t = 0.0;
+      int ok = 0;
+      if (q < p1) {
+        const int32_t o = owner[AT.col_idx[q]];
+        if (o >= 0 && o < n && col_delete[o]) {
+          t = -(c[o] * AT.vals[q] / sval[o]);
+          ok = 1;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const double tk = __shfl(t, k, G);
+        if (__shfl(ok, k, G)) updated += tk;
+      }
+    }
+    if (lane == 0) c[j] = updated;  // reads only deleted owner columns, which never change here
   }
-  c[j] = updated;  // reads only deleted owner columns, which never change here
 }
+template <int G>
+struct ScCostK {
+  static constexpr auto kernel = k_sc_cost<G>;
+};
 
 __global__ void k_sc_sub_counts(gp_csr A, int32_t n, const int32_t* owner, const uint8_t* col_delete,
                                 const uint8_t* keep_col, double zt, int64_t* r, int64_t* ni, int64_t* nv) {
@@ -1765,8 +1790,11 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
     hipLaunchKernelGGL(k_sc_touch, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, n, owner.data(), col_delete.data(),
                        touched.data());
     GP_LAUNCH_CHECK("singleton columns touch");
-    hipLaunchKernelGGL(k_sc_cost, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask, col_delete.data(),
-                       touched.data(), owner.data(), s.singleton_column_val, plan.new_c);
+    launch_grouped<ScCostK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
+                            static_cast<const uint8_t*>(plan.keep_col_mask),
+                            static_cast<const uint8_t*>(col_delete.data()), static_cast<const uint8_t*>(touched.data()),
+                            static_cast<const int32_t*>(owner.data()),
+                            static_cast<const double*>(s.singleton_column_val), plan.new_c);
     GP_LAUNCH_CHECK("singleton columns cost");
   }
   const int32_t mn = m > n ? m : n;
