@@ -1450,6 +1450,10 @@ struct ActivityFold {
   double* max_fin;
   int32_t* min_ninf;
   int32_t* max_pinf;
+  BsRow* rec;  // optional
+  const double* AL;
+  const double* AU;
+  const int32_t* row_nnz;
   struct State {
     RowAcc lo, hi;
     int32_t ni, pi;
@@ -1471,23 +1475,27 @@ struct ActivityFold {
     else if (hi > 0.0) ++s.pi;
   }
   __device__ void finish(int64_t i, const State& s, bool) const {
-    min_fin[i] = s.lo.value();
-    max_fin[i] = s.hi.value();
+    const double lo = s.lo.value(), hi = s.hi.value();
+    min_fin[i] = lo;
+    max_fin[i] = hi;
     min_ninf[i] = s.ni;
     max_pinf[i] = s.pi;
+    if (rec) rec[i] = BsRow{lo, hi, AL[i], AU[i], s.ni, s.pi, row_nnz[i], active(i) ? 1 : 0};
   }
 };
 }  // namespace
 
 void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
-                          int32_t* max_pinf) {
+                          int32_t* max_pinf, BsRow* rec, const double* AL, const double* AU,
+                          const int32_t* row_nnz) {
   if (A.rows <= 0) return;
   const PackedMask cols(keep_col, A.cols);
   // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per row
   const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows;
   TimedScope ts(GP_KT_ACTIVITY, bytes);
-  launch_row_fold(A, lu, cols.view(), ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf});
+  launch_row_fold(A, lu, cols.view(),
+                  ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz});
   GP_LAUNCH_CHECK("activity summary");
 }
 

@@ -724,10 +724,9 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
 // bound matches.  G lanes per column take partial minima (order-free).
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_bs_support(
-    gp_csr AT, Bits keep_row, const int32_t* row_nnz, const double* AL, const double* AU, const double* l,
-    const double* u, const double* min_fin, const double* max_fin, const int32_t* min_ninf, const int32_t* max_pinf,
-    const uint8_t* lch, const uint8_t* uch, const double* fl, const double* fu, double zero_tol, double tol,
-    double support_tol, int32_t* sl_out, int32_t* su_out, int64_t* r, int64_t* ci, int64_t* cv) {
+    gp_csr AT, const BsRow* rows, const double* l, const double* u, const uint8_t* lch, const uint8_t* uch,
+    const double* fl, const double* fu, double zero_tol, double tol, double support_tol, int32_t* sl_out,
+    int32_t* su_out, int64_t* r, int64_t* ci, int64_t* cv) {
   const int lane = threadIdx.x % G;
   const int64_t groups = static_cast<int64_t>(gridDim.x) * (blockDim.x / G);
   for (int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; j < AT.rows; j += groups) {
@@ -738,11 +737,12 @@ __global__ void __launch_bounds__(kBlock) k_bs_support(
       const gp_offset p1 = AT.row_ptr[j + 1];
       for (gp_offset p = AT.row_ptr[j] + lane; p < p1; p += G) {
         const int32_t i = AT.col_idx[p];
-        if (!keep_row[i] || row_nnz[i] <= 1) continue;
         const double a = AT.vals[p];
+        const BsRow R = rows[i];
+        if (!R.live || R.row_nnz <= 1) continue;
         if (fabs(a) <= zero_tol) continue;
         double il, iu;
-        implied_bounds(a, lj, uj, AL[i], AU[i], min_fin[i], min_ninf[i], max_fin[i], max_pinf[i], il, iu);
+        implied_bounds(a, lj, uj, R.al, R.au, R.min_fin, R.min_ninf, R.max_fin, R.max_pinf, il, iu);
         if (need_l && flj > lj + tol && isfinite(il) && fabs(il - flj) <= support_tol) sl = i < sl ? i : sl;
         if (need_u && fuj < uj - tol && isfinite(iu) && fabs(iu - fuj) <= support_tol) su = i < su ? i : su;
       }
@@ -896,8 +896,11 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   d2d(cl.data(), plan.new_l, n);
   d2d(cu.data(), plan.new_u, n);
   const PackedBounds lu(plan.new_l, plan.new_u, n);
+  // the support-row search (tape only) reads each row's summary as one record
+  DVec<BsRow> row_rec(tape_on(plan, p) && s.row_nnz ? static_cast<size_t>(m) : 0);
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
-                       min_ninf.data(), max_pinf.data());
+                       min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
+                       plan.new_AU, s.row_nnz);
   {
     const PackedMask live_cols(keep_col, n);
     // A, keep_row, row_nnz, AL, AU, the row summary (24 B), (l, u) and a mask bit per column,
@@ -912,7 +915,6 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     GP_LAUNCH_CHECK("bound candidates");
   }
   Flags flags;
-  const PackedMask rows_all(plan.keep_row_mask, m);
   {
     TimedScope outer(GP_KT_RULES_OTHER, 0.0);
     const bool masked = s.live_masks != 0;
@@ -981,13 +983,10 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     const double support_tol = cmax(10.0 * p.bound_tol, 1.0e-10);
     DVec<int32_t> sl(n), su(n);
     AppendCounts cnt(n);
+    if (!row_rec.size()) throw Error(GP_ERR_INVALID_ARGUMENT, "bound strengthening tape needs live row counts");
     launch_grouped<BsSupportK>(lanes_for_mean(n ? static_cast<double>(lp.AT.nnz) / n : 0.0), n, lp.AT,
-                               rows_all.view(), static_cast<const int32_t*>(s.row_nnz),
-                               static_cast<const double*>(plan.new_AL), static_cast<const double*>(plan.new_AU),
-                               static_cast<const double*>(plan.new_l), static_cast<const double*>(plan.new_u),
-                               static_cast<const double*>(min_fin.data()), static_cast<const double*>(max_fin.data()),
-                               static_cast<const int32_t*>(min_ninf.data()),
-                               static_cast<const int32_t*>(max_pinf.data()), static_cast<const uint8_t*>(lch.data()),
+                               static_cast<const BsRow*>(row_rec.data()), static_cast<const double*>(plan.new_l),
+                               static_cast<const double*>(plan.new_u), static_cast<const uint8_t*>(lch.data()),
                                static_cast<const uint8_t*>(uch.data()), static_cast<const double*>(fl.data()),
                                static_cast<const double*>(fu.data()), p.zero_tol, p.bound_tol, support_tol, sl.data(),
                                su.data(), cnt.r.data(), cnt.i.data(), cnt.v.data());
