@@ -154,12 +154,24 @@ __global__ void k_sc_dirty_cols(gp_csr AT, const double2* lu, const double2* lu_
   for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) dirty[AT.col_idx[p]] = 1;
 }
 
+// dirty rows appended to the list with one atomic per wavefront (any order:
+// listed rows are refreshed independently)
 __global__ void k_sc_dirty_rows(int32_t m, const uint8_t* keep_row, const uint8_t* keep_row_old, uint8_t* dirty,
                                 int32_t* list, int32_t* count) {
   const int64_t i = gtid();
-  if (i >= m) return;
-  if (keep_row[i] != keep_row_old[i]) dirty[i] = 1;
-  if (dirty[i]) list[atomicAdd(count, 1)] = static_cast<int32_t>(i);
+  bool d = false;
+  if (i < m) {
+    if (keep_row[i] != keep_row_old[i]) dirty[i] = 1;
+    d = dirty[i] != 0;
+  }
+  const unsigned long long b = __ballot(d);
+  if (!b) return;
+  const int lane = static_cast<int>(__lane_id());
+  const int leader = __ffsll(static_cast<long long>(b)) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, static_cast<int32_t>(__popcll(b)));
+  base = __shfl(base, leader, 64);
+  if (d) list[base + __popcll(b & ((1ull << lane) - 1ull))] = static_cast<int32_t>(i);
 }
 
 __global__ void k_sc_refresh_rows(gp_csr A, const int32_t* list, int32_t count, const uint8_t* keep_row,
