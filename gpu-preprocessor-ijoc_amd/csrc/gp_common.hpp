@@ -348,6 +348,8 @@ void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total);
 int64_t exclusive_scan_i32_total(int32_t* data, int64_t n);  // returns total
 // exclusive scan whose total is written to *dev_total (device), no host read
 void exclusive_scan_i64_device_total(int64_t* data, int64_t n, int64_t* dev_total);
+// the same for three arrays of length n at once; totals to dev_totals[0..2]
+void exclusive_scan3_i64_device_total(int64_t* a0, int64_t* a1, int64_t* a2, int64_t n, int64_t* dev_totals);
 // old_to_new[i] = keep[i] ? rank : -1 ; returns number kept.
 int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new);
 // Sum in the fixed pairwise order (chunks of 1024, stride halving, recursive).

@@ -43,9 +43,7 @@ AppendCounts::AppendCounts(int64_t n)
 
 void AppendCounts::scan(int64_t n) {
   DVec<int64_t> totals(3);
-  exclusive_scan_i64_device_total(r.data(), n, totals.data());
-  exclusive_scan_i64_device_total(i.data(), n, totals.data() + 1);
-  exclusive_scan_i64_device_total(v.data(), n, totals.data() + 2);
+  exclusive_scan3_i64_device_total(r.data(), i.data(), v.data(), n, totals.data());
   int64_t h[3];
   d2h(h, totals.data(), 3);  // one round trip for the three totals
   R = h[0];
