@@ -368,6 +368,7 @@ void fill_u8(uint8_t* p, uint8_t v, int64_t n);
 //   void init(State&)
 //   void add(State&, pos_in_row, lo, hi, skipped)   -- called for every stored entry
 //   void finish(row, const State&, active)  -- called for every row of the grid
+//   void gathered(entry, (l, u))            -- the bounds gathered for a used entry
 // Read-once matrix stream: the non-temporal hint asks L2 and the Infinity
 // Cache not to keep these lines, so the streamed CSR does not evict the
 // gathered bound and mask arrays.
@@ -403,6 +404,7 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
         double lo = 0.0, hi = 0.0;
         if (!skip) {
           const double2 b = lu[j];
+          pol.gathered(q, b);
           lo = a >= 0.0 ? a * b.x : a * b.y;
           hi = a >= 0.0 ? a * b.y : a * b.x;
         }
@@ -631,11 +633,14 @@ struct alignas(64) BsRow {
   int32_t min_ninf, max_pinf, row_nnz, live;
 };
 // rec (optional): the summary of each row with its AL, AU, row_nnz and
-// liveness, written by the same sweep.
+// liveness, written by the same sweep.  entry_lu (optional, nnz pairs): the
+// (l, u) pair gathered for each used entry (live column, |a| > zero_tol) of an
+// active row, stored at the entry's position; other entries are left as they are.
 void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf, BsRow* rec = nullptr, const double* AL = nullptr,
-                          const double* AU = nullptr, const int32_t* row_nnz = nullptr);
+                          const double* AU = nullptr, const int32_t* row_nnz = nullptr,
+                          double2* entry_lu = nullptr);
 
 // Kept rows of A with an entry in a column whose col_mark byte is set, appended
 // to list (any order); *count (device, caller-zeroed) receives their number.

@@ -1556,12 +1556,16 @@ struct ActivityFold {
   const double* AL;
   const double* AU;
   const int32_t* row_nnz;
+  double2* entry_lu;  // optional: (l, u) of each used entry, at the entry's position
   struct State {
     RowAcc lo, hi;
     int32_t ni, pi;
   };
   __device__ bool active(int64_t i) const { return !keep_row || keep_row[i]; }
   __device__ bool use(double a) const { return fabs(a) > zero_tol; }
+  __device__ void gathered(int64_t q, double2 b) const {
+    if (entry_lu) entry_lu[q] = b;
+  }
   __device__ void init(State& s) const {
     s.lo = RowAcc{};
     s.hi = RowAcc{};
@@ -1590,14 +1594,15 @@ struct ActivityFold {
 void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf, BsRow* rec, const double* AL, const double* AU,
-                          const int32_t* row_nnz) {
+                          const int32_t* row_nnz, double2* entry_lu) {
   if (A.rows <= 0) return;
   const PackedMask cols(keep_col, A.cols);
   // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per row
   const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows;
   TimedScope ts(GP_KT_ACTIVITY, bytes);
   launch_row_fold(A, lu, cols.view(),
-                  ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz});
+                  ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz,
+                               entry_lu});
   GP_LAUNCH_CHECK("activity summary");
 }
 

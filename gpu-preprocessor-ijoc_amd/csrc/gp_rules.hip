@@ -376,6 +376,7 @@ struct RedundantFold {
     return !(isfinite(al) && isfinite(au) && fabs(au - al) <= tol);
   }
   __device__ bool use(double) const { return true; }
+  __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.lo = Act{};
     s.hi = Act{};
@@ -540,10 +541,13 @@ __device__ inline void implied_bounds(double a, double lj, double uj, double al,
 
 // Candidate bounds: G lanes per row, each entry independent; the per-column
 // result is a total-order max / min, so the lane schedule does not matter.
+// entry_lu holds the (l, u) pair of each used entry at the entry's position,
+// stored by the activity sweep (which uses the same entries), so the bounds
+// are read coalesced instead of gathered per column.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_bs_candidates(gp_csr A, const uint8_t* keep_row, Bits keep_col,
                                                            const int32_t* row_nnz, const double* AL, const double* AU,
-                                                           const double2* lu, const double* min_fin,
+                                                           const double2* entry_lu, const double* min_fin,
                                                            const double* max_fin, const int32_t* min_ninf,
                                                            const int32_t* max_pinf, double zero_tol, double* cl,
                                                            double* cu) {
@@ -559,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_bs_candidates(gp_csr A, const uint8_
       const double a = ld_stream(A.vals + p);
       if (!keep_col[j]) continue;
       if (fabs(a) <= zero_tol) continue;
-      const double2 b = lu[j];
+      const double2 b = entry_lu[p];
       const double lj = b.x, uj = b.y;
       double il, iu;
       implied_bounds(a, lj, uj, al, au, mnf, mni, mxf, mxp, il, iu);
@@ -856,6 +860,7 @@ struct FixedRowsFold {
   };
   __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
   __device__ bool use(double) const { return true; }
+  __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.acc = RowAcc{};
     s.fixed = 0;
@@ -898,17 +903,18 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   const PackedBounds lu(plan.new_l, plan.new_u, n);
   // the support-row search (tape only) reads each row's summary as one record
   DVec<BsRow> row_rec(tape_on(plan, p) && s.row_nnz ? static_cast<size_t>(m) : 0);
+  DVec<double2> entry_lu(static_cast<size_t>(lp.A.nnz > 0 ? lp.A.nnz : 1));
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
-                       plan.new_AU, s.row_nnz);
+                       plan.new_AU, s.row_nnz, entry_lu.data());
   {
     const PackedMask live_cols(keep_col, n);
-    // A, keep_row, row_nnz, AL, AU, the row summary (24 B), (l, u) and a mask bit per column,
-    // candidate bounds cl / cu
-    TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 45.0 * m + 32.0 * n + n / 8.0);
+    // A, the entries' (l, u) pairs, keep_row, row_nnz, AL, AU, the row summary (24 B), a mask bit
+    // per column, candidate bounds cl / cu
+    TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 28.0 * lp.A.nnz + 45.0 * m + 16.0 * n + n / 8.0);
     launch_grouped<BsCandidatesK>(pick_lanes(tuning().lanes_candidates, m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
                                   plan.keep_row_mask, live_cols.view(), s.row_nnz, plan.new_AL, plan.new_AU,
-                                  static_cast<const double2*>(lu.data()), static_cast<const double*>(min_fin.data()),
+                                  static_cast<const double2*>(entry_lu.data()), static_cast<const double*>(min_fin.data()),
                                   static_cast<const double*>(max_fin.data()),
                                   static_cast<const int32_t*>(min_ninf.data()),
                                   static_cast<const int32_t*>(max_pinf.data()), p.zero_tol, cl.data(), cu.data());
