@@ -368,6 +368,7 @@ void fill_u8(uint8_t* p, uint8_t v, int64_t n);
 //   void init(State&)
 //   void add(State&, pos_in_row, lo, hi, skipped)   -- called for every stored entry
 //   void finish(row, const State&, active)  -- called for every row of the grid
+//   double2 bounds(entry, col, a, lu)       -- (l, u) of a used entry (normally lu[col])
 //   void gathered(entry, (l, u))            -- the bounds gathered for a used entry
 // Read-once matrix stream: the non-temporal hint asks L2 and the Infinity
 // Cache not to keep these lines, so the streamed CSR does not evict the
@@ -403,7 +404,7 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
         const bool skip = !keep_col[j] || !pol.use(a);
         double lo = 0.0, hi = 0.0;
         if (!skip) {
-          const double2 b = lu[j];
+          const double2 b = pol.bounds(q, j, a, lu);
           pol.gathered(q, b);
           lo = a >= 0.0 ? a * b.x : a * b.y;
           hi = a >= 0.0 ? a * b.y : a * b.x;

@@ -1563,6 +1563,7 @@ struct ActivityFold {
   };
   __device__ bool active(int64_t i) const { return !keep_row || keep_row[i]; }
   __device__ bool use(double a) const { return fabs(a) > zero_tol; }
+  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
   __device__ void gathered(int64_t q, double2 b) const {
     if (entry_lu) entry_lu[q] = b;
   }

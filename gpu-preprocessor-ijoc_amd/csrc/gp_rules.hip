@@ -357,6 +357,29 @@ struct Act {
 
 // Row fold of the redundancy test: full activity range of each live row
 // with two or more live entries that is not an equality.
+// The (l, u) pairs bound strengthening's activity sweep gathered, stored per
+// used entry, and the columns whose bounds the rule then changed.  Valid for
+// the rule applied right after it on the same plan (redundant rows); any
+// other rule invalidates it.
+struct BsEntryCache {
+  DVec<double2> entry_lu;
+  DVec<uint32_t> changed;  // bit per column: tightened or fixed
+  int64_t nnz = -1;
+  int32_t m = -1, n = -1;
+  double zero_tol = 0.0;
+  bool masked = false, valid = false;
+};
+
+// bit j: column j's bounds changed (tightened or fixed); none when lch is null
+__global__ void k_changed_bits(int32_t n, const uint8_t* lch, const uint8_t* uch, const uint8_t* fixed,
+                               uint32_t* words) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool c = j < n && lch && (lch[j] || uch[j] || fixed[j]);
+  const unsigned long long b = __ballot(c);
+  const int lane = static_cast<int>(__lane_id());
+  if ((lane == 0 || lane == 32) && j < n) words[j >> 5] = static_cast<uint32_t>(lane == 0 ? b : b >> 32);
+}
+
 struct RedundantFold {
   const uint8_t* keep_row;
   const int32_t* row_nnz;
@@ -367,6 +390,11 @@ struct RedundantFold {
   uint8_t* dl;
   uint8_t* du;
   int32_t* flags;
+  // optional: the (l, u) pairs the bound-strengthening sweep stored per used
+  // entry, valid for columns whose bounds it did not change (changed bits)
+  const double2* entry_lu;
+  Bits changed;
+  double entry_zero_tol;
   struct State {
     Act lo, hi;
   };
@@ -377,6 +405,9 @@ struct RedundantFold {
   }
   __device__ bool use(double) const { return true; }
   __device__ void gathered(int64_t, double2) const {}
+  __device__ double2 bounds(int64_t q, int32_t j, double a, const double2* lu) const {
+    return entry_lu && fabs(a) > entry_zero_tol && !changed[j] ? entry_lu[q] : lu[j];
+  }
   __device__ void init(State& s) const {
     s.lo = Act{};
     s.hi = Act{};
@@ -473,9 +504,24 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   {
   // A, (l, u) and a mask bit per column, keep_row, row_nnz, AL, AU, 3 flags out per row
   TimedScope ts(GP_KT_REDUNDANT, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + lp.A.cols / 8.0 + 24.0 * m);
+  // bound strengthening ran right before on this plan and matrix: its stored
+  // per-entry bounds hold for every column it did not change
+  const double2* entry_lu = nullptr;
+  Bits changed{nullptr};
+  if (plan.rule_cache) {
+    if (void* c = static_cast<RuleCaches*>(plan.rule_cache)->bs_entry) {
+      BsEntryCache& ec = *static_cast<BsEntryCache*>(c);
+      if (ec.valid && ec.nnz == lp.A.nnz && ec.m == m && ec.n == lp.A.cols && ec.zero_tol == p.zero_tol &&
+          ec.masked == (s.live_masks != 0)) {
+        entry_lu = ec.entry_lu.data();
+        changed = Bits{ec.changed.data()};
+      }
+      ec.valid = false;
+    }
+  }
   launch_row_fold(lp.A, lu.data(), live_cols.view(),
                   RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
-                                dl.data(), du.data(), flags.d.data()});
+                                dl.data(), du.data(), flags.d.data(), entry_lu, changed, p.zero_tol});
   GP_LAUNCH_CHECK("redundant rows");
   }
   const int32_t* f = flags.read();
@@ -860,6 +906,7 @@ struct FixedRowsFold {
   };
   __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
   __device__ bool use(double) const { return true; }
+  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
   __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.acc = RowAcc{};
@@ -903,10 +950,28 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   const PackedBounds lu(plan.new_l, plan.new_u, n);
   // the support-row search (tape only) reads each row's summary as one record
   DVec<BsRow> row_rec(tape_on(plan, p) && s.row_nnz ? static_cast<size_t>(m) : 0);
-  DVec<double2> entry_lu(static_cast<size_t>(lp.A.nnz > 0 ? lp.A.nnz : 1));
+  RuleCaches& rcache = rule_caches(plan);
+  if (!rcache.bs_entry) rcache.bs_entry = new BsEntryCache();
+  BsEntryCache& ec = *static_cast<BsEntryCache*>(rcache.bs_entry);
+  ec.valid = false;
+  ec.entry_lu.resize(static_cast<size_t>(lp.A.nnz > 0 ? lp.A.nnz : 1));
+  ec.changed.resize(static_cast<size_t>((n + 31) / 32 > 0 ? (n + 31) / 32 : 1));
+  ec.nnz = lp.A.nnz;
+  ec.m = m;
+  ec.n = n;
+  ec.zero_tol = p.zero_tol;
+  ec.masked = keep_col != nullptr;
+  // marks the stored pairs valid for the next rule, with the changed columns
+  auto publish = [&](const uint8_t* lc, const uint8_t* uc, const uint8_t* fx) {
+    hipLaunchKernelGGL(k_changed_bits, dim3(grid_for((static_cast<int64_t>(n) + 31) / 32 * 32)), dim3(kBlock), 0, 0,
+                       n, lc, uc, fx, ec.changed.data());
+    GP_LAUNCH_CHECK("changed bounds bits");
+    ec.valid = true;
+  };
+  const DVec<double2>& entry_lu = ec.entry_lu;
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
-                       plan.new_AU, s.row_nnz, entry_lu.data());
+                       plan.new_AU, s.row_nnz, ec.entry_lu.data());
   {
     const PackedMask live_cols(keep_col, n);
     // A, the entries' (l, u) pairs, keep_row, row_nnz, AL, AU, the row summary (24 B), a mask bit
@@ -982,7 +1047,10 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     plan.has_infeasible = 1;
     return;
   }
-  if (!(f[1] || f[2])) return;
+  if (!(f[1] || f[2])) {
+    publish(nullptr, nullptr, nullptr);  // no bounds changed
+    return;
+  }
   const bool f2 = f[2] != 0;
   TimedScope ts(GP_KT_RULES_OTHER, 0.0);
   if (tape_on(plan, p) && f[1]) {
@@ -1039,6 +1107,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   plan.has_row_action = plan.has_row_action || f2;
   plan.has_col_action = plan.has_col_action || f2;
   plan.has_change = 1;
+  publish(lch.data(), uch.data(), fixed.data());
 }
 
 RuleCaches& rule_caches(gp_plan& plan) {
@@ -1047,16 +1116,22 @@ RuleCaches& rule_caches(gp_plan& plan) {
 }
 
 void col_max_cache_free(void* cache) { delete static_cast<ColMaxCache*>(cache); }
+void bs_entry_cache_free(void* cache) { delete static_cast<BsEntryCache*>(cache); }
 
 void rule_cache_free(void* cache) {
   if (!cache) return;
   RuleCaches* rc = static_cast<RuleCaches*>(cache);
   sc_cache_free(rc->sc);
   col_max_cache_free(rc->col_max);
+  bs_entry_cache_free(rc->bs_entry);
   delete rc;
 }
 
 void apply_rule(int32_t rule, gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
+  // bound strengthening's stored entry bounds serve only the rule right after it
+  if (rule != GP_RULE_REDUNDANT_ROWS && plan.rule_cache) {
+    if (void* c = static_cast<RuleCaches*>(plan.rule_cache)->bs_entry) static_cast<BsEntryCache*>(c)->valid = false;
+  }
   switch (rule) {
     case GP_RULE_FIXED_VARIABLES: rule_fixed_variables(plan, lp, p); break;
     case GP_RULE_EMPTY_COLUMNS: rule_empty_columns(plan, lp, s, p); break;

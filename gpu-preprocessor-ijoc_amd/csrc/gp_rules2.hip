@@ -60,6 +60,7 @@ struct ScActivityFold {
   };
   __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
   __device__ bool use(double) const { return true; }
+  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
   __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.mn = RowAcc{};

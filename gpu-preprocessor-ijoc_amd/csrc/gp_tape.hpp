@@ -56,8 +56,9 @@ inline DeviceTape* plan_tape(const gp_plan& plan) { return static_cast<DeviceTap
 // Per-plan caches of the rules, hung off gp_plan::rule_cache.  Each rule
 // source file owns the type of its slot.
 struct RuleCaches {
-  void* sc = nullptr;       // singleton-column row activities (gp_rules2.hip)
-  void* col_max = nullptr;  // column max |a| (gp_rules.hip)
+  void* sc = nullptr;        // singleton-column row activities (gp_rules2.hip)
+  void* col_max = nullptr;   // column max |a| (gp_rules.hip)
+  void* bs_entry = nullptr;  // per-entry bounds of the last bound strengthening (gp_rules.hip)
 };
 RuleCaches& rule_caches(gp_plan& plan);  // created on first use
 void rule_cache_free(void* cache);
