@@ -1598,8 +1598,10 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const int32_t* row_nnz, double2* entry_lu) {
   if (A.rows <= 0) return;
   const PackedMask cols(keep_col, A.cols);
-  // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per row
-  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows;
+  // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per
+  // row; with the optional outputs, 16 bytes per entry (its (l, u) pair) and a 64-byte record per row
+  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows +
+                       (entry_lu ? 16.0 * A.nnz : 0.0) + (rec ? 64.0 * A.rows : 0.0);
   TimedScope ts(GP_KT_ACTIVITY, bytes);
   launch_row_fold(A, lu, cols.view(),
                   ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz,
