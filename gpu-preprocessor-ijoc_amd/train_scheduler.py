@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--init-seed", type=int, default=0)
     ap.add_argument("--data", default=None, help="reuse a samples .npz written by an earlier run")
-    ap.add_argument("--out", default="gpurun_out/learned")
+    ap.add_argument("--out", default="scheduler_training")
     ap.add_argument("--install", action="store_true")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)

@@ -1,4 +1,4 @@
-"""CPU BASELINE ONLY: ctypes front of _ref/libpslp.so, the reference's bundled
+"""CPU baseline only: ctypes front of _ref/libpslp.so, the reference's bundled
 CPU presolver PSLP 0.0.11 built from its sources by ref.mk."""
 from __future__ import annotations
 

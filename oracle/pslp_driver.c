@@ -1,5 +1,5 @@
 /*
- * pslp_driver.c -- CPU BASELINE ONLY.  Runs the reference's bundled CPU
+ * pslp_driver.c -- CPU baseline only.  Runs the reference's bundled CPU
  * presolver (PSLP 0.0.11, third_party/pslp) on one CSR LP with its default
  * settings and reports the time the reference reports
  * (solver_interface/adapters/pslp.cpp:232-275: wall time of new_presolver +

@@ -1,7 +1,7 @@
 # Builds the reference's bundled CPU presolver, PSLP 0.0.11
 # (third_party/pslp, Apache-2.0), from its sources where they lie, with the
 # reference's flags (solver_interface/CMakeLists.txt:22-36), together with
-# pslp_driver.c.  CPU BASELINE ONLY: output goes to _ref/ (not committed).
+# pslp_driver.c.  CPU baseline only: output goes to _ref/ (not committed).
 REF ?= /root/reference
 PSLP := $(REF)/third_party/pslp
 PSLP_SRC := $(filter-out $(PSLP)/src/core/Debugger.c,$(wildcard $(PSLP)/src/core/*.c $(PSLP)/src/explorers/*.c))
