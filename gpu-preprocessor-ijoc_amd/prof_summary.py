@@ -24,7 +24,7 @@ def short_name(name: str) -> str:
         step = re.search(r"detail::(radix_sort_onesweep_iteration|radix_sort_onesweep_global_offsets"
                          r"|radix_sort_block_sort|merge_sort_block_merge_impl|merge_\w+?_config_selector"
                          r"|transform_impl)", n)
-        types = re.search(r"_config<[^,<>]*default_config, ([\w ]+), ([\w ]+)>", n)
+        types = re.search(r"_config<[^,<>]*default_config, ([\w: ]+), ([\w: ]+)>", n)
         out = "rocprim " + (step.group(1) if step else "kernel")
         return out + (f"<{types.group(1)}, {types.group(2)}>" if types else "")
     if n.startswith("void "):
