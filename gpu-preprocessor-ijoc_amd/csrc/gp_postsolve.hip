@@ -367,6 +367,41 @@ __device__ void replay_row_implied(PsState& s, const int32_t* ix, int64_t i0, co
   s.retrieved[col] = 1;
 }
 
+// Projected column (FmeCol; postsolve_projected_variables.cuh:14-68).
+// indices: K, eliminated column, K deleted rows, N, L, (orient, len) x L,
+// N support columns; values: (constant, len coefficients) x L.  The column
+// takes the largest lower side, else the smallest upper side, else 0.
+__device__ void replay_projected(PsState& s, const int32_t* ix, int64_t i0, const double* v, int64_t v0, bool primal,
+                                 bool dual) {
+  const int32_t K = ix[i0], col = ix[i0 + 1];
+  const int32_t L = ix[i0 + 3 + K];
+  if (primal) {
+    int64_t meta = i0 + 4 + K, cp = meta + 2 * static_cast<int64_t>(L), vp = v0;
+    double lo = -INFINITY, hi = INFINITY;
+    bool has_lo = false, has_hi = false;
+    for (int32_t side = 0; side < L; ++side) {
+      const int32_t orient = ix[meta++];
+      const int32_t len = ix[meta++];
+      double cand = v[vp];
+      for (int32_t t = 0; t < len; ++t) cand += v[vp + 1 + t] * s.x[ix[cp++]];
+      if (orient > 0) {
+        lo = cmax(lo, cand);
+        has_lo = true;
+      } else {
+        hi = cmin(hi, cand);
+        has_hi = true;
+      }
+      vp += 1 + len;
+    }
+    s.x[col] = has_lo ? lo : (has_hi ? hi : 0.0);
+  }
+  if (dual) {
+    s.z[col] = 0.0;
+    s.retrieved[col] = 1;
+    for (int32_t t = 0; t < K; ++t) s.y[ix[i0 + 2 + t]] = 0.0;
+  }
+}
+
 __device__ void replay_record(PsState& s, const PsTape& t, int64_t k, bool primal, bool dual) {
   const int32_t type = t.types[k];
   const int64_t i0 = t.is[k], i1 = t.is[k + 1] - 1;
@@ -377,6 +412,7 @@ __device__ void replay_record(PsState& s, const PsTape& t, int64_t k, bool prima
     case GP_RED_SINGLETON_COLUMN_SUBSTITUTION: replay_singleton_column(s, t.idx, i0, t.vals, v0, primal, dual); break;
     case GP_RED_FIXED_VARIABLE: replay_fixed(s, t.idx, i0, i1, t.vals, v0, primal, dual); break;
     case GP_RED_INFINITE_BOUND_DUAL_FIX: replay_infinite(s, t.idx, i0, t.vals, v0, primal, dual); break;
+    case GP_RED_FME_COL: replay_projected(s, t.idx, i0, t.vals, v0, primal, dual); break;
     case GP_RED_ROW_DUAL_SHIFT:
       if (dual) s.y[t.idx[i0]] += t.vals[v0];
       break;
@@ -456,6 +492,18 @@ __device__ void record_accesses(const PsState& s, const PsTape& t, int64_t k, bo
       f(ix[i0], true);
       f(ix[i0 + 1], primal);
       break;
+    case GP_RED_FME_COL: {
+      const int32_t K = ix[i0];
+      f(ix[i0 + 1], true);
+      if (primal) {
+        const int32_t N = ix[i0 + 2 + K], L = ix[i0 + 3 + K];
+        const int64_t cp = i0 + 4 + K + 2 * static_cast<int64_t>(L);
+        for (int32_t q = 0; q < N; ++q) f(ix[cp + q], false);
+      } else {
+        for (int32_t q = 0; q < K; ++q) f(n0 + ix[i0 + 2 + q], true);
+      }
+      break;
+    }
     case GP_RED_DOUBLETON_EQUATION: {
       const int32_t ec = ix[i0], kc = ix[i0 + 1], row = ix[i0 + 2];
       f(ec, true);
@@ -802,7 +850,419 @@ __global__ void k_validate(const int32_t* types, const gp_offset* is, const gp_o
 
 }  // namespace
 
+namespace {
 
+// ---- structural primal recoveries (Startup rules' payloads) -----------------------
+// postsolve_piecewise_linear_reformulation_absolute_value.cu:9-103 and
+// postsolve_piecewise_linear_reformulation_variable_pair.cu:9-41.
+__global__ void k_restore_splits(double* x, const int32_t* t_col, const int32_t* e_col, const double* rho,
+                                 int32_t k) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const double t = x[t_col[i]], e = x[e_col[i]];
+  x[t_col[i]] = t + e;
+  x[e_col[i]] = (t - e) / rho[i];
+}
+__global__ void k_restore_pairs(double* x, const int32_t* bound_col, const int32_t* free_col, int32_t k) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) x[free_col[i]] = x[bound_col[i]];
+}
+__global__ void k_restore_linked(double* x, const int32_t* slack_col, const int32_t* t_col, const double* factor,
+                                 int32_t k) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) x[slack_col[i]] = factor[i] * x[t_col[i]];
+}
+// one slack per lane: max of its lower bound (0 when absent or infinite) and
+// factor * x over its columns
+__global__ void k_restore_max(double* x, const int32_t* slack_col, const gp_offset* starts, const int32_t* t_cols,
+                              const double* factors, int32_t k, const double* lower) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= k || starts[i + 1] == starts[i]) return;
+  const int32_t sc = slack_col[i];
+  double v = lower ? lower[sc] : 0.0;
+  if (!isfinite(v)) v = 0.0;
+  for (gp_offset q = starts[i]; q < starts[i + 1]; ++q) v = cmax(v, factors[q] * x[t_cols[q]]);
+  x[sc] = v;
+}
+__global__ void k_cache_roots(const double* x, const int32_t* rp, const int32_t* rm, double* cp, double* cm,
+                              int32_t k) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) {
+    cp[i] = x[rp[i]];
+    cm[i] = x[rm[i]];
+  }
+}
+__global__ void k_restore_members(double* x, const int32_t* pc, const int32_t* mc, const double* ps, const double* ms,
+                                  const double* cp, const double* cm, int32_t k) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) {
+    x[pc[i]] = ps[i] + cp[i];
+    x[mc[i]] = ms[i] + cm[i];
+  }
+}
+
+template <class T>
+DVec<T> upload(const T* src, int64_t count) {
+  DVec<T> d(static_cast<size_t>(count > 0 ? count : 1));
+  if (count > 0) h2d(d.data(), src, static_cast<size_t>(count));
+  return d;
+}
+
+void restore_abs_value(double* x, const gp_abs_value_step& st, const double* lower) {
+  if (st.split_count > 0) {
+    auto t = upload(st.split_t_col, st.split_count), e = upload(st.split_e_col, st.split_count);
+    auto r = upload(st.split_rho, st.split_count);
+    hipLaunchKernelGGL(k_restore_splits, dim3(grid_for(st.split_count)), dim3(kBlock), 0, 0, x, t.data(), e.data(),
+                       r.data(), st.split_count);
+  }
+  if (st.pair_count > 0) {
+    auto b = upload(st.pair_bound_col, st.pair_count), f = upload(st.pair_free_col, st.pair_count);
+    hipLaunchKernelGGL(k_restore_pairs, dim3(grid_for(st.pair_count)), dim3(kBlock), 0, 0, x, b.data(), f.data(),
+                       st.pair_count);
+  }
+  if (st.linked_count > 0) {
+    auto sc = upload(st.linked_slack_col, st.linked_count), t = upload(st.linked_t_col, st.linked_count);
+    auto fa = upload(st.linked_factor, st.linked_count);
+    hipLaunchKernelGGL(k_restore_linked, dim3(grid_for(st.linked_count)), dim3(kBlock), 0, 0, x, sc.data(), t.data(),
+                       fa.data(), st.linked_count);
+  }
+  if (st.max_count > 0) {
+    const int64_t total = st.max_starts[st.max_count];
+    auto sc = upload(st.max_slack_col, st.max_count);
+    auto starts = upload(st.max_starts, static_cast<int64_t>(st.max_count) + 1);
+    auto t = upload(st.max_t_cols, total);
+    auto fa = upload(st.max_factors, total);
+    hipLaunchKernelGGL(k_restore_max, dim3(grid_for(st.max_count)), dim3(kBlock), 0, 0, x, sc.data(), starts.data(),
+                       t.data(), fa.data(), st.max_count, lower);
+  }
+  GP_LAUNCH_CHECK("structural recovery (absolute value)");
+}
+
+void restore_var_pair(double* x, const gp_var_pair_step& st) {
+  if (st.count <= 0) return;
+  const int32_t k = st.count;
+  auto pc = upload(st.plus_cols, k), mc = upload(st.minus_cols, k);
+  auto rp = upload(st.root_plus_cols, k), rm = upload(st.root_minus_cols, k);
+  auto ps = upload(st.plus_shift, k), ms = upload(st.minus_shift, k);
+  DVec<double> cp(k), cm(k);
+  hipLaunchKernelGGL(k_cache_roots, dim3(grid_for(k)), dim3(kBlock), 0, 0, x, rp.data(), rm.data(), cp.data(),
+                     cm.data(), k);
+  hipLaunchKernelGGL(k_restore_members, dim3(grid_for(k)), dim3(kBlock), 0, 0, x, pc.data(), mc.data(), ps.data(),
+                     ms.data(), cp.data(), cm.data(), k);
+  GP_LAUNCH_CHECK("structural recovery (variable pair)");
+}
+
+// Reverse replay of tape records [begin, end) in one pass.
+void replay_range(const PsState& s, const PsTape& t, int64_t begin, int64_t end, int32_t m0, int32_t n0,
+                  bool primal) {
+  if (end <= begin) return;
+  const PsTape v{t.types + begin, t.is + begin, t.vs + begin, t.idx, t.vals};
+  const int64_t count = end - begin;
+  const bool rounds = count >= parallel_min_records() && count < INT32_MAX;
+  if (!(rounds && replay_rounds(s, v, static_cast<int32_t>(count), m0, n0, primal)))
+    hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, v, count, primal ? 1 : 0, primal ? 0 : 1);
+  GP_LAUNCH_CHECK("postsolve replay");
+}
+
+// Device-side inputs of one postsolve (maps, fixed columns, tape) plus the
+// host-side structural timeline (may be empty).
+struct PostsolveInputs {
+  int32_t m0, n0, m1, n1;
+  const int32_t* rows;
+  const int32_t* cols;
+  const int32_t* fixed_idx;
+  const double* fixed_val;
+  int64_t fixed_count;
+  PsTape tape;
+  int64_t count;
+  const gp_host_record* timeline;  // checkpoints and payloads, or null
+};
+
+// postsolve_gpu (gpu_postsolve.cu:535-670): scatter, fixed restore, primal
+// replay between structural checkpoints in reverse, dual replay, projection.
+void postsolve_run(const PostsolveInputs& in, const double* x, const double* y, const double* z,
+                   const gp_device_lp* original, const double* original_l, double* x_out, double* y_out,
+                   double* z_out) {
+  const int32_t m0 = in.m0, n0 = in.n0;
+  if (!original_l && original) original_l = original->l;
+  DVec<double> xo(n0 ? n0 : 1), yo(m0 ? m0 : 1), zo(n0 ? n0 : 1);
+  DVec<uint8_t> ret(n0 ? n0 : 1);
+  GP_CHECK(hipMemsetAsync(xo.data(), 0, sizeof(double) * n0, 0));
+  GP_CHECK(hipMemsetAsync(zo.data(), 0, sizeof(double) * n0, 0));
+  GP_CHECK(hipMemsetAsync(yo.data(), 0, sizeof(double) * m0, 0));
+  GP_CHECK(hipMemsetAsync(ret.data(), 0, n0, 0));
+  if (in.n1 > 0) {
+    if (x) hipLaunchKernelGGL(k_scatter, dim3(grid_for(in.n1)), dim3(kBlock), 0, 0, xo.data(), x, in.cols, in.n1);
+    if (z) hipLaunchKernelGGL(k_scatter, dim3(grid_for(in.n1)), dim3(kBlock), 0, 0, zo.data(), z, in.cols, in.n1);
+    hipLaunchKernelGGL(k_mark, dim3(grid_for(in.n1)), dim3(kBlock), 0, 0, ret.data(), in.cols, in.n1);
+  }
+  if (in.m1 > 0 && y)
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(in.m1)), dim3(kBlock), 0, 0, yo.data(), y, in.rows, in.m1);
+  if (in.fixed_count > 0)
+    hipLaunchKernelGGL(k_restore_fixed, dim3(grid_for(in.fixed_count)), dim3(kBlock), 0, 0, xo.data(), in.fixed_idx,
+                       in.fixed_val, in.fixed_count);
+  GP_LAUNCH_CHECK("postsolve scatter");
+  PsState s{};
+  s.x = xo.data();
+  s.y = yo.data();
+  s.z = zo.data();
+  s.retrieved = ret.data();
+  s.has_model = original != nullptr;
+  if (original) s.model = *original;
+  s.tol = 1e-7;
+  // primal: the tape ranges between checkpoints, newest first, each followed
+  // by its structural payload; the dual pass is analysed after the primal
+  // replay (its pruning reads the final x)
+  int64_t range_end = in.count;
+  const gp_host_record* tl = in.timeline;
+  for (int32_t c = tl ? tl->checkpoint_count - 1 : -1; c >= 0; --c) {
+    const gp_checkpoint& cp = tl->checkpoints[c];
+    replay_range(s, in.tape, cp.tape_position, range_end, m0, n0, true);
+    if (cp.kind == GP_RECOVERY_ABS_VALUE) restore_abs_value(s.x, tl->abs_value[cp.payload_index], original_l);
+    else restore_var_pair(s.x, tl->var_pair[cp.payload_index]);
+    range_end = cp.tape_position;
+  }
+  replay_range(s, in.tape, 0, range_end, m0, n0, true);
+  replay_range(s, in.tape, 0, in.count, m0, n0, false);
+  // projection needs x, y and z (the reference has no y without rows) and the
+  // model's AT, c, l and u
+  if (original && n0 > 0 && m0 > 0 && original->AT.rows > 0 && original->AT.row_ptr && original->c && original->l &&
+      original->u) {
+    DVec<uint8_t> protect(n0);
+    GP_CHECK(hipMemsetAsync(protect.data(), 0, n0, 0));
+    if (in.count > 0)
+      hipLaunchKernelGGL(k_protect, dim3(grid_for(in.count)), dim3(kBlock), 0, 0, in.tape.types, in.tape.is,
+                         in.tape.idx, in.count, n0, protect.data());
+    hipLaunchKernelGGL(k_project, dim3(grid_for(n0)), dim3(kBlock), 0, 0, s, protect.data(), n0);
+    GP_LAUNCH_CHECK("postsolve projection");
+  }
+  if (x_out) d2h(x_out, xo.data(), static_cast<size_t>(n0));
+  if (y_out) d2h(y_out, yo.data(), static_cast<size_t>(m0));
+  if (z_out) d2h(z_out, zo.data(), static_cast<size_t>(n0));
+}
+
+// ---- host-side validation of a caller-built record ---------------------------------
+[[noreturn]] void bad_record(const char* what) { throw Error(GP_ERR_RUNTIME, what); }
+
+// validate_postsolve_tape (gpu_postsolve.cu:400-437), plus the index ranges
+// each record's replay reads, so a malformed record is an error, not a fault.
+void validate_host_tape(const gp_host_record& r) {
+  const int64_t count = r.record_count;
+  if (count < 0 || r.index_count < 0 || r.value_count < 0) bad_record("malformed postsolve tape offsets");
+  if (count == 0) {
+    if ((r.index_starts && r.index_starts[0] != 0) || (r.value_starts && r.value_starts[0] != 0))
+      bad_record("postsolve tape offsets must start at zero");
+    if (r.index_count != 0 || r.value_count != 0) bad_record("postsolve tape terminal offsets do not match data");
+    return;
+  }
+  if (!r.types || !r.index_starts || !r.value_starts || !r.dual_modes) bad_record("malformed postsolve tape offsets");
+  if (r.index_starts[0] != 0 || r.value_starts[0] != 0) bad_record("postsolve tape offsets must start at zero");
+  for (int64_t k = 0; k < count; ++k) {
+    if (r.index_starts[k] < 0 || r.index_starts[k] > r.index_starts[k + 1] || r.value_starts[k] < 0 ||
+        r.value_starts[k] > r.value_starts[k + 1])
+      bad_record("postsolve tape offsets are not monotone");
+    if (r.types[k] < GP_RED_FIXED_VARIABLE || r.types[k] > GP_RED_FME_COL)
+      bad_record("postsolve tape contains an unknown reduction");
+    if (r.dual_modes[k] > GP_DUAL_MINIMAL) bad_record("postsolve tape contains an unknown dual mode");
+  }
+  if (r.index_starts[count] != r.index_count || r.value_starts[count] != r.value_count)
+    bad_record("postsolve tape terminal offsets do not match data");
+  if ((r.index_count && !r.indices) || (r.value_count && !r.vals)) bad_record("postsolve tape data missing");
+  const int32_t* ix = r.indices;
+  auto col = [&](int64_t p) {
+    if (ix[p] < 0 || ix[p] >= r.n0) bad_record("postsolve record names an invalid original column");
+  };
+  auto row = [&](int64_t p, bool allow_none) {
+    if (allow_none && ix[p] < 0) return;
+    if (ix[p] < 0 || ix[p] >= r.m0) bad_record("postsolve record names an invalid original row");
+  };
+  for (int64_t k = 0; k < count; ++k) {
+    const int64_t i0 = r.index_starts[k], ni = r.index_starts[k + 1] - i0;
+    const int64_t nv = r.value_starts[k + 1] - r.value_starts[k];
+    auto need = [&](int64_t i, int64_t v) {
+      if (ni < i || nv < v) bad_record("postsolve record payload too short");
+    };
+    switch (r.types[k]) {
+      case GP_RED_FIXED_VARIABLE:
+        need(1, 1 + ni);
+        col(i0);
+        for (int64_t p = 1; p < ni; ++p) row(i0 + p, false);
+        break;
+      case GP_RED_INFINITE_BOUND_DUAL_FIX: {
+        need(3, 1);
+        col(i0 + 1);
+        int64_t ip = 3, vp = 1;
+        for (int32_t q = 0; q < ix[i0 + 2]; ++q) {
+          need(ip + 1, vp);
+          const int64_t len = ix[i0 + ip];
+          if (len < 0) bad_record("postsolve record payload too short");
+          need(ip + 1 + len, vp + 1 + len);
+          for (int64_t t = 0; t < len; ++t) col(i0 + ip + 1 + t);
+          ip += len + 1;
+          vp += len + 1;
+        }
+        break;
+      }
+      case GP_RED_SINGLETON_COLUMN_SUBSTITUTION: {
+        need(3, 6);
+        col(i0);
+        row(i0 + 1, true);
+        const int64_t sup = ix[i0 + 2];
+        if (sup < 0) bad_record("postsolve record payload too short");
+        need(3 + sup, 6 + sup);
+        for (int64_t t = 0; t < sup; ++t) col(i0 + 3 + t);
+        break;
+      }
+      case GP_RED_PARALLEL_COLUMN:
+        need(2, 5);
+        col(i0);
+        col(i0 + 1);
+        break;
+      case GP_RED_PARALLEL_ROW:
+        if (ni >= 2 && nv >= 5) {
+          row(i0, false);
+          row(i0 + 1, false);
+        }
+        break;
+      case GP_RED_DELETED_ROW:
+        if (ni >= 2 && nv >= 3) {
+          row(i0, false);
+          col(i0 + 1);
+        }
+        break;
+      case GP_RED_ROW_DUAL_SHIFT:
+        need(1, 1);
+        row(i0, false);
+        break;
+      case GP_RED_ROW_IMPLIED_BOUND_CHANGE:
+        if (ni >= 2) {
+          need(2, 4);
+          col(i0);
+          row(i0 + 1, false);
+        }
+        break;
+      case GP_RED_DOUBLETON_EQUATION: {
+        need(3, 10);
+        col(i0);
+        col(i0 + 1);
+        row(i0 + 2, true);
+        if (ni >= 4 && nv >= 12) {
+          const int64_t sc = ix[i0 + 3];
+          if (sc < 0) bad_record("postsolve record payload too short");
+          need(4 + sc, 12 + sc);
+          for (int64_t t = 0; t < sc; ++t) row(i0 + 4 + t, false);
+        }
+        break;
+      }
+      case GP_RED_FME_COL: {
+        need(2, 0);
+        const int64_t K = ix[i0];
+        if (K < 0) bad_record("postsolve record payload too short");
+        need(4 + K, 0);
+        col(i0 + 1);
+        for (int64_t t = 0; t < K; ++t) row(i0 + 2 + t, false);
+        const int64_t N = ix[i0 + 2 + K], L = ix[i0 + 3 + K];
+        if (N < 0 || L < 0) bad_record("postsolve record payload too short");
+        need(4 + K + 2 * L + N, 0);
+        int64_t lens = 0;
+        for (int64_t q = 0; q < L; ++q) {
+          const int64_t len = ix[i0 + 4 + K + 2 * q + 1];
+          if (len < 0) bad_record("postsolve record payload too short");
+          lens += len;
+        }
+        if (lens > N) bad_record("postsolve record payload too short");
+        need(4 + K + 2 * L + N, L + lens);
+        for (int64_t t = 0; t < N; ++t) col(i0 + 4 + K + 2 * L + t);
+        break;
+      }
+      default: break;  // row / variable bound changes replay nothing
+    }
+  }
+}
+
+void recovery_col(int32_t c, int32_t n0, const char* what) {
+  if (c < 0 || c >= n0) throw Error(GP_ERR_RUNTIME, std::string(what) + " contains an invalid original column");
+}
+
+// validate_primal_recovery_timeline (gpu_postsolve.cu:439-503)
+void validate_timeline(const gp_host_record& r) {
+  const int32_t na = r.abs_value_count > 0 ? r.abs_value_count : 0;
+  const int32_t nvp = r.var_pair_count > 0 ? r.var_pair_count : 0;
+  if ((na && !r.abs_value) || (nvp && !r.var_pair) || (r.checkpoint_count > 0 && !r.checkpoints))
+    bad_record("primal recovery payload missing");
+  std::vector<uint8_t> seen_a(na, 0), seen_v(nvp, 0);
+  int64_t prev = 0;
+  for (int32_t c = 0; c < r.checkpoint_count; ++c) {
+    const gp_checkpoint& cp = r.checkpoints[c];
+    if (cp.tape_position < prev || cp.tape_position > r.record_count)
+      bad_record("primal recovery checkpoints are not chronological");
+    prev = cp.tape_position;
+    if (cp.kind == GP_RECOVERY_ABS_VALUE) {
+      if (cp.payload_index < 0 || cp.payload_index >= na || seen_a[cp.payload_index])
+        bad_record("invalid or duplicate absolute-value recovery checkpoint");
+      seen_a[cp.payload_index] = 1;
+      const gp_abs_value_step& st = r.abs_value[cp.payload_index];
+      for (int32_t i = 0; i < st.split_count; ++i) {
+        recovery_col(st.split_t_col[i], r.n0, "absolute-value split recovery");
+        recovery_col(st.split_e_col[i], r.n0, "absolute-value split recovery");
+      }
+      for (int32_t i = 0; i < st.pair_count; ++i) {
+        recovery_col(st.pair_bound_col[i], r.n0, "absolute-value outer-pair recovery");
+        recovery_col(st.pair_free_col[i], r.n0, "absolute-value outer-pair recovery");
+      }
+      for (int32_t i = 0; i < st.linked_count; ++i) {
+        recovery_col(st.linked_slack_col[i], r.n0, "absolute-value linked-slack recovery");
+        recovery_col(st.linked_t_col[i], r.n0, "absolute-value linked-slack recovery");
+      }
+      if (st.max_count > 0) {
+        if (!st.max_starts || st.max_starts[0] != 0)
+          bad_record("absolute-value max-slack recovery has mismatched payloads");
+        for (int32_t i = 0; i < st.max_count; ++i) {
+          if (st.max_starts[i + 1] < st.max_starts[i])
+            bad_record("absolute-value max-slack recovery has mismatched payloads");
+          recovery_col(st.max_slack_col[i], r.n0, "absolute-value max-slack recovery");
+          for (gp_offset q = st.max_starts[i]; q < st.max_starts[i + 1]; ++q)
+            recovery_col(st.max_t_cols[q], r.n0, "absolute-value max-slack recovery");
+        }
+      }
+    } else if (cp.kind == GP_RECOVERY_VAR_PAIR) {
+      if (cp.payload_index < 0 || cp.payload_index >= nvp || seen_v[cp.payload_index])
+        bad_record("invalid or duplicate variable-pair recovery checkpoint");
+      seen_v[cp.payload_index] = 1;
+      const gp_var_pair_step& st = r.var_pair[cp.payload_index];
+      if (st.count < 0) bad_record("variable-pair recovery has mismatched payloads");
+      for (int32_t i = 0; i < st.count; ++i) {
+        recovery_col(st.plus_cols[i], r.n0, "variable-pair recovery");
+        recovery_col(st.minus_cols[i], r.n0, "variable-pair recovery");
+        recovery_col(st.root_plus_cols[i], r.n0, "variable-pair recovery");
+        recovery_col(st.root_minus_cols[i], r.n0, "variable-pair recovery");
+      }
+    } else {
+      bad_record("unknown primal recovery checkpoint kind");
+    }
+  }
+  for (uint8_t v : seen_a)
+    if (!v) bad_record("absolute-value recovery payload is missing a checkpoint");
+  for (uint8_t v : seen_v)
+    if (!v) bad_record("variable-pair recovery payload is missing a checkpoint");
+}
+
+void validate_host_record(const gp_host_record& r) {
+  if (r.m0 < 0 || r.n0 < 0 || r.m1 < 0 || r.n1 < 0 || r.m1 > r.m0 || r.n1 > r.n0 || r.fixed_count < 0)
+    throw std::invalid_argument("invalid record dimensions");
+  if ((r.m1 && !r.row_red2org) || (r.n1 && !r.col_red2org) || (r.fixed_count && (!r.fixed_idx || !r.fixed_val)))
+    throw std::invalid_argument("record maps missing");
+  for (int32_t i = 0; i < r.m1; ++i)
+    if (r.row_red2org[i] < 0 || r.row_red2org[i] >= r.m0) bad_record("record row map out of range");
+  for (int32_t j = 0; j < r.n1; ++j)
+    if (r.col_red2org[j] < 0 || r.col_red2org[j] >= r.n0) bad_record("record column map out of range");
+  for (int32_t k = 0; k < r.fixed_count; ++k)
+    if (r.fixed_idx[k] < 0 || r.fixed_idx[k] >= r.n0) bad_record("record fixed column out of range");
+  validate_host_tape(r);
+  validate_timeline(r);
+}
+
+}  // namespace
 
 }  // namespace gp
 
@@ -828,50 +1288,57 @@ extern "C" int gp_postprocess(const double* x, const double* y, const double* z,
       GP_LAUNCH_CHECK("validate tape");
       if (read1(bad.data())) throw Error(GP_ERR_RUNTIME, "malformed postsolve tape");
     }
-    const int32_t m0 = r.m0, n0 = r.n0;
-    DVec<double> xo(n0 ? n0 : 1), yo(m0 ? m0 : 1), zo(n0 ? n0 : 1);
-    DVec<uint8_t> ret(n0 ? n0 : 1);
-    GP_CHECK(hipMemsetAsync(xo.data(), 0, sizeof(double) * n0, 0));
-    GP_CHECK(hipMemsetAsync(zo.data(), 0, sizeof(double) * n0, 0));
-    GP_CHECK(hipMemsetAsync(yo.data(), 0, sizeof(double) * m0, 0));
-    GP_CHECK(hipMemsetAsync(ret.data(), 0, n0, 0));
-    if (r.n1 > 0) {
-      if (x) hipLaunchKernelGGL(k_scatter, dim3(grid_for(r.n1)), dim3(kBlock), 0, 0, xo.data(), x, r.cols, r.n1);
-      if (z) hipLaunchKernelGGL(k_scatter, dim3(grid_for(r.n1)), dim3(kBlock), 0, 0, zo.data(), z, r.cols, r.n1);
-      hipLaunchKernelGGL(k_mark, dim3(grid_for(r.n1)), dim3(kBlock), 0, 0, ret.data(), r.cols, r.n1);
+    PostsolveInputs in{};
+    in.m0 = r.m0;
+    in.n0 = r.n0;
+    in.m1 = r.m1;
+    in.n1 = r.n1;
+    in.rows = r.rows;
+    in.cols = r.cols;
+    in.fixed_idx = r.fixed_idx;
+    in.fixed_val = r.fixed_val;
+    in.fixed_count = r.fixed_count;
+    in.tape = PsTape{t.types.data(), t.istarts.data(), t.vstarts.data(), t.idx.data(), t.vals.data()};
+    in.count = t.count;
+    in.timeline = nullptr;
+    postsolve_run(in, x, y, z, original, nullptr, x_out, y_out, z_out);
+  });
+}
+
+extern "C" int gp_postsolve_record(const double* x_red, const double* y_red, const double* z_red,
+                                   const gp_host_record* record, const gp_device_lp* original,
+                                   const double* original_l, double* x_out, double* y_out, double* z_out) {
+  return guarded([&] {
+    if (!record) throw std::invalid_argument("null record");
+    const gp_host_record& r = *record;
+    validate_host_record(r);
+    if (original) {
+      if (original->AT.row_ptr && original->AT.rows != r.n0) throw std::invalid_argument("original AT shape");
+      if (original->A.row_ptr && original->A.rows != r.m0) throw std::invalid_argument("original A shape");
     }
-    if (r.m1 > 0 && y) hipLaunchKernelGGL(k_scatter, dim3(grid_for(r.m1)), dim3(kBlock), 0, 0, yo.data(), y, r.rows, r.m1);
-    if (r.fixed_count > 0)
-      hipLaunchKernelGGL(k_restore_fixed, dim3(grid_for(r.fixed_count)), dim3(kBlock), 0, 0, xo.data(), r.fixed_idx,
-                         r.fixed_val, r.fixed_count);
-    GP_LAUNCH_CHECK("postsolve scatter");
-    PsState s{};
-    s.x = xo.data();
-    s.y = yo.data();
-    s.z = zo.data();
-    s.retrieved = ret.data();
-    s.has_model = original != nullptr;
-    if (original) s.model = *original;
-    s.tol = 1e-7;
-    PsTape pt{t.types.data(), t.istarts.data(), t.vstarts.data(), t.idx.data(), t.vals.data()};
-    const bool rounds = t.count > 0 && t.count >= parallel_min_records() && t.count < INT32_MAX;
-    // the dual pass is analysed after the primal replay: its pruning reads the final x
-    if (t.count > 0 && !(rounds && replay_rounds(s, pt, static_cast<int32_t>(t.count), m0, n0, true)))
-      hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 1, 0);
-    if (t.count > 0 && !(rounds && replay_rounds(s, pt, static_cast<int32_t>(t.count), m0, n0, false)))
-      hipLaunchKernelGGL(k_replay_serial, dim3(1), dim3(64), 0, 0, s, pt, t.count, 0, 1);
-    GP_LAUNCH_CHECK("postsolve replay");
-    if (original && n0 > 0) {
-      DVec<uint8_t> protect(n0);
-      GP_CHECK(hipMemsetAsync(protect.data(), 0, n0, 0));
-      if (t.count > 0)
-        hipLaunchKernelGGL(k_protect, dim3(grid_for(t.count)), dim3(kBlock), 0, 0, t.types.data(), t.istarts.data(),
-                           t.idx.data(), t.count, n0, protect.data());
-      hipLaunchKernelGGL(k_project, dim3(grid_for(n0)), dim3(kBlock), 0, 0, s, protect.data(), n0);
-      GP_LAUNCH_CHECK("postsolve projection");
-    }
-    if (x_out) d2h(x_out, xo.data(), static_cast<size_t>(n0));
-    if (y_out) d2h(y_out, yo.data(), static_cast<size_t>(m0));
-    if (z_out) d2h(z_out, zo.data(), static_cast<size_t>(n0));
+    auto rows = upload(r.row_red2org, r.m1), cols = upload(r.col_red2org, r.n1);
+    auto fidx = upload(r.fixed_idx, r.fixed_count);
+    auto fval = upload(r.fixed_val, r.fixed_count);
+    const int64_t count = r.record_count;
+    const gp_offset zero = 0;
+    auto types = upload(r.types, count);
+    auto is = upload(count > 0 ? r.index_starts : &zero, count + 1);
+    auto vs = upload(count > 0 ? r.value_starts : &zero, count + 1);
+    auto idx = upload(r.indices, r.index_count);
+    auto vals = upload(r.vals, r.value_count);
+    PostsolveInputs in{};
+    in.m0 = r.m0;
+    in.n0 = r.n0;
+    in.m1 = r.m1;
+    in.n1 = r.n1;
+    in.rows = rows.data();
+    in.cols = cols.data();
+    in.fixed_idx = fidx.data();
+    in.fixed_val = fval.data();
+    in.fixed_count = r.fixed_count;
+    in.tape = PsTape{types.data(), is.data(), vs.data(), idx.data(), vals.data()};
+    in.count = count;
+    in.timeline = &r;
+    postsolve_run(in, x_red, y_red, z_red, original, original_l, x_out, y_out, z_out);
   });
 }

@@ -798,6 +798,13 @@ int gp_refresh_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan
 int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp, const gp_stats* stats, const gp_params* params) {
   return guarded([&] {
     if (!plan || !lp || !stats || !params) throw std::invalid_argument("null argument");
+    // A caller of this entry point may rewrite the plan's vectors and masks,
+    // or pass another LP, between calls.  The per-plan caches (stored entry
+    // bounds, singleton-column row activities, column maxima) assume neither
+    // happens, so each public call starts without them, as the reference's
+    // stateless rule calls do; only the scheduler reuses them.
+    rule_cache_free(plan->rule_cache);
+    plan->rule_cache = nullptr;
     apply_rule(rule, *plan, *lp, *stats, *params);
     GP_CHECK(hipDeviceSynchronize());
   });

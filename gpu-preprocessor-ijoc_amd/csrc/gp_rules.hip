@@ -360,7 +360,14 @@ struct Act {
 // The (l, u) pairs bound strengthening's activity sweep gathered, stored per
 // used entry, and the columns whose bounds the rule then changed.  Valid for
 // the rule applied right after it on the same plan (redundant rows); any
-// other rule invalidates it.
+// other rule invalidates it, and so does every call of the public rule entry
+// point, since its caller may rewrite the plan in between.
+// Invariant the reuse rests on: between the two rules keep_row and keep_col
+// only lose entries.  The sweep stores pairs only for entries it used (live
+// row, live column, |a| > zero_tol, in a workgroup with an active row); an
+// entry the redundancy fold reads is live then, so it was live in the sweep,
+// and its pair was stored unless its column's bounds changed in between,
+// which the changed bits route to a fresh gather.
 struct BsEntryCache {
   DVec<double2> entry_lu;
   DVec<uint32_t> changed;  // bit per column: tightened or fixed

@@ -41,6 +41,10 @@ RED_ROW_DUAL_SHIFT = 10
 RED_VARIABLE_BOUND_CHANGE = 11
 RED_ROW_IMPLIED_BOUND_CHANGE = 12
 RED_DOUBLETON_EQUATION = 13
+RED_FME_COL = 14
+# structural primal recovery kinds (GP_RECOVERY_* of gpupre.h)
+RECOVERY_ABS_VALUE = 0
+RECOVERY_VAR_PAIR = 1
 PHASES = ["cleanup", "propagation", "aggregation", "parallel", "end"]
 KERNEL_FAMILIES = ["live_stats", "activity", "bound_candidates", "compact", "transpose",
                    "rules_other", "hash_sort", "sc_activity", "redundant", "col_max", "live_stats_inc"]
@@ -248,6 +252,7 @@ def load() -> C.CDLL:
         "gp_default_params": (None, [C.POINTER(GpParams)]),
         "gp_preprocess": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpParams), C.POINTER(GpSummary)]),
         "gp_postprocess": (C.c_int, [vp, vp, vp, C.POINTER(GpSummary), C.POINTER(GpDeviceLp), vp, vp, vp]),
+        "gp_postsolve_record": (C.c_int, [vp, vp, vp, C.POINTER(GpHostRecord), C.POINTER(GpDeviceLp), vp, vp, vp, vp]),
         "gp_free_summary": (None, [C.POINTER(GpSummary)]),
         "gp_record_get_sizes": (C.c_int, [C.POINTER(GpSummary), C.POINTER(GpRecordSizes)]),
         "gp_record_copy": (C.c_int, [C.POINTER(GpSummary)] + [vp] * 10),
@@ -488,6 +493,134 @@ def postprocess(summary: Summary, x: np.ndarray, y: np.ndarray, z: np.ndarray, o
     check(load().gp_postprocess(dx.ptr, dy.ptr, dz.ptr, C.byref(summary.s),
                                 C.byref(original.lp) if original is not None else None,
                                 _ptr(xo), _ptr(yo), _ptr(zo)))
+    return xo, yo, zo
+
+
+# ---------------------------------------------------------------- postsolve of a built record
+class GpAbsValueStep(C.Structure):
+    _fields_ = [("split_count", C.c_int32), ("split_t_col", C.c_void_p), ("split_e_col", C.c_void_p),
+                ("split_rho", C.c_void_p), ("pair_count", C.c_int32), ("pair_bound_col", C.c_void_p),
+                ("pair_free_col", C.c_void_p), ("linked_count", C.c_int32), ("linked_slack_col", C.c_void_p),
+                ("linked_t_col", C.c_void_p), ("linked_factor", C.c_void_p), ("max_count", C.c_int32),
+                ("max_slack_col", C.c_void_p), ("max_starts", C.c_void_p), ("max_t_cols", C.c_void_p),
+                ("max_factors", C.c_void_p)]
+
+
+class GpVarPairStep(C.Structure):
+    _fields_ = [("count", C.c_int32), ("plus_cols", C.c_void_p), ("minus_cols", C.c_void_p),
+                ("root_plus_cols", C.c_void_p), ("root_minus_cols", C.c_void_p), ("plus_shift", C.c_void_p),
+                ("minus_shift", C.c_void_p)]
+
+
+class GpCheckpoint(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("payload_index", OFFSET), ("tape_position", OFFSET)]
+
+
+class GpHostRecord(C.Structure):
+    _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
+                ("row_red2org", C.c_void_p), ("col_red2org", C.c_void_p), ("fixed_count", C.c_int32),
+                ("fixed_idx", C.c_void_p), ("fixed_val", C.c_void_p), ("record_count", OFFSET),
+                ("types", C.c_void_p), ("index_starts", C.c_void_p), ("value_starts", C.c_void_p),
+                ("dual_modes", C.c_void_p), ("index_count", OFFSET), ("indices", C.c_void_p),
+                ("value_count", OFFSET), ("vals", C.c_void_p), ("abs_value_count", C.c_int32),
+                ("abs_value", C.POINTER(GpAbsValueStep)), ("var_pair_count", C.c_int32),
+                ("var_pair", C.POINTER(GpVarPairStep)), ("checkpoint_count", C.c_int32),
+                ("checkpoints", C.POINTER(GpCheckpoint))]
+
+
+class HostRecord:
+    """PresolveRecordGpu as host arrays (gp_host_record), kept alive here.
+
+    maps / fixed / tape arguments are array-likes; abs_value is a list of
+    dicts with optional keys splits [(t, e, rho)], outer_pairs [(bound, free)],
+    linked_slacks [(slack, t, factor)], max_slacks [(slack, [t...], [f...])];
+    var_pair is a list of dicts with plus_cols, minus_cols, root_plus_cols,
+    root_minus_cols, plus_shift, minus_shift; checkpoints are
+    (kind, payload_index, tape_position) triples."""
+
+    def __init__(self, m0, n0, m1, n1, row_red2org=(), col_red2org=(), fixed_idx=(), fixed_val=(), types=(),
+                 index_starts=(0,), value_starts=(0,), dual_modes=(), indices=(), vals=(), abs_value=(),
+                 var_pair=(), checkpoints=()):
+        keep = self._keep = []
+
+        def arr(v, dt):
+            a = np.ascontiguousarray(np.asarray(v, dtype=dt))
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p) if a.size else None
+
+        r = self.rec = GpHostRecord()
+        r.m0, r.n0, r.m1, r.n1 = int(m0), int(n0), int(m1), int(n1)
+        r.row_red2org, r.col_red2org = arr(row_red2org, np.int32), arr(col_red2org, np.int32)
+        r.fixed_count = len(fixed_idx)
+        r.fixed_idx, r.fixed_val = arr(fixed_idx, np.int32), arr(fixed_val, np.float64)
+        r.record_count = len(types)
+        r.types = arr(types, np.int32)
+        r.index_starts, r.value_starts = arr(index_starts, NP_OFFSET), arr(value_starts, NP_OFFSET)
+        r.dual_modes = arr(dual_modes, np.uint8)
+        r.index_count, r.value_count = len(indices), len(vals)
+        r.indices, r.vals = arr(indices, np.int32), arr(vals, np.float64)
+        steps = (GpAbsValueStep * max(1, len(abs_value)))()
+        for k, st in enumerate(abs_value):
+            sp = list(st.get("splits", ()))
+            op = list(st.get("outer_pairs", ()))
+            ls = list(st.get("linked_slacks", ()))
+            ms = list(st.get("max_slacks", ()))
+            o = steps[k]
+            o.split_count = len(sp)
+            o.split_t_col = arr([a[0] for a in sp], np.int32)
+            o.split_e_col = arr([a[1] for a in sp], np.int32)
+            o.split_rho = arr([a[2] for a in sp], np.float64)
+            o.pair_count = len(op)
+            o.pair_bound_col = arr([a[0] for a in op], np.int32)
+            o.pair_free_col = arr([a[1] for a in op], np.int32)
+            o.linked_count = len(ls)
+            o.linked_slack_col = arr([a[0] for a in ls], np.int32)
+            o.linked_t_col = arr([a[1] for a in ls], np.int32)
+            o.linked_factor = arr([a[2] for a in ls], np.float64)
+            o.max_count = len(ms)
+            o.max_slack_col = arr([a[0] for a in ms], np.int32)
+            o.max_starts = arr(np.concatenate([[0], np.cumsum([len(a[1]) for a in ms])]) if ms else [0], NP_OFFSET)
+            o.max_t_cols = arr([c for a in ms for c in a[1]], np.int32)
+            o.max_factors = arr([f for a in ms for f in a[2]], np.float64)
+        keep.append(steps)
+        r.abs_value_count = len(abs_value)
+        r.abs_value = C.cast(steps, C.POINTER(GpAbsValueStep))
+        pairs = (GpVarPairStep * max(1, len(var_pair)))()
+        for k, st in enumerate(var_pair):
+            o = pairs[k]
+            o.count = len(st["plus_cols"])
+            o.plus_cols, o.minus_cols = arr(st["plus_cols"], np.int32), arr(st["minus_cols"], np.int32)
+            o.root_plus_cols = arr(st["root_plus_cols"], np.int32)
+            o.root_minus_cols = arr(st["root_minus_cols"], np.int32)
+            o.plus_shift, o.minus_shift = arr(st["plus_shift"], np.float64), arr(st["minus_shift"], np.float64)
+        keep.append(pairs)
+        r.var_pair_count = len(var_pair)
+        r.var_pair = C.cast(pairs, C.POINTER(GpVarPairStep))
+        cps = (GpCheckpoint * max(1, len(checkpoints)))()
+        for k, (kind, payload, pos) in enumerate(checkpoints):
+            cps[k].kind, cps[k].payload_index, cps[k].tape_position = int(kind), int(payload), int(pos)
+        keep.append(cps)
+        r.checkpoint_count = len(checkpoints)
+        r.checkpoints = C.cast(cps, C.POINTER(GpCheckpoint))
+
+    @staticmethod
+    def from_record(rec: "Record") -> "HostRecord":
+        return HostRecord(rec.m0, rec.n0, rec.m1, rec.n1, rec.row_red2org, rec.col_red2org, rec.fixed_idx,
+                          rec.fixed_val, rec.types, rec.index_starts, rec.value_starts, rec.dual_modes,
+                          rec.indices, rec.vals)
+
+
+def postsolve_record(rec: HostRecord, x, y, z, original: GpDeviceLp | None = None, original_l=None):
+    """gp_postsolve_record: x, y, z host arrays in reduced order or None (a
+    null device pointer); `original` a device LP struct (fields may be null);
+    returns original-order host x, y, z."""
+    r = rec.rec
+    dev = [DeviceArray.from_host(np.asarray(v, dtype=np.float64)) if v is not None else None for v in (x, y, z)]
+    ptrs = [d.ptr if d is not None else None for d in dev]
+    xo, yo, zo = np.empty(r.n0), np.empty(r.m0), np.empty(r.n0)
+    check(load().gp_postsolve_record(ptrs[0], ptrs[1], ptrs[2], C.byref(r),
+                                     C.byref(original) if original is not None else None,
+                                     original_l, _ptr(xo), _ptr(yo), _ptr(zo)))
     return xo, yo, zo
 
 

@@ -324,6 +324,98 @@ int gp_record_copy(const gp_summary* summary,
                    int32_t* types, gp_offset* index_starts, gp_offset* value_starts,
                    uint8_t* dual_modes, int32_t* indices, double* vals);
 
+/* ---- postsolve of a caller-built recovery record -------------------------
+ * PresolveRecordGpu (presolve_structs.hpp:338-368) as plain host arrays, for
+ * postsolve_gpu(x_red, y_red, z_red, record, original_model, original_l)
+ * (gpu_postsolve.cu:504-670, declared in gpu_postsolve.hpp).  The primal-only
+ * structural recoveries of the reference's Startup rules are carried by the
+ * same chronological timeline: a checkpoint names a payload and the number of
+ * tape records that preceded it. */
+
+/* PiecewiseLinearReformulationAbsoluteValuePrimalRecoveryStep
+ * (presolve_structs.hpp:207-236): splits x_t <- x_t + x_e,
+ * x_e <- (x_t - x_e) / rho; outer pairs x_free <- x_bound; linked slacks
+ * x_slack <- factor * x_t; max slacks x_slack <- max(l_slack or 0,
+ * max_k factors[k] * x_{t_cols[k]}) over max_t_cols[max_starts[i] ..
+ * max_starts[i+1]). */
+typedef struct gp_abs_value_step {
+  int32_t split_count;
+  const int32_t* split_t_col;
+  const int32_t* split_e_col;
+  const double* split_rho;
+  int32_t pair_count;
+  const int32_t* pair_bound_col;
+  const int32_t* pair_free_col;
+  int32_t linked_count;
+  const int32_t* linked_slack_col;
+  const int32_t* linked_t_col;
+  const double* linked_factor;
+  int32_t max_count;
+  const int32_t* max_slack_col;
+  const gp_offset* max_starts; /* max_count + 1 */
+  const int32_t* max_t_cols;
+  const double* max_factors;
+} gp_abs_value_step;
+
+/* PiecewiseLinearReformulationVariablePairRecoveryStep
+ * (presolve_structs.hpp:238-250): x_plus <- plus_shift + x_root_plus,
+ * x_minus <- minus_shift + x_root_minus, roots read before any write. */
+typedef struct gp_var_pair_step {
+  int32_t count;
+  const int32_t* plus_cols;
+  const int32_t* minus_cols;
+  const int32_t* root_plus_cols;
+  const int32_t* root_minus_cols;
+  const double* plus_shift;
+  const double* minus_shift;
+} gp_var_pair_step;
+
+/* PrimalRecoveryKind / PrimalRecoveryCheckpoint (presolve_structs.hpp:252-266). */
+enum { GP_RECOVERY_ABS_VALUE = 0, GP_RECOVERY_VAR_PAIR = 1 };
+typedef struct gp_checkpoint {
+  int32_t kind;
+  gp_offset payload_index;
+  gp_offset tape_position;
+} gp_checkpoint;
+
+typedef struct gp_host_record {
+  int32_t m0, n0, m1, n1;
+  const int32_t* row_red2org; /* m1 */
+  const int32_t* col_red2org; /* n1 */
+  int32_t fixed_count;
+  const int32_t* fixed_idx;
+  const double* fixed_val;
+  gp_offset record_count;
+  const int32_t* types;
+  const gp_offset* index_starts; /* record_count + 1 (may be NULL when empty) */
+  const gp_offset* value_starts; /* record_count + 1 (may be NULL when empty) */
+  const uint8_t* dual_modes;
+  gp_offset index_count;
+  const int32_t* indices;
+  gp_offset value_count;
+  const double* vals;
+  int32_t abs_value_count;
+  const gp_abs_value_step* abs_value;
+  int32_t var_pair_count;
+  const gp_var_pair_step* var_pair;
+  int32_t checkpoint_count;
+  const gp_checkpoint* checkpoints;
+} gp_host_record;
+
+/* postsolve_gpu() over a host record: validates the tape and the timeline
+ * (validate_postsolve_tape / validate_primal_recovery_timeline,
+ * gpu_postsolve.cu:400-503; GP_ERR_RUNTIME when malformed), scatters the
+ * reduced vectors, restores fixed columns, replays primal ranges between
+ * checkpoints in reverse with their structural payloads, then the dual pass,
+ * then projects column duals from `original` when it carries AT, c, l and u
+ * and the LP has rows.  x_red, y_red, z_red are device arrays in reduced
+ * order and may be NULL; `original` (device LP, fields may be NULL) and
+ * `original_l` (device, n0; defaults to original->l) may be NULL.  Outputs
+ * are host arrays x_out (n0), y_out (m0), z_out (n0). */
+int gp_postsolve_record(const double* x_red, const double* y_red, const double* z_red,
+                        const gp_host_record* record, const gp_device_lp* original,
+                        const double* original_l, double* x_out, double* y_out, double* z_out);
+
 /* ---- launch-schedule autotuner (no reference counterpart) ---------------- */
 
 /* Result-invariant launch schedule: lanes per row of the grouped sweeps

@@ -91,6 +91,11 @@ void or_result_free(or_result* r);
 
 /* Postsolve (gpu_postsolve.cu:535-670). x,y,z in reduced order; outputs in
  * original order.  `original` may be NULL (no original-model handlers). */
+/* postsolve_gpu over a caller-built record (gpu_postsolve.cu:400-670):
+ * returns -1 when the tape or the structural timeline is malformed. */
+int or_postsolve_record(const gp_host_record* r, const gp_device_lp* original, const double* original_l,
+                        const double* x, const double* y, const double* z, double* x_out, double* y_out,
+                        double* z_out);
 int or_postsolve(const or_result* r, const gp_device_lp* original, const double* x,
                  const double* y, const double* z, double* x_out, double* y_out, double* z_out);
 

@@ -51,6 +51,8 @@ def load():
             "or_presolve": (C.c_int, [C.POINTER(G.GpDeviceLp), C.POINTER(G.GpParams), C.POINTER(OrResult)]),
             "or_result_free": (None, [C.POINTER(OrResult)]),
             "or_postsolve": (C.c_int, [C.POINTER(OrResult), C.POINTER(G.GpDeviceLp), vp, vp, vp, vp, vp, vp]),
+            "or_postsolve_record": (C.c_int, [C.POINTER(G.GpHostRecord), C.POINTER(G.GpDeviceLp), vp, vp, vp, vp,
+                                              vp, vp, vp]),
             "or_plan_create": (C.c_int, [C.POINTER(G.GpDeviceLp), C.POINTER(G.GpPlan)]),
             "or_plan_destroy": (None, [C.POINTER(G.GpPlan)]),
             "or_plan_enable_tape": (None, [C.POINTER(G.GpPlan)]),
@@ -162,6 +164,22 @@ def postsolve(res: Result, original: "G.HostLP | None", x, y, z):
     x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
     load().or_postsolve(C.byref(res._raw), C.byref(view.lp) if view else None, _p(x), _p(y), _p(z),
                         _p(xo), _p(yo), _p(zo))
+    return xo, yo, zo
+
+
+def postsolve_record(rec: "G.HostRecord", original: "G.GpDeviceLp | None", x, y, z, original_l=None):
+    """or_postsolve_record over a host record; `original` is a gp_device_lp of
+    HOST arrays (fields may be null).  Returns (x, y, z) or None when the
+    record is rejected as malformed."""
+    r = rec.rec
+    xo, yo, zo = np.empty(r.n0), np.empty(r.m0), np.empty(r.n0)
+    ins = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None for v in (x, y, z)]
+    ol = np.ascontiguousarray(original_l, dtype=np.float64) if original_l is not None else None
+    rc = load().or_postsolve_record(C.byref(r), C.byref(original) if original is not None else None,
+                                    _p(ol) if ol is not None else None,
+                                    *[_p(v) if v is not None else None for v in ins], _p(xo), _p(yo), _p(zo))
+    if rc != 0:
+        return None
     return xo, yo, zo
 
 

@@ -389,8 +389,50 @@ static void replay_row_implied(ps_state* s, const int32_t* ix, int64_t i0, const
   s->retrieved[col] = 1;
 }
 
+/* postsolve_projected_variables.cuh:14-68 (FmeCol): indices K, column, K
+ * rows, N, L, (orient, len) x L, N columns; values (constant, coefficients)
+ * x L.  Largest lower side, else smallest upper side, else 0. */
+static void replay_projected(ps_state* s, const int32_t* ix, int64_t i0, const double* v, int64_t v0,
+                             int primal, int dual) {
+  const int32_t K = ix[i0], col = ix[i0 + 1];
+  const int32_t L = ix[i0 + 3 + K];
+  if (primal) {
+    int64_t meta = i0 + 4 + K, cp = meta + 2 * (int64_t)L, vp = v0;
+    double lo = -INFINITY, hi = INFINITY;
+    int has_lo = 0, has_hi = 0;
+    for (int32_t side = 0; side < L; ++side) {
+      const int32_t orient = ix[meta++];
+      const int32_t len = ix[meta++];
+      double cand = v[vp];
+      for (int32_t t = 0; t < len; ++t) cand += v[vp + 1 + t] * s->x[ix[cp++]];
+      if (orient > 0) {
+        lo = or_cmax(lo, cand);
+        has_lo = 1;
+      } else {
+        hi = or_cmin(hi, cand);
+        has_hi = 1;
+      }
+      vp += 1 + len;
+    }
+    s->x[col] = has_lo ? lo : (has_hi ? hi : 0.0);
+  }
+  if (dual) {
+    s->z[col] = 0.0;
+    s->retrieved[col] = 1;
+    for (int32_t t = 0; t < K; ++t) s->y[ix[i0 + 2 + t]] = 0.0;
+  }
+}
+
+typedef struct ps_tape {
+  const int32_t* types;
+  const gp_offset* istarts;
+  const gp_offset* vstarts;
+  const int32_t* idx;
+  const double* vals;
+} ps_tape;
+
 /* replay_postsolve_record dispatch (gpu_postsolve.cu:85-147) */
-static void replay_record(ps_state* s, const or_tape* t, int64_t k, int primal, int dual) {
+static void replay_record(ps_state* s, const ps_tape* t, int64_t k, int primal, int dual) {
   const int32_t type = t->types[k];
   const int64_t i0 = t->istarts[k], i1 = t->istarts[k + 1] - 1;
   const int64_t v0 = t->vstarts[k], v1 = t->vstarts[k + 1] - 1;
@@ -403,6 +445,7 @@ static void replay_record(ps_state* s, const or_tape* t, int64_t k, int primal, 
     case GP_RED_SINGLETON_COLUMN_SUBSTITUTION: replay_singleton_column(s, ix, i0, v, v0, primal, dual); break;
     case GP_RED_FIXED_VARIABLE: replay_fixed(s, ix, i0, i1, v, v0, primal, dual); break;
     case GP_RED_INFINITE_BOUND_DUAL_FIX: replay_infinite(s, ix, i0, v, v0, primal, dual); break;
+    case GP_RED_FME_COL: replay_projected(s, ix, i0, v, v0, primal, dual); break;
     case GP_RED_ROW_DUAL_SHIFT:
       if (dual) s->y[ix[i0]] += v[v0];
       break;
@@ -439,9 +482,134 @@ static void project_column(ps_state* s, int32_t col) {
   s->z[col] = zj;
 }
 
-int or_postsolve(const or_result* r, const gp_device_lp* original, const double* x, const double* y,
-                 const double* z, double* x_out, double* y_out, double* z_out) {
+/* Structural primal recoveries: absolute-value steps
+ * (postsolve_piecewise_linear_reformulation_absolute_value.cu:9-103,199-205)
+ * and variable-pair steps (postsolve_piecewise_linear_reformulation_variable_pair.cu:9-41). */
+static void restore_abs_value(double* x, const gp_abs_value_step* st, const double* lower) {
+  for (int32_t i = 0; i < st->split_count; ++i) {
+    const double t = x[st->split_t_col[i]], e = x[st->split_e_col[i]];
+    x[st->split_t_col[i]] = t + e;
+    x[st->split_e_col[i]] = (t - e) / st->split_rho[i];
+  }
+  for (int32_t i = 0; i < st->pair_count; ++i) x[st->pair_free_col[i]] = x[st->pair_bound_col[i]];
+  for (int32_t i = 0; i < st->linked_count; ++i)
+    x[st->linked_slack_col[i]] = st->linked_factor[i] * x[st->linked_t_col[i]];
+  for (int32_t i = 0; i < st->max_count; ++i) {
+    if (st->max_starts[i + 1] == st->max_starts[i]) continue;
+    const int32_t sc = st->max_slack_col[i];
+    double v = lower ? lower[sc] : 0.0;
+    if (!isfinite(v)) v = 0.0;
+    for (gp_offset q = st->max_starts[i]; q < st->max_starts[i + 1]; ++q)
+      v = or_cmax(v, st->max_factors[q] * x[st->max_t_cols[q]]);
+    x[sc] = v;
+  }
+}
+
+static void restore_var_pair(double* x, const gp_var_pair_step* st) {
+  double* cp = (double*)or_xcalloc((size_t)(st->count > 0 ? st->count : 1), sizeof(double));
+  double* cm = (double*)or_xcalloc((size_t)(st->count > 0 ? st->count : 1), sizeof(double));
+  for (int32_t i = 0; i < st->count; ++i) {
+    cp[i] = x[st->root_plus_cols[i]];
+    cm[i] = x[st->root_minus_cols[i]];
+  }
+  for (int32_t i = 0; i < st->count; ++i) {
+    x[st->plus_cols[i]] = st->plus_shift[i] + cp[i];
+    x[st->minus_cols[i]] = st->minus_shift[i] + cm[i];
+  }
+  free(cp);
+  free(cm);
+}
+
+static int bad_col(int32_t c, int32_t n0) { return c < 0 || c >= n0; }
+
+/* validate_postsolve_tape (gpu_postsolve.cu:400-437) */
+static int validate_tape(const gp_host_record* r) {
+  const int64_t count = r->record_count;
+  if (count < 0 || r->index_count < 0 || r->value_count < 0) return -1;
+  if (count == 0)
+    return ((r->index_starts && r->index_starts[0] != 0) || (r->value_starts && r->value_starts[0] != 0) ||
+            r->index_count != 0 || r->value_count != 0) ? -1 : 0;
+  if (!r->types || !r->index_starts || !r->value_starts || !r->dual_modes) return -1;
+  if (r->index_starts[0] != 0 || r->value_starts[0] != 0) return -1;
+  for (int64_t k = 0; k < count; ++k) {
+    if (r->index_starts[k] < 0 || r->index_starts[k] > r->index_starts[k + 1] || r->value_starts[k] < 0 ||
+        r->value_starts[k] > r->value_starts[k + 1])
+      return -1;
+    if (r->types[k] < GP_RED_FIXED_VARIABLE || r->types[k] > GP_RED_FME_COL) return -1;
+    if (r->dual_modes[k] > GP_DUAL_MINIMAL) return -1;
+  }
+  if (r->index_starts[count] != r->index_count || r->value_starts[count] != r->value_count) return -1;
+  return 0;
+}
+
+/* validate_primal_recovery_timeline (gpu_postsolve.cu:439-503) */
+static int validate_timeline(const gp_host_record* r) {
+  const int32_t na = r->abs_value_count > 0 ? r->abs_value_count : 0;
+  const int32_t nv = r->var_pair_count > 0 ? r->var_pair_count : 0;
+  uint8_t* seen_a = (uint8_t*)or_xcalloc((size_t)(na ? na : 1), 1);
+  uint8_t* seen_v = (uint8_t*)or_xcalloc((size_t)(nv ? nv : 1), 1);
+  int bad = 0;
+  int64_t prev = 0;
+  for (int32_t c = 0; c < r->checkpoint_count && !bad; ++c) {
+    const gp_checkpoint* cp = &r->checkpoints[c];
+    if (cp->tape_position < prev || cp->tape_position > r->record_count) {
+      bad = 1;
+      break;
+    }
+    prev = cp->tape_position;
+    if (cp->kind == GP_RECOVERY_ABS_VALUE) {
+      if (cp->payload_index < 0 || cp->payload_index >= na || seen_a[cp->payload_index]) {
+        bad = 1;
+        break;
+      }
+      seen_a[cp->payload_index] = 1;
+      const gp_abs_value_step* st = &r->abs_value[cp->payload_index];
+      for (int32_t i = 0; i < st->split_count; ++i)
+        bad |= bad_col(st->split_t_col[i], r->n0) || bad_col(st->split_e_col[i], r->n0);
+      for (int32_t i = 0; i < st->pair_count; ++i)
+        bad |= bad_col(st->pair_bound_col[i], r->n0) || bad_col(st->pair_free_col[i], r->n0);
+      for (int32_t i = 0; i < st->linked_count; ++i)
+        bad |= bad_col(st->linked_slack_col[i], r->n0) || bad_col(st->linked_t_col[i], r->n0);
+      for (int32_t i = 0; i < st->max_count && !bad; ++i) {
+        bad |= st->max_starts[i + 1] < st->max_starts[i] || bad_col(st->max_slack_col[i], r->n0);
+        for (gp_offset q = st->max_starts[i]; q < st->max_starts[i + 1] && !bad; ++q)
+          bad |= bad_col(st->max_t_cols[q], r->n0);
+      }
+    } else if (cp->kind == GP_RECOVERY_VAR_PAIR) {
+      if (cp->payload_index < 0 || cp->payload_index >= nv || seen_v[cp->payload_index]) {
+        bad = 1;
+        break;
+      }
+      seen_v[cp->payload_index] = 1;
+      const gp_var_pair_step* st = &r->var_pair[cp->payload_index];
+      for (int32_t i = 0; i < st->count; ++i)
+        bad |= bad_col(st->plus_cols[i], r->n0) || bad_col(st->minus_cols[i], r->n0) ||
+               bad_col(st->root_plus_cols[i], r->n0) || bad_col(st->root_minus_cols[i], r->n0);
+    } else {
+      bad = 1;
+    }
+  }
+  for (int32_t i = 0; i < na && !bad; ++i) bad |= !seen_a[i];
+  for (int32_t i = 0; i < nv && !bad; ++i) bad |= !seen_v[i];
+  free(seen_a);
+  free(seen_v);
+  return bad ? -1 : 0;
+}
+
+static void replay_range(ps_state* s, const ps_tape* t, int64_t begin, int64_t end, int primal) {
+  for (int64_t k = end - 1; k >= begin; --k) replay_record(s, t, k, primal, !primal);
+}
+
+/* postsolve_gpu (gpu_postsolve.cu:535-670), serial: scatter, fixed restore,
+ * primal replay between checkpoints in reverse with their structural
+ * payloads, dual replay, projection (needs x, y, z and the model's AT, c, l,
+ * u; the reference has no y without rows). */
+int or_postsolve_record(const gp_host_record* r, const gp_device_lp* original, const double* original_l,
+                        const double* x, const double* y, const double* z, double* x_out, double* y_out,
+                        double* z_out) {
+  if (validate_tape(r) || validate_timeline(r)) return -1;
   const int32_t m0 = r->m0, n0 = r->n0;
+  if (!original_l && original) original_l = original->l;
   ps_state s;
   s.x = x_out;
   s.y = y_out;
@@ -460,20 +628,30 @@ int or_postsolve(const or_result* r, const gp_device_lp* original, const double*
   for (int32_t i = 0; i < r->m1; ++i)
     if (y) y_out[r->row_red2org[i]] = y[i];
   for (int32_t k = 0; k < r->fixed_count; ++k) x_out[r->fixed_idx[k]] = r->fixed_val[k];
-  for (int64_t k = r->tape.count - 1; k >= 0; --k) replay_record(&s, &r->tape, k, 1, 0);
-  for (int64_t k = r->tape.count - 1; k >= 0; --k) replay_record(&s, &r->tape, k, 0, 1);
-  if (original && n0 > 0) {
+  const ps_tape t = {r->types, r->index_starts, r->value_starts, r->indices, r->vals};
+  int64_t range_end = r->record_count;
+  for (int32_t c = r->checkpoint_count - 1; c >= 0; --c) {
+    const gp_checkpoint* cp = &r->checkpoints[c];
+    replay_range(&s, &t, cp->tape_position, range_end, 1);
+    if (cp->kind == GP_RECOVERY_ABS_VALUE) restore_abs_value(x_out, &r->abs_value[cp->payload_index], original_l);
+    else restore_var_pair(x_out, &r->var_pair[cp->payload_index]);
+    range_end = cp->tape_position;
+  }
+  replay_range(&s, &t, 0, range_end, 1);
+  replay_range(&s, &t, 0, r->record_count, 0);
+  if (original && n0 > 0 && m0 > 0 && original->AT.rows > 0 && original->AT.row_ptr && original->c &&
+      original->l && original->u) {
     uint8_t* protect = (uint8_t*)or_xcalloc((size_t)n0, 1);
-    for (int64_t k = 0; k < r->tape.count; ++k) {
-      const int32_t type = r->tape.types[k];
-      const int64_t i0 = r->tape.istarts[k];
+    for (int64_t k = 0; k < r->record_count; ++k) {
+      const int32_t type = r->types[k];
+      const int64_t i0 = r->index_starts[k];
       int32_t a = -1, b = -1;
-      if (type == GP_RED_SINGLETON_COLUMN_SUBSTITUTION || type == GP_RED_PARALLEL_COLUMN) a = r->tape.idx[i0];
+      if (type == GP_RED_SINGLETON_COLUMN_SUBSTITUTION || type == GP_RED_PARALLEL_COLUMN) a = r->indices[i0];
       if (type == GP_RED_DOUBLETON_EQUATION) {
-        a = r->tape.idx[i0];
-        b = r->tape.idx[i0 + 1];
+        a = r->indices[i0];
+        b = r->indices[i0 + 1];
       }
-      if (type == GP_RED_INFINITE_BOUND_DUAL_FIX) a = r->tape.idx[i0 + 1];
+      if (type == GP_RED_INFINITE_BOUND_DUAL_FIX) a = r->indices[i0 + 1];
       if (a >= 0 && a < n0) protect[a] = 1;
       if (b >= 0 && b < n0) protect[b] = 1;
     }
@@ -483,4 +661,29 @@ int or_postsolve(const or_result* r, const gp_device_lp* original, const double*
   }
   free(s.retrieved);
   return 0;
+}
+
+int or_postsolve(const or_result* r, const gp_device_lp* original, const double* x, const double* y,
+                 const double* z, double* x_out, double* y_out, double* z_out) {
+  gp_host_record h;
+  memset(&h, 0, sizeof h);
+  h.m0 = r->m0;
+  h.n0 = r->n0;
+  h.m1 = r->m1;
+  h.n1 = r->n1;
+  h.row_red2org = r->row_red2org;
+  h.col_red2org = r->col_red2org;
+  h.fixed_count = r->fixed_count;
+  h.fixed_idx = r->fixed_idx;
+  h.fixed_val = r->fixed_val;
+  h.record_count = (gp_offset)r->tape.count;
+  h.types = r->tape.types;
+  h.index_starts = r->tape.istarts;
+  h.value_starts = r->tape.vstarts;
+  h.dual_modes = r->tape.modes;
+  h.index_count = (gp_offset)r->tape.icount;
+  h.indices = r->tape.idx;
+  h.value_count = (gp_offset)r->tape.vcount;
+  h.vals = r->tape.vals;
+  return or_postsolve_record(&h, original, NULL, x, y, z, x_out, y_out, z_out);
 }

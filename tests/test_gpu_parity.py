@@ -299,3 +299,55 @@ def test_incremental_live_stats_match_full_refresh():
         assert_same(getattr(r1, k), getattr(r0, k), f"record {k}")
     for k in ("row_ptr", "col_idx", "vals", "c", "l", "u", "AL", "AU"):
         assert_same(getattr(d1, k), getattr(d0, k), f"reduced {k}")
+
+
+@pytest.mark.parametrize("first,second,seed", [("bound_strengthening", "redundant_rows", 0),
+                                               ("singleton_columns", "singleton_columns", 2),
+                                               ("bound_strengthening", "bound_strengthening", 0)])
+def test_rule_sequence_with_plan_rewrites(first, second, seed):
+    """A caller of the public rule entry point rewrites the plan between two
+    rule calls (bounds of a column the first rule left alone, a column turned
+    back on).  The second call must see the rewritten plan: nothing cached by
+    the first call may survive into it."""
+    print(f"[rule sequence] {first} -> rewrite -> {second}", flush=True)
+    lp = small_lp(4)
+    params = G.parity_params(record_postsolve_tape_cpu=1)
+    # 15% of the rows and 1% of the columns off; seeds where both calls reduce
+    rng = np.random.default_rng(seed)
+    kr = (rng.random(lp.m) > 0.15).astype(np.uint8)
+    kc0 = (rng.random(lp.n) > 0.01).astype(np.uint8)
+    ob, gb = OracleBackend(lp), GpuBackend(lp)
+    try:
+        for b in (ob, gb):
+            b.set("keep_row_mask", kr)
+            b.set("keep_col_mask", kc0)
+            b.refresh()
+            b.apply(first, params)
+        for f in O.Plan._FIELDS:
+            assert_same(gb.get(f), ob.get(f), f"{first}: {f}")
+        l0, u0 = ob.get("new_l"), ob.get("new_u")
+        kc = ob.get("keep_col_mask")
+        same = np.nonzero((l0 == lp.l) & (u0 == lp.u) & (kc == 1))[0]
+        off = np.nonzero(kc == 0)[0]
+        rng = np.random.default_rng(5)
+        pick = rng.choice(same, size=min(40, same.size), replace=False)
+        l1 = l0.copy()
+        # tighten finite lower bounds upward, leave infinite ones
+        fin = np.isfinite(l1[pick]) & np.isfinite(u0[pick])
+        l1[pick[fin]] = l1[pick[fin]] + 0.25 * (u0[pick[fin]] - l1[pick[fin]])
+        kc1 = kc.copy()
+        if off.size:
+            kc1[off[: max(1, off.size // 3)]] = 1
+        for b in (ob, gb):
+            b.set("new_l", l1)
+            b.set("keep_col_mask", kc1)
+            b.refresh()
+            b.apply(second, params)
+        for f in O.Plan._FIELDS:
+            assert_same(gb.get(f), ob.get(f), f"{second} after rewrite: {f}")
+        assert gb.flags() == ob.flags(), (gb.flags(), ob.flags())
+        tg, to = gb.tape(), ob.tape()
+        for k in to:
+            assert_same(tg[k], to[k], f"{second}: tape {k}")
+    finally:
+        gb.free()
