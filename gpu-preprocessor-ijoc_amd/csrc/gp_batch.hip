@@ -207,6 +207,48 @@ double obj_delta(const double* contrib, int64_t n, const int32_t* item_line, boo
   return total;
 }
 
+namespace {
+// one lane per segment of the ordered column list: sequential x87 sum
+__global__ void k_x87_segments(const double* c, const double* v, const int32_t* cols, const int64_t* seg,
+                               int32_t segs, double* out) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= segs) return;
+  X80 acc = x80_zero();
+  for (int64_t i = seg[k]; i < seg[k + 1]; ++i) {
+    const int32_t j = cols[i];
+    acc = x80_add(acc, x80_mul(x80_from_double(c[j]), x80_from_double(v[j])));
+  }
+  out[k] = x80_to_double(acc);
+}
+}  // namespace
+
+double obj_delta_x87(const double* c, const double* v, const int32_t* cols, int64_t count) {
+  if (count <= 0) return 0.0;
+  DVec<int64_t> whole(2);
+  const int64_t h[2] = {0, count};
+  h2d(whole.data(), h, 2);
+  DVec<double> total(1);
+  hipLaunchKernelGGL(k_x87_segments, dim3(1), dim3(64), 0, 0, c, v, cols, static_cast<const int64_t*>(whole.data()), 1,
+                     total.data());
+  GP_LAUNCH_CHECK("extended objective sum");
+  BatchContext* bc = batch_context();
+  if (bc && bc->count > 0) {
+    if (!bc->col_map) throw Error(GP_ERR_RUNTIME, "batch context without line maps");
+    const int32_t segs = bc->count;
+    DVec<int64_t> seg(static_cast<size_t>(segs) + 1);
+    DVec<double> sums(static_cast<size_t>(segs));
+    hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(segs + 1)), dim3(kBlock), 0, 0, cols, count, bc->col_map->data(),
+                       bc->col_off.data(), segs, seg.data());
+    hipLaunchKernelGGL(k_x87_segments, dim3(grid_for(segs)), dim3(kBlock), 0, 0, c, v, cols,
+                       static_cast<const int64_t*>(seg.data()), segs, sums.data());
+    GP_LAUNCH_CHECK("batch extended objective split");
+    std::vector<double> hs(static_cast<size_t>(segs));
+    d2h(hs.data(), sums.data(), hs.size());
+    for (int32_t k = 0; k < segs; ++k) bc->obj[k] += hs[k];
+  }
+  return read1(total.data());
+}
+
 }  // namespace gp
 
 using namespace gp;

@@ -679,6 +679,188 @@ BatchContext*& batch_context();
 // when item_rows is set, else a column; items must be in increasing line order.
 double obj_delta(const double* contrib, int64_t n, const int32_t* item_line = nullptr, bool item_rows = false);
 
+// ---- x87 extended-precision accumulation ------------------------------------------------
+// The reference sums the objective of the columns bound strengthening fixes on
+// the host in long double, sequentially in column order
+// (fixed_variable_tape.cu:305-325): each product c * v and each partial sum is
+// rounded to a 64-bit significand (x86 extended precision), and the total is
+// rounded to binary64 at the end.  X80 reproduces that arithmetic exactly in
+// integer code: value = (-1)^neg * m * 2^e with m normalized (bit 63 set),
+// round-to-nearest-even at every step.  Products of binary64 values and their
+// sums stay far inside the extended exponent range, so only the final
+// conversion handles overflow and subnormal results.
+struct X80 {
+  unsigned long long m;
+  int32_t e;
+  uint32_t neg;
+  uint32_t cls;  // 0 zero, 1 finite, 2 infinity, 3 NaN
+};
+
+__device__ inline X80 x80_zero() { return X80{0ull, 0, 0u, 0u}; }
+
+__device__ inline X80 x80_from_double(double d) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(d));
+  X80 r{0ull, 0, static_cast<uint32_t>(b >> 63), 0u};
+  const int ef = static_cast<int>((b >> 52) & 0x7ff);
+  const unsigned long long f = b & ((1ull << 52) - 1);
+  if (ef == 0x7ff) {
+    r.cls = f ? 3u : 2u;
+    return r;
+  }
+  unsigned long long mant = ef ? (f | (1ull << 52)) : f;
+  if (mant == 0) return r;  // signed zero
+  const int ex = ef ? ef - 1075 : -1074;
+  const int sh = __clzll(static_cast<long long>(mant));
+  r.m = mant << sh;
+  r.e = ex - sh;
+  r.cls = 1u;
+  return r;
+}
+
+// round the 128-bit significand (hi:lo, bit 127 set) to 64 bits; value = hi:lo * 2^(e-64)
+__device__ inline X80 x80_round128(unsigned long long hi, unsigned long long lo, int32_t e, uint32_t neg) {
+  const bool round = (lo >> 63) != 0;
+  const bool sticky = (lo & ~(1ull << 63)) != 0;
+  if (round && (sticky || (hi & 1ull))) {
+    ++hi;
+    if (hi == 0) {
+      hi = 1ull << 63;
+      ++e;
+    }
+  }
+  return X80{hi, e, neg, 1u};
+}
+
+__device__ inline X80 x80_mul(const X80& a, const X80& b) {
+  const uint32_t neg = a.neg ^ b.neg;
+  if (a.cls == 3u || b.cls == 3u || (a.cls == 2u && b.cls == 0u) || (a.cls == 0u && b.cls == 2u))
+    return X80{0ull, 0, 1u, 3u};
+  if (a.cls == 2u || b.cls == 2u) return X80{0ull, 0, neg, 2u};
+  if (a.cls == 0u || b.cls == 0u) return X80{0ull, 0, neg, 0u};
+  unsigned long long hi = __umul64hi(a.m, b.m);
+  unsigned long long lo = a.m * b.m;
+  int32_t e = a.e + b.e + 64;
+  if (!(hi >> 63)) {  // product in [2^126, 2^127): one normalizing shift
+    hi = (hi << 1) | (lo >> 63);
+    lo <<= 1;
+    e -= 1;
+  }
+  return x80_round128(hi, lo, e, neg);
+}
+
+__device__ inline X80 x80_add(const X80& a, const X80& b) {
+  if (a.cls == 3u || b.cls == 3u) return X80{0ull, 0, 1u, 3u};
+  if (a.cls == 2u || b.cls == 2u) {
+    if (a.cls == 2u && b.cls == 2u && a.neg != b.neg) return X80{0ull, 0, 1u, 3u};
+    return a.cls == 2u ? a : b;
+  }
+  if (a.cls == 0u && b.cls == 0u) return X80{0ull, 0, a.neg & b.neg, 0u};  // -0 only for -0 + -0
+  if (a.cls == 0u) return b;
+  if (b.cls == 0u) return a;
+  // big >= small in magnitude
+  const bool a_big = a.e > b.e || (a.e == b.e && a.m >= b.m);
+  const X80& big = a_big ? a : b;
+  const X80& sml = a_big ? b : a;
+  const uint32_t d = static_cast<uint32_t>(big.e - sml.e);
+  // 128-bit significands: big = (big.m : 0), small shifted right by d with sticky
+  unsigned long long bh = big.m, bl = 0ull, sh, sl;
+  if (d == 0) {
+    sh = sml.m;
+    sl = 0ull;
+  } else if (d < 64) {
+    sh = sml.m >> d;
+    sl = sml.m << (64 - d);
+  } else if (d < 128) {
+    sh = 0ull;
+    sl = d == 64 ? sml.m : (sml.m >> (d - 64));
+    const unsigned long long lost = d == 64 ? 0ull : (sml.m << (128 - d));
+    if (lost) sl |= 1ull;
+  } else {
+    sh = 0ull;
+    sl = 1ull;  // sticky only
+  }
+  int32_t e = big.e;
+  if (a.neg == b.neg) {
+    unsigned long long rl = bl + sl;
+    unsigned long long carry = rl < bl ? 1ull : 0ull;
+    unsigned long long rh = bh + sh + carry;
+    const bool out = rh < bh || (rh == bh && (sh | carry) != 0ull);
+    if (out) {  // carry beyond bit 127: shift right one, keep the sticky bit
+      const unsigned long long st = rl & 1ull;
+      rl = (rl >> 1) | (rh << 63) | st;
+      rh = (rh >> 1) | (1ull << 63);
+      e += 1;
+    }
+    return x80_round128(rh, rl, e, big.neg);
+  }
+  // magnitude difference (big - small >= 0)
+  unsigned long long rl = bl - sl;
+  const unsigned long long borrow = bl < sl ? 1ull : 0ull;
+  unsigned long long rh = bh - sh - borrow;
+  if (rh == 0ull && rl == 0ull) return X80{0ull, 0, 0u, 0u};  // exact cancellation: +0
+  int z;
+  if (rh) {
+    z = __clzll(static_cast<long long>(rh));
+  } else {
+    z = 64 + __clzll(static_cast<long long>(rl));
+  }
+  if (z >= 64) {
+    rh = rl << (z - 64);
+    rl = 0ull;
+  } else if (z > 0) {
+    rh = (rh << z) | (rl >> (64 - z));
+    rl <<= z;
+  }
+  e -= z;
+  return x80_round128(rh, rl, e, big.neg);
+}
+
+// round to binary64, nearest-even (subnormal results and overflow included)
+__device__ inline double x80_to_double(const X80& x) {
+  const unsigned long long sign = static_cast<unsigned long long>(x.neg) << 63;
+  if (x.cls == 0u) return __longlong_as_double(static_cast<long long>(sign));
+  if (x.cls == 2u) return __longlong_as_double(static_cast<long long>(sign | 0x7ff0000000000000ull));
+  if (x.cls == 3u) return __longlong_as_double(static_cast<long long>(sign | 0x7ff8000000000000ull));
+  const int32_t ue = x.e + 63;  // value in [2^ue, 2^(ue+1))
+  if (ue > 1023) return __longlong_as_double(static_cast<long long>(sign | 0x7ff0000000000000ull));
+  int32_t shift = ue >= -1022 ? 11 : 11 + (-1022 - ue);
+  unsigned long long q;
+  bool rb, st;
+  if (shift >= 65) {
+    q = 0ull;
+    rb = false;
+    st = true;
+  } else if (shift == 64) {
+    q = 0ull;
+    rb = (x.m >> 63) != 0ull;
+    st = (x.m & ~(1ull << 63)) != 0ull;
+  } else {
+    q = x.m >> shift;
+    rb = ((x.m >> (shift - 1)) & 1ull) != 0ull;
+    st = (x.m & ((1ull << (shift - 1)) - 1ull)) != 0ull;
+  }
+  if (rb && (st || (q & 1ull))) ++q;
+  unsigned long long bits;
+  if (ue >= -1022) {
+    // q in [2^52, 2^53]; a carry to 2^53 moves to the next binade
+    int32_t be = ue + 1023;
+    if (q >> 53) {
+      q >>= 1;
+      ++be;
+    }
+    if (be >= 0x7ff) return __longlong_as_double(static_cast<long long>(sign | 0x7ff0000000000000ull));
+    bits = (static_cast<unsigned long long>(be) << 52) | (q & ((1ull << 52) - 1ull));
+  } else {
+    bits = q;  // subnormal; a carry to 2^52 is the smallest normal, encoded alike
+  }
+  return __longlong_as_double(static_cast<long long>(sign | bits));
+}
+
+// Objective delta of fixed columns the reference's way: sum over the columns
+// listed in `cols` (increasing), in order, of c[j] * v[j] in x87 extended
+// precision, rounded to binary64 once; in a batch also per instance.
+double obj_delta_x87(const double* c, const double* v, const int32_t* cols, int64_t count);
+
 // ---- learned pass scheduler (gp_learned.hip) ---------------------------------------------
 int rule_group(int32_t rule);
 void schedule_features(const gp_round_trace* hist, int32_t done, int32_t phase, int64_t nnz_now, int64_t nnz0,

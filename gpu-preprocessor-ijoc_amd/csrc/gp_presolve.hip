@@ -795,6 +795,24 @@ int gp_refresh_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan
   });
 }
 
+int gp_update_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan* plan, const uint8_t* old_keep_row,
+                         const uint8_t* old_keep_col, int32_t* accepted) {
+  return guarded([&] {
+    if (!stats || !lp || !plan || !old_keep_row || !old_keep_col || !accepted)
+      throw std::invalid_argument("null argument");
+    const int32_t m = lp->A.rows, n = lp->A.cols;
+    DVec<uint8_t> touched_rows(m ? m : 1), touched_cols(n ? n : 1);
+    DVec<int64_t> total(1);
+    DVec<int32_t> scratch(4);
+    const int64_t before = stats->live_nnz;
+    h2d(total.data(), &before, 1);
+    const bool ok = update_live_stats(*stats, *lp, *plan, old_keep_row, old_keep_col, touched_rows.data(),
+                                      touched_cols.data(), total.data(), scratch.data());
+    *accepted = ok ? 1 : 0;
+    if (ok) stats->live_nnz = read1(total.data());
+  });
+}
+
 int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp, const gp_stats* stats, const gp_params* params) {
   return guarded([&] {
     if (!plan || !lp || !stats || !params) throw std::invalid_argument("null argument");
@@ -807,6 +825,23 @@ int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp, const gp_
     plan->rule_cache = nullptr;
     apply_rule(rule, *plan, *lp, *stats, *params);
     GP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int gp_fixed_variable_tape(gp_plan* plan, const uint8_t* fixed, const uint8_t* excluded, const double* fixed_val,
+                           const uint8_t* keep_row, const double* c, const gp_csr* AT, int32_t n,
+                           int32_t compute_objective, double* objective) {
+  return guarded([&] {
+    if (objective) *objective = 0.0;
+    DeviceTape* tape = plan ? plan_tape(*plan) : nullptr;
+    if (n <= 0 || (!tape && !compute_objective)) return;
+    if (!fixed || !fixed_val || !keep_row || !c || !AT) throw std::invalid_argument("null argument");
+    if (AT->rows != n) throw std::invalid_argument("AT rows must equal n");
+    const double obj = fixed_variable_records(tape, fixed, fixed_val, keep_row, c, *AT, n, true,
+                                              compute_objective ? FixedObjective::Extended : FixedObjective::Skip,
+                                              excluded);
+    GP_CHECK(hipDeviceSynchronize());
+    if (objective) *objective = obj;
   });
 }
 
@@ -834,6 +869,15 @@ int gp_row_activity_summary(const gp_csr* A, const double* l, const double* u, d
   return guarded([&] {
     if (!A) throw std::invalid_argument("null matrix");
     row_activity_summary(*A, l, u, zero_tol, keep_row, keep_col, min_fin, max_fin, min_ninf, max_pinf);
+    GP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int gp_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, double zero_tol,
+                           uint64_t* hash) {
+  return guarded([&] {
+    if (!A || !keep_row || !hash) throw std::invalid_argument("null argument");
+    parallel_row_hashes(*A, keep_row, keep_col, zero_tol, hash);
     GP_CHECK(hipDeviceSynchronize());
   });
 }

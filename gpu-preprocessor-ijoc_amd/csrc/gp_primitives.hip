@@ -1491,7 +1491,9 @@ __global__ void k_inc_finalize(gp_csr A, gp_csr AT, const uint8_t* old_rows, con
                   s.singleton_column_row, s.singleton_column_val, invalid);
 }
 
+// a rejected update leaves the live total as it was
 __global__ void k_inc_total(int64_t* total, const unsigned long long* removed, int32_t* invalid) {
+  if (*invalid) return;
   if (static_cast<unsigned long long>(*total) < *removed) {
     *invalid = 1;
     return;

@@ -1109,7 +1109,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   double obj = 0.0;
   if (f2)
     obj = fixed_variable_records(tape_on(plan, p) ? plan_tape(plan) : nullptr, fixed.data(), fval.data(),
-                                 plan.keep_row_mask, plan.new_c, lp.AT, n, true);
+                                 plan.keep_row_mask, plan.new_c, lp.AT, n, true, FixedObjective::Extended);
   plan.obj_constant_delta += obj;
   plan.has_row_action = plan.has_row_action || f2;
   plan.has_col_action = plan.has_col_action || f2;

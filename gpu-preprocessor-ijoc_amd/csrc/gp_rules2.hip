@@ -1908,6 +1908,19 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
   plan.has_change = 1;
 }
 
+// compute_parallel_row_hashes (rule_parallel_rows.cu:583-627)
+void parallel_row_hashes(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, double tol,
+                         uint64_t* hash) {
+  const int32_t m = A.rows;
+  if (m <= 0) return;
+  const int use_xor = static_cast<int64_t>(A.nnz) > 32 * static_cast<int64_t>(m) ? 1 : 0;
+  DVec<int32_t> len(m), ids(m);
+  DVec<gp_offset> first(m);
+  first_live(A, LivePr{keep_col}, len.data(), first.data());
+  launch_line_stream(A, PrHash{keep_row, LivePr{keep_col}, len.data(), first.data(), A.vals, tol, use_xor, hash,
+                               ids.data()});
+}
+
 // ---------------------------------------------------------------- parallel rows
 void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
   if (plan.has_infeasible || plan.has_unbounded) return;

@@ -171,22 +171,27 @@ void tape_copy_to_host(const DeviceTape& t, int32_t* types, gp_offset* istarts, 
 
 // ---- FixedVariable records ---------------------------------------------------------------
 namespace {
-__global__ void k_fixed_counts(const uint8_t* fixed, const uint8_t* keep_row, gp_csr AT, int32_t n, int with_rows,
-                               int64_t* r, int64_t* ni, int64_t* nv) {
+// a column is recorded when fixed and not excluded (fixed_variable_tape.cu:43-48)
+__device__ inline bool fixed_selected(const uint8_t* fixed, const uint8_t* excluded, int64_t j) {
+  return fixed[j] != 0 && (!excluded || excluded[j] == 0);
+}
+__global__ void k_fixed_counts(const uint8_t* fixed, const uint8_t* excluded, const uint8_t* keep_row, gp_csr AT,
+                               int32_t n, int with_rows, int64_t* r, int64_t* ni, int64_t* nv) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
+  const bool sel = fixed_selected(fixed, excluded, j);
   int64_t k = 0;
-  if (fixed[j] && with_rows)
+  if (sel && with_rows)
     for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) k += keep_row[AT.col_idx[p]] != 0;
-  r[j] = fixed[j] ? 1 : 0;
-  ni[j] = fixed[j] ? 1 + k : 0;
-  nv[j] = fixed[j] ? 2 + k : 0;
+  r[j] = sel ? 1 : 0;
+  ni[j] = sel ? 1 + k : 0;
+  nv[j] = sel ? 2 + k : 0;
 }
-__global__ void k_fixed_pack(TapeView out, const uint8_t* fixed, const double* fval, const uint8_t* keep_row,
-                             const double* c, gp_csr AT, int32_t n, int with_rows, const int64_t* r,
-                             const int64_t* ni, const int64_t* nv) {
+__global__ void k_fixed_pack(TapeView out, const uint8_t* fixed, const uint8_t* excluded, const double* fval,
+                             const uint8_t* keep_row, const double* c, gp_csr AT, int32_t n, int with_rows,
+                             const int64_t* r, const int64_t* ni, const int64_t* nv) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= n || !fixed[j]) return;
+  if (j >= n || !fixed_selected(fixed, excluded, j)) return;
   const int64_t rec = out.rbase + r[j];
   int64_t ip = out.ibase + ni[j], vp = out.vbase + nv[j];
   out.types[rec] = GP_RED_FIXED_VARIABLE;
@@ -208,25 +213,49 @@ __global__ void k_fixed_contrib(const uint8_t* fixed, const double* fval, const 
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < n) out[j] = fixed[j] ? c[j] * fval[j] : 0.0;
 }
+__global__ void k_fixed_flags(const uint8_t* fixed, const uint8_t* excluded, int32_t n, int32_t* flag) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n) flag[j] = fixed_selected(fixed, excluded, j) ? 1 : 0;
+}
+__global__ void k_fixed_list(const uint8_t* fixed, const uint8_t* excluded, int32_t n, const int32_t* rank,
+                             int32_t* list) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n && fixed_selected(fixed, excluded, j)) list[rank[j]] = static_cast<int32_t>(j);
+}
 }  // namespace
 
 double fixed_variable_records(DeviceTape* tape, const uint8_t* fixed, const double* fval, const uint8_t* keep_row,
-                              const double* c, const gp_csr& AT, int32_t n, bool with_rows) {
+                              const double* c, const gp_csr& AT, int32_t n, bool with_rows, FixedObjective objective,
+                              const uint8_t* excluded) {
   if (n <= 0) return 0.0;
   if (tape) {
     AppendCounts cnt(n);
-    hipLaunchKernelGGL(k_fixed_counts, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, keep_row, AT, n,
+    hipLaunchKernelGGL(k_fixed_counts, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, excluded, keep_row, AT, n,
                        with_rows ? 1 : 0, cnt.r.data(), cnt.i.data(), cnt.v.data());
     GP_LAUNCH_CHECK("fixed counts");
     cnt.scan(n);
     if (cnt.R > 0) {
       tape->reserve_more(cnt.R, cnt.I, cnt.V);
       TapeView v = tape_view(*tape);
-      hipLaunchKernelGGL(k_fixed_pack, dim3(grid_for(n)), dim3(kBlock), 0, 0, v, fixed, fval, keep_row, c, AT, n,
-                         with_rows ? 1 : 0, cnt.r.data(), cnt.i.data(), cnt.v.data());
+      hipLaunchKernelGGL(k_fixed_pack, dim3(grid_for(n)), dim3(kBlock), 0, 0, v, fixed, excluded, fval, keep_row, c,
+                         AT, n, with_rows ? 1 : 0, cnt.r.data(), cnt.i.data(), cnt.v.data());
       GP_LAUNCH_CHECK("fixed pack");
       tape->commit(cnt.R, cnt.I, cnt.V);
     }
+  }
+  if (objective == FixedObjective::Skip) return 0.0;
+  if (objective == FixedObjective::Extended) {
+    // the selected columns in increasing order, then one sequential sum
+    DVec<int32_t> rank(static_cast<size_t>(n));
+    hipLaunchKernelGGL(k_fixed_flags, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, excluded, n, rank.data());
+    GP_LAUNCH_CHECK("fixed flags");
+    const int64_t count = exclusive_scan_i32_total(rank.data(), n);
+    if (count == 0) return 0.0;
+    DVec<int32_t> list(static_cast<size_t>(count));
+    hipLaunchKernelGGL(k_fixed_list, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, excluded, n, rank.data(),
+                       list.data());
+    GP_LAUNCH_CHECK("fixed list");
+    return obj_delta_x87(c, fval, list.data(), count);
   }
   DVec<double> contrib(static_cast<size_t>(n));
   hipLaunchKernelGGL(k_fixed_contrib, dim3(grid_for(n)), dim3(kBlock), 0, 0, fixed, fval, c, n, contrib.data());

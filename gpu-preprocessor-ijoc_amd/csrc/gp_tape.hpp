@@ -77,12 +77,19 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
 void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p);
 void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p);
 void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p);
+void parallel_row_hashes(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, double tol,
+                         uint64_t* hash);
 void apply_rule(int32_t rule, gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p);
 
-// FixedVariable records + objective for columns in `fixed` (rows limited to
-// keep_row); returns the objective delta sum c_j * v_j (fixed order).
+// FixedVariable records + objective for the columns in `fixed` and not in
+// `excluded` (rows limited to keep_row); returns the objective delta sum
+// c_j * v_j: Pairwise = the fixed-order det_sum, Extended = the reference's
+// sequential x87 long-double sum in column order (bound strengthening,
+// fixed_variable_tape.cu:305-325), Skip = 0.
+enum class FixedObjective { Pairwise, Extended, Skip };
 double fixed_variable_records(DeviceTape* tape, const uint8_t* fixed, const double* fval, const uint8_t* keep_row,
-                              const double* c, const gp_csr& AT, int32_t n, bool with_rows);
+                              const double* c, const gp_csr& AT, int32_t n, bool with_rows,
+                              FixedObjective objective = FixedObjective::Pairwise, const uint8_t* excluded = nullptr);
 
 // Read-only view of the opaque recovery record (defined in gp_presolve.hip).
 struct RecordView {

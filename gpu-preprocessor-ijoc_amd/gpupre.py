@@ -262,13 +262,18 @@ def load() -> C.CDLL:
         "gp_stats_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(GpStats)]),
         "gp_stats_destroy": (None, [C.POINTER(GpStats)]),
         "gp_refresh_live_stats": (C.c_int, [C.POINTER(GpStats), C.POINTER(GpDeviceLp), C.POINTER(GpPlan)]),
+        "gp_update_live_stats": (C.c_int, [C.POINTER(GpStats), C.POINTER(GpDeviceLp), C.POINTER(GpPlan), vp, vp,
+                                           C.POINTER(C.c_int32)]),
         "gp_rule_apply": (C.c_int, [C.c_int32, C.POINTER(GpPlan), C.POINTER(GpDeviceLp), C.POINTER(GpStats),
                                      C.POINTER(GpParams)]),
         "gp_plan_tape_sizes": (C.c_int, [C.POINTER(GpPlan), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64)]),
         "gp_plan_tape_copy": (C.c_int, [C.POINTER(GpPlan)] + [vp] * 6),
+        "gp_fixed_variable_tape": (C.c_int, [C.POINTER(GpPlan), vp, vp, vp, vp, vp, C.POINTER(GpCsr), C.c_int32,
+                                             C.c_int32, C.POINTER(C.c_double)]),
         "gp_row_activity_summary": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp, vp]),
         "gp_transpose_csr": (C.c_int, [C.POINTER(GpCsr), C.POINTER(GpCsr)]),
+        "gp_parallel_row_hashes": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp]),
         "gp_compact_csr": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.POINTER(GpCsr)]),
         "gp_csr_free": (None, [C.POINTER(GpCsr)]),
         "gp_device_count": (C.c_int, [C.POINTER(C.c_int32)]),

@@ -505,9 +505,29 @@ void gp_stats_destroy(gp_stats* stats);
 int gp_refresh_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan* plan);
 
 /* Apply one general rule (GP_RULE_*) to the plan. */
+/* update_live_stats_from_deletions (gpu_presolve_kernels.cu:520-682): bring
+ * statistics computed for the masks (old_keep_row, old_keep_col) to the
+ * plan's current masks by visiting only the rows and columns deleted since;
+ * the result equals gp_refresh_live_stats bit for bit.  *accepted = 0 (the
+ * statistics then need a full refresh, live_nnz is left as it was) when a row
+ * or column came back to life.  stats->live_nnz must hold the old total. */
+int gp_update_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan* plan,
+                         const uint8_t* old_keep_row, const uint8_t* old_keep_col, int32_t* accepted);
 int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp,
                   const gp_stats* stats, const gp_params* params);
 /* Export the plan tape (local indices). */
+/* append_compact_fixed_variable_tape_from_device (fixed_variable_tape.cu:141-327):
+ * FixedVariable records (idx [col, kept rows...], vals [v, c, coefficients...],
+ * Minimal dual mode) of every column with fixed[j] != 0 and, when `excluded`
+ * is given, excluded[j] == 0, in column order, appended to the plan's tape
+ * (no records when `plan` is NULL or has no tape).  With compute_objective
+ * the objective delta sum c_j * v_j is formed as the reference forms it: a
+ * sequential sum in x86 extended precision, rounded to binary64 once.
+ * Device arrays: fixed, excluded (n or NULL), fixed_val, c (n), keep_row
+ * (AT->cols); AT is the transpose with AT->rows == n. */
+int gp_fixed_variable_tape(gp_plan* plan, const uint8_t* fixed, const uint8_t* excluded,
+                           const double* fixed_val, const uint8_t* keep_row, const double* c,
+                           const gp_csr* AT, int32_t n, int32_t compute_objective, double* objective);
 int gp_plan_tape_sizes(const gp_plan* plan, int64_t* records, int64_t* indices, int64_t* values);
 int gp_plan_tape_copy(const gp_plan* plan, int32_t* types, gp_offset* index_starts,
                       gp_offset* value_starts, uint8_t* dual_modes, int32_t* indices,
@@ -516,6 +536,13 @@ int gp_plan_enable_tape(gp_plan* plan);
 
 /* Row activity summary (compute_row_activity_summary): finite min/max sums
  * (compensated, fixed order) and counts of infinite contributions. */
+/* compute_parallel_row_hashes (rule_parallel_rows.cu:141-196,583-627): per row
+ * the FNV-1a chain over (column, bits of a / first live a) of its live
+ * entries, XOR-aggregated per entry when nnz > 32 rows; `row` for rows not
+ * kept, mix(offset, row) for rows without live entries or with a pivot at or
+ * below zero_tol.  Device arrays; keep_col may be NULL (every column live). */
+int gp_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, double zero_tol,
+                           uint64_t* hash);
 int gp_row_activity_summary(const gp_csr* A, const double* l, const double* u, double zero_tol,
                             const uint8_t* keep_row, const uint8_t* keep_col,
                             double* min_fin, double* max_fin, int32_t* min_ninf, int32_t* max_pinf);

@@ -55,6 +55,14 @@ int or_plan_create(const gp_device_lp* lp, gp_plan* plan);
 void or_plan_destroy(gp_plan* plan);
 void or_plan_enable_tape(gp_plan* plan);
 
+/* FixedVariable records + long-double objective (fixed_variable_tape.cu:141-327). */
+double or_fixed_variable_tape(or_tape* t, const uint8_t* fixed, const uint8_t* excluded, const double* val,
+                              const uint8_t* keep_row, const double* c, const gp_csr* AT, int32_t n, int objective);
+
+/* compute_parallel_row_hashes (rule_parallel_rows.cu:141-196,583-627). */
+void or_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, double tol,
+                            uint64_t* out);
+
 /* Rules. */
 int or_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp, const gp_stats* stats,
                   const gp_params* params);

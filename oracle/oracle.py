@@ -63,6 +63,11 @@ def load():
                                         C.POINTER(G.GpStats), C.POINTER(G.GpParams)]),
             "or_row_activity_summary": (None, [C.POINTER(G.GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp, vp]),
             "or_det_sum": (C.c_double, [vp, C.c_int64]),
+            "or_tape_init": (None, [C.POINTER(OrTape)]),
+            "or_parallel_row_hashes": (None, [C.POINTER(G.GpCsr), vp, vp, C.c_double, vp]),
+            "or_tape_free": (None, [C.POINTER(OrTape)]),
+            "or_fixed_variable_tape": (C.c_double, [C.POINTER(OrTape), vp, vp, vp, vp, vp, C.POINTER(G.GpCsr),
+                                                    C.c_int32, C.c_int]),
             "or_row_sum_finite": (C.c_double, [vp, C.c_int64]),
             "or_transpose": (None, [C.POINTER(G.GpCsr), C.POINTER(G.GpCsr)]),
             "or_csr_free": (None, [C.POINTER(G.GpCsr)]),
@@ -181,6 +186,43 @@ def postsolve_record(rec: "G.HostRecord", original: "G.GpDeviceLp | None", x, y,
     if rc != 0:
         return None
     return xo, yo, zo
+
+
+def fixed_variable_tape(AT, fixed, excluded, values, keep_row, costs, with_tape=True, objective=True):
+    """or_fixed_variable_tape on host arrays; AT = (rows, cols, row_ptr,
+    col_idx, vals).  Returns (objective, tape dict or None)."""
+    rows, cols, rp, ci, va = AT
+    keep = [np.ascontiguousarray(a, dt) for a, dt in ((rp, G.NP_OFFSET), (ci, np.int32), (va, np.float64))]
+    at = G.GpCsr(rows, cols, len(keep[1]), _p(keep[0]), _p(keep[1]), _p(keep[2]))
+    arrs = [None if a is None else np.ascontiguousarray(a, dt)
+            for a, dt in ((fixed, np.uint8), (excluded, np.uint8), (values, np.float64), (keep_row, np.uint8),
+                          (costs, np.float64))]
+    t = OrTape()
+    load().or_tape_init(C.byref(t))
+    try:
+        obj = load().or_fixed_variable_tape(C.byref(t) if with_tape else None,
+                                            *[_p(a) if a is not None else None for a in arrs], C.byref(at), rows,
+                                            1 if objective else 0)
+        tape = None
+        if with_tape:
+            tape = dict(types=_arr(t.types, t.count, np.int32), index_starts=_arr(t.istarts, t.count + 1, G.NP_OFFSET),
+                        value_starts=_arr(t.vstarts, t.count + 1, G.NP_OFFSET),
+                        dual_modes=_arr(t.modes, t.count, np.uint8), indices=_arr(t.idx, t.icount, np.int32),
+                        vals=_arr(t.vals, t.vcount, np.float64))
+        return obj, tape
+    finally:
+        load().or_tape_free(C.byref(t))
+
+
+def parallel_row_hashes(m, n, row_ptr, col_idx, vals, keep_row, keep_col=None, tol=1e-12):
+    keep = [np.ascontiguousarray(a, dt) for a, dt in ((row_ptr, G.NP_OFFSET), (col_idx, np.int32),
+                                                     (vals, np.float64))]
+    A = G.GpCsr(m, n, len(keep[1]), _p(keep[0]), _p(keep[1]), _p(keep[2]))
+    kr = np.ascontiguousarray(keep_row, np.uint8)
+    kc = np.ascontiguousarray(keep_col, np.uint8) if keep_col is not None else None
+    out = np.empty(m, np.uint64)
+    load().or_parallel_row_hashes(C.byref(A), _p(kr), _p(kc) if kc is not None else None, float(tol), _p(out))
+    return out
 
 
 def det_sum(x) -> float:

@@ -70,6 +70,12 @@ static inline void or_cand_min(double* slot, double v) {
   if (or_total_greater(*slot, v)) *slot = v;
 }
 
+/* Objective of fixed columns: sequential long-double (x87 extended) sum of
+ * c[j] * v[j] over selected (fixed and not excluded) columns in increasing
+ * order, rounded to double once (fixed_variable_tape.cu:305-325). */
+double or_fixed_objective_ld(const uint8_t* fixed, const uint8_t* excluded, const double* c, const double* v,
+                             int32_t n);
+
 void* or_xmalloc(size_t bytes);
 void* or_xcalloc(size_t count, size_t size);
 void* or_xrealloc(void* p, size_t bytes);

@@ -653,11 +653,9 @@ int or_rule_bound_strengthening(gp_plan* plan, const gp_device_lp* lp, const gp_
     }
     double obj = 0.0;
     if (f2) {
-      double* contrib = (double*)or_xcalloc((size_t)n, sizeof(double));
-      for (int32_t j = 0; j < n; ++j)
-        if (fixed[j]) contrib[j] = plan->new_c[j] * fval[j];
-      obj = or_det_sum(contrib, n);
-      free(contrib);
+      /* sequential long-double sum in column order (rule_bound_strengthening.cu:1537-1548,
+       * fixed_variable_tape.cu:305-325) */
+      obj = or_fixed_objective_ld(fixed, NULL, plan->new_c, fval, n);
       if (tape_on(plan, p)) {
         or_tape* t = (or_tape*)plan->tape;
         for (int32_t j = 0; j < n; ++j) {

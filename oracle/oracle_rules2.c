@@ -20,6 +20,45 @@ static int tape_on(const gp_plan* plan, const gp_params* p) {
   return plan->tape != NULL && p->record_postsolve_tape;
 }
 
+double or_fixed_objective_ld(const uint8_t* fixed, const uint8_t* excluded, const double* c, const double* v,
+                             int32_t n) {
+  long double acc = 0.0L;
+  for (int32_t j = 0; j < n; ++j)
+    if (fixed[j] && (!excluded || !excluded[j])) acc += (long double)c[j] * (long double)v[j];
+  return (double)acc;
+}
+
+/* append_compact_fixed_variable_tape_from_device (fixed_variable_tape.cu:141-327):
+ * FixedVariable records of the selected columns in column order, idx [col,
+ * kept rows...], vals [v, c, coeffs...]; returns the long-double objective
+ * when objective != 0, else 0. */
+double or_fixed_variable_tape(or_tape* t, const uint8_t* fixed, const uint8_t* excluded, const double* val,
+                              const uint8_t* keep_row, const double* c, const gp_csr* AT, int32_t n, int objective) {
+  if (n <= 0 || (!t && !objective)) return 0.0;
+  if (t) {
+    for (int32_t j = 0; j < n; ++j) {
+      if (!fixed[j] || (excluded && excluded[j])) continue;
+      const gp_offset a0 = AT->row_ptr[j], a1 = AT->row_ptr[j + 1];
+      int32_t* ix = (int32_t*)or_xmalloc(sizeof(int32_t) * (size_t)(a1 - a0 + 1));
+      double* vx = (double*)or_xmalloc(sizeof(double) * (size_t)(a1 - a0 + 2));
+      int64_t ni = 0, nv = 0;
+      ix[ni++] = j;
+      vx[nv++] = val[j];
+      vx[nv++] = c[j];
+      for (gp_offset q = a0; q < a1; ++q) {
+        const int32_t r = AT->col_idx[q];
+        if (!keep_row[r]) continue;
+        ix[ni++] = r;
+        vx[nv++] = AT->vals[q];
+      }
+      or_tape_push(t, GP_RED_FIXED_VARIABLE, GP_DUAL_MINIMAL, ix, ni, vx, nv);
+      free(ix);
+      free(vx);
+    }
+  }
+  return objective ? or_fixed_objective_ld(fixed, excluded, c, val, n) : 0.0;
+}
+
 /* FixedVariable records with their kept-row payload, in column order
  * (fixed_variable_tape.cu:36-120): idx [col, rows...], vals [v, c, coeffs...]. */
 static void push_fixed_with_rows(or_tape* t, const uint8_t* sel, const double* val,
@@ -1019,6 +1058,13 @@ static uint64_t row_hash(const gp_csr* A, int32_t i, const uint8_t* keep_row, co
     }
   }
   return use_xor ? fnv_mix(h, agg) : h;
+}
+
+/* compute_parallel_row_hashes (rule_parallel_rows.cu:583-627): one hash per row. */
+void or_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, double tol,
+                            uint64_t* out) {
+  const int use_xor = (int64_t)A->nnz > 32 * (int64_t)A->rows;
+  for (int32_t i = 0; i < A->rows; ++i) out[i] = row_hash(A, i, keep_row, keep_col, tol, use_xor);
 }
 
 /* _parallel_row_ratio (:312-346): ratio = a_i / a_k on the first live pair,
