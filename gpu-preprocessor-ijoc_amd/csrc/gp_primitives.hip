@@ -1620,3 +1620,70 @@ void row_activity_summary(const gp_csr& A, const double* l, const double* u, dou
 }
 
 }  // namespace gp
+
+// ---- row activity summary without allocation ---------------------------------------------
+// For callers that cannot allocate device memory (code under stream capture,
+// tight serving loops).  The two scratch buffers of the activity sweep -- the
+// interleaved (l, u) pairs and the packed column-mask bits -- live in one
+// file-scope device array of fixed size instead of pool blocks.  Every call on
+// a device shares that device's array, so calls that overlap on one device
+// race; each call rewrites every scratch word it reads.
+namespace gp {
+namespace {
+constexpr int64_t kNoAllocMaxCols = int64_t{1} << 21;  // 2,097,152 columns
+constexpr size_t kScratchBoundsBytes = static_cast<size_t>(kNoAllocMaxCols) * sizeof(double2);
+constexpr size_t kScratchBytes = kScratchBoundsBytes + static_cast<size_t>(kNoAllocMaxCols / 32) * sizeof(uint32_t);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device
+// with that device current.  The host-side name of the array is not a device
+// address; only this pointer is passed to kernels.
+unsigned char* scratch_on_current_device() {
+  int dev = 0;
+  GP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(GP_ERR_RUNTIME, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    GP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(GP_ERR_RUNTIME, "no device address for the scratch array");
+    g_scratch_addr[dev] = p;
+  }
+  return static_cast<unsigned char*>(g_scratch_addr[dev]);
+}
+}  // namespace
+}  // namespace gp
+
+extern "C" int gp_row_activity_summary_noalloc(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                                               const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
+                                               double* max_fin, int32_t* min_ninf, int32_t* max_pinf) {
+  using namespace gp;
+  return guarded([&] {
+    if (!A || !l || !u || !keep_row || !min_fin || !max_fin || !min_ninf || !max_pinf)
+      throw std::invalid_argument("null argument");
+    if (A->rows < 0 || A->cols < 0) throw std::invalid_argument("negative matrix dimension");
+    if (A->cols > kNoAllocMaxCols)
+      throw std::invalid_argument("gp_row_activity_summary_noalloc: more than 2097152 columns (use "
+                                  "gp_row_activity_summary)");
+    if (A->rows == 0) return;
+    unsigned char* base = scratch_on_current_device();
+    double2* lu = reinterpret_cast<double2*>(base);
+    uint32_t* words = reinterpret_cast<uint32_t*>(base + kScratchBoundsBytes);
+    const int32_t n = A->cols;
+    if (n > 0) {
+      hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu);
+      if (keep_col)
+        hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(((static_cast<int64_t>(n) + 31) / 32) * 32)), dim3(kBlock), 0, 0,
+                           keep_col, static_cast<int64_t>(n), words);
+      GP_LAUNCH_CHECK("pack activity scratch");
+    }
+    launch_row_fold(*A, lu, Bits{keep_col ? words : nullptr},
+                    ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf});
+    GP_LAUNCH_CHECK("activity summary (no allocation)");
+    GP_CHECK(hipDeviceSynchronize());
+  });
+}

@@ -272,6 +272,8 @@ def load() -> C.CDLL:
         "gp_fixed_variable_tape": (C.c_int, [C.POINTER(GpPlan), vp, vp, vp, vp, vp, C.POINTER(GpCsr), C.c_int32,
                                              C.c_int32, C.POINTER(C.c_double)]),
         "gp_row_activity_summary": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp, vp]),
+        "gp_row_activity_summary_noalloc": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp,
+                                                      vp]),
         "gp_transpose_csr": (C.c_int, [C.POINTER(GpCsr), C.POINTER(GpCsr)]),
         "gp_parallel_row_hashes": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp]),
         "gp_compact_csr": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.POINTER(GpCsr)]),

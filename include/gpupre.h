@@ -536,6 +536,18 @@ int gp_plan_enable_tape(gp_plan* plan);
 
 /* Row activity summary (compute_row_activity_summary): finite min/max sums
  * (compensated, fixed order) and counts of infinite contributions. */
+/* gp_row_activity_summary without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The sweep's scratch (interleaved column bounds and packed column-mask bits)
+ * is a fixed device array of the library, 32.25 MiB per device, so A->cols is
+ * limited to 2097152 (GP_ERR_INVALID_ARGUMENT beyond).  Calls that overlap on
+ * one device share that array and race.  Same arguments and results as
+ * gp_row_activity_summary. */
+int gp_row_activity_summary_noalloc(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                                    const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
+                                    double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
+
 /* compute_parallel_row_hashes (rule_parallel_rows.cu:141-196,583-627): per row
  * the FNV-1a chain over (column, bits of a / first live a) of its live
  * entries, XOR-aggregated per entry when nnz > 32 rows; `row` for rows not
