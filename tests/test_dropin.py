@@ -1,7 +1,9 @@
 """The C++ front end with the reference's API names (include/gpu_preprocessor/
 gpu_preprocessor.hpp): it compiles as plain host C++ (layout checks against
-the C ABI are static_asserts), and on a GPU the reference's own toy example
-(examples/presolve_example.cpp) runs unchanged in logic against it."""
+the C ABI are static_asserts), and on a GPU the example caller
+(examples/presolve_example.cpp) runs preprocess / postprocess / free_summary
+on an LP that presolve solves completely and checks the recovered point's
+KKT conditions."""
 import os
 import subprocess
 
@@ -30,10 +32,11 @@ def test_shim_compiles_as_host_cpp(tmp_path):
 
 
 @pytest.mark.gpu
-def test_reference_example_runs():
+def test_example_caller_runs():
     exe = os.path.join(PKG, "presolve_example")
     assert os.path.exists(exe), "build the package first"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
-    assert "original primal: 1, 1; objective: 2" in r.stdout
+    assert "x = (2, 3, 1, 1.5), y = (1, -2, 0.5), objective 1: KKT conditions hold" in r.stdout
+    assert "unchanged run borrows the input LP" in r.stdout
