@@ -637,6 +637,36 @@ void gp_default_params(gp_params* p) {
   p->parallel_columns_scale_guard_max_group_size = 4;
   p->parallel_columns_scale_guard_min_groups = 128;
   p->enable_structure_specific_rules = 1;
+  p->enable_two_row_variable_elimination = 1;
+  p->two_row_variable_elimination_max_row_nnz = 8;
+  p->two_row_variable_elimination_safety_tol = 1.0e-8;
+  p->two_row_variable_elimination_min_candidates = 128;
+  p->two_row_variable_elimination_min_candidate_ratio = 0.01;
+  p->enable_zero_cost_product_variables = 1;
+  p->zero_cost_product_variables_min_candidates = 128;
+  p->zero_cost_product_variables_min_candidate_ratio = 0.01;
+  p->enable_piecewise_linear_reformulation_variable_pair = 1;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_probe_pairs = 256;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_min_pairs = 4096;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_min_potential_gain = 0.20;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_min_actual_gain = 0.20;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_max_tail_cols = 8;
+  p->piecewise_linear_reformulation_variable_pair_epigraph_max_quotient_terms_per_row = 5;
+  p->piecewise_linear_reformulation_variable_pair_shifted_antipodal_probe_pairs = 256;
+  p->piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_pairs = 4096;
+  p->piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_potential_gain = 0.20;
+  p->piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_actual_gain = 0.20;
+  p->piecewise_linear_reformulation_variable_pair_shifted_antipodal_max_quotient_terms_per_row = 6;
+  p->enable_column_domination = 1;
+  p->column_domination_probe_rows = 64;
+  p->column_domination_probe_cols = 256;
+  p->column_domination_probe_nnz = 256;
+  p->column_domination_min_fixed_cols = 4096;
+  p->column_domination_min_candidate_ratio = 0.20;
+  p->enable_piecewise_linear_reformulation_absolute_value = 1;
+  p->piecewise_linear_reformulation_absolute_value_residual_bound_mode = 0;
+  p->piecewise_linear_reformulation_absolute_value_residual_bound_as_free_min = 1.0e8;
+  p->enable_redundant_variable_bounds = 0;
   p->enable_startup = 1;
   p->bound_strengthening_bound_only_nnz_round_budget = 8000000;
   p->incremental_live_stats = 1;
@@ -662,6 +692,16 @@ int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary
     out->original_cols = input->A.cols;
     out->presolve_enabled = params->enable_presolve;
     out->folding_applied = 0;
+    // Requested stages this engine does not run: folding (folding.cu:185-209)
+    // and the structure-specific Startup rules (gpu_presolve.cu:2088-2094,
+    // 2361-2454).  The reduction is the one with those stages off.
+    if (params->enable_folding) out->unsupported_stages |= GP_STAGE_FOLDING;
+    if (params->enable_presolve && params->enable_startup && params->max_iters > 0 &&
+        params->enable_structure_specific_rules &&
+        (params->enable_column_domination || params->enable_two_row_variable_elimination ||
+         params->enable_zero_cost_product_variables || params->enable_piecewise_linear_reformulation_variable_pair ||
+         params->enable_piecewise_linear_reformulation_absolute_value))
+      out->unsupported_stages |= GP_STAGE_STRUCTURE_SPECIFIC_RULES;
     const double setup_end = now_s();
     out->workspace_setup_seconds = setup_end - t0;
     try {

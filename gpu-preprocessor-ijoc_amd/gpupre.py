@@ -87,7 +87,38 @@ class GpParams(C.Structure):
         ("enable_parallel_columns", C.c_int32), ("parallel_columns_max_abs_ratio", C.c_double),
         ("parallel_columns_scale_guard_max_group_size", C.c_int32),
         ("parallel_columns_scale_guard_min_groups", C.c_int32),
-        ("enable_structure_specific_rules", C.c_int32), ("enable_startup", C.c_int32),
+        ("enable_structure_specific_rules", C.c_int32),
+        ("enable_two_row_variable_elimination", C.c_int32),
+        ("two_row_variable_elimination_max_row_nnz", C.c_int32),
+        ("two_row_variable_elimination_safety_tol", C.c_double),
+        ("two_row_variable_elimination_min_candidates", C.c_int32),
+        ("two_row_variable_elimination_min_candidate_ratio", C.c_double),
+        ("enable_zero_cost_product_variables", C.c_int32),
+        ("zero_cost_product_variables_min_candidates", C.c_int32),
+        ("zero_cost_product_variables_min_candidate_ratio", C.c_double),
+        ("enable_piecewise_linear_reformulation_variable_pair", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_probe_pairs", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_min_pairs", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_min_potential_gain", C.c_double),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_min_actual_gain", C.c_double),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_max_tail_cols", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_epigraph_max_quotient_terms_per_row", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_shifted_antipodal_probe_pairs", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_pairs", C.c_int32),
+        ("piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_potential_gain", C.c_double),
+        ("piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_actual_gain", C.c_double),
+        ("piecewise_linear_reformulation_variable_pair_shifted_antipodal_max_quotient_terms_per_row", C.c_int32),
+        ("enable_column_domination", C.c_int32),
+        ("column_domination_probe_rows", C.c_int32),
+        ("column_domination_probe_cols", C.c_int32),
+        ("column_domination_probe_nnz", C.c_int32),
+        ("column_domination_min_fixed_cols", C.c_int32),
+        ("column_domination_min_candidate_ratio", C.c_double),
+        ("enable_piecewise_linear_reformulation_absolute_value", C.c_int32),
+        ("piecewise_linear_reformulation_absolute_value_residual_bound_mode", C.c_int32),
+        ("piecewise_linear_reformulation_absolute_value_residual_bound_as_free_min", C.c_double),
+        ("enable_redundant_variable_bounds", C.c_int32),
+        ("enable_startup", C.c_int32),
         ("bound_strengthening_bound_only_nnz_round_budget", C.c_int64),
         ("collect_trace", C.c_int32), ("learned_schedule", C.c_int32),
         ("scheduler_weights", C.c_void_p), ("kernel_timing", C.c_int32),
@@ -123,7 +154,7 @@ class GpSummary(C.Structure):
         ("has_unbounded", C.c_int32), ("reduced_lp", GpDeviceLp), ("owns_reduced_lp", C.c_int32),
         ("record", C.c_void_p), ("trace_count", C.c_int32), ("trace", C.POINTER(GpRoundTrace)),
         ("kernel_ms", C.c_double * GP_KT_COUNT), ("kernel_launches", C.c_int64 * GP_KT_COUNT),
-        ("kernel_bytes", C.c_double * GP_KT_COUNT),
+        ("kernel_bytes", C.c_double * GP_KT_COUNT), ("unsupported_stages", C.c_int32),
     ]
 
 

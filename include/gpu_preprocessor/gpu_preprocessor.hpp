@@ -21,7 +21,9 @@
 //  * Errors: std::invalid_argument, std::length_error, std::runtime_error as in
 //    the reference; infeasible / unbounded are flags, not errors.
 //  * Folding (enable_folding) and the structure-specific Startup rules are not
-//    part of this engine: the fields are accepted and ignored.
+//    part of this engine: the fields are accepted, the reduction is the one
+//    with those stages off, and Summary::unsupported_stages reports a request
+//    for them (GP_STAGE_* bits); folding.applied stays false.
 #pragma once
 
 #include <cstddef>
@@ -86,6 +88,56 @@ struct PresolveParams : gp_params {
   PresolveParams() : gp_params() { gp_default_params(this); }
 };
 
+// PostsolveTape (presolve_structs.hpp:317-332), host copy.
+struct PostsolveTape {
+  std::vector<std::int32_t> types;
+  std::vector<NnzOffset> index_starts{0};
+  std::vector<NnzOffset> value_starts{0};
+  std::vector<std::uint8_t> dual_modes;
+  std::vector<std::int32_t> indices;
+  std::vector<double> vals;
+};
+
+// PresolveRecordGpu (presolve_structs.hpp:338-368): dimensions and objective
+// constants always; the maps, fixed columns and tape stay on the device until
+// export_recovery_record() copies them into the vectors.
+struct PresolveRecordGpu {
+  const gp_summary* device = nullptr;  // the owning summary's device record
+  std::int32_t m0 = 0;
+  std::int32_t n0 = 0;
+  std::int32_t m1 = 0;
+  std::int32_t n1 = 0;
+  std::vector<std::int32_t> row_red2org;
+  std::vector<std::int32_t> col_red2org;
+  std::vector<std::int32_t> fixed_idx;
+  std::vector<double> fixed_val;
+  double obj_constant_old = 0.0;
+  double obj_constant_new = 0.0;
+  PostsolveTape tape;
+};
+
+// export_recovery_record (device_recovery.hpp): device record -> host vectors.
+inline void export_recovery_record(PresolveRecordGpu& record) {
+  if (!record.device) return;
+  gp_record_sizes sz{};
+  detail::raise(gp_record_get_sizes(record.device, &sz));
+  record.row_red2org.resize(static_cast<std::size_t>(sz.m1));
+  record.col_red2org.resize(static_cast<std::size_t>(sz.n1));
+  record.fixed_idx.resize(static_cast<std::size_t>(sz.fixed_count));
+  record.fixed_val.resize(static_cast<std::size_t>(sz.fixed_count));
+  PostsolveTape& t = record.tape;
+  t.types.resize(static_cast<std::size_t>(sz.record_count));
+  t.index_starts.resize(static_cast<std::size_t>(sz.record_count + 1));
+  t.value_starts.resize(static_cast<std::size_t>(sz.record_count + 1));
+  t.dual_modes.resize(static_cast<std::size_t>(sz.record_count));
+  t.indices.resize(static_cast<std::size_t>(sz.index_count));
+  t.vals.resize(static_cast<std::size_t>(sz.value_count));
+  detail::raise(gp_record_copy(record.device, record.row_red2org.data(), record.col_red2org.data(),
+                               record.fixed_idx.data(), record.fixed_val.data(), t.types.data(),
+                               t.index_starts.data(), t.value_starts.data(), t.dual_modes.data(), t.indices.data(),
+                               t.vals.data()));
+}
+
 // GpuPresolveSummary (gpu_presolve.hpp:14-30), a view of the owning record.
 struct GpuPresolveSummary {
   std::int32_t original_rows = 0;
@@ -100,7 +152,7 @@ struct GpuPresolveSummary {
   double obj_constant_delta = 0.0;
   bool has_infeasible = false;
   bool has_unbounded = false;
-  const void* record = nullptr;  // opaque, device resident (gp_record_copy exports it)
+  PresolveRecordGpu record;
   LPInfoGpu reduced_lp;
   bool owns_reduced_lp = false;
 };
@@ -108,6 +160,77 @@ struct GpuPresolveSummary {
 }  // namespace presolve
 
 namespace folding {
+
+// Folding profile and run summary (folding_structs.hpp:15-92).  This engine
+// does not fold: applied stays false, the folded dimensions equal the
+// original ones and every timer is zero.
+struct FoldingRefinementProfile {
+  double total_seconds = 0.0;
+  double init_color_seconds = 0.0;
+  double spmv_at_seconds = 0.0;
+  double refine_col_seconds = 0.0;
+  double spmv_a_seconds = 0.0;
+  double refine_row_seconds = 0.0;
+  std::int32_t rounds = 0;
+  bool refinement_ok = false;
+  std::int32_t final_row_colors = 0;
+  std::int32_t final_col_colors = 0;
+};
+
+struct FoldingReduceProfile {
+  double total_seconds = 0.0;
+  double count_color_seconds = 0.0;
+  double emit_triplet_seconds = 0.0;
+  double sort_reduce_seconds = 0.0;
+  double copy_unique_to_host_seconds = 0.0;
+  double host_filter_rebuild_seconds = 0.0;
+  double host_csr_build_seconds = 0.0;
+  double copy_reduced_matrix_to_device_seconds = 0.0;
+  double transpose_seconds = 0.0;
+  double sum_cost_seconds = 0.0;
+  double row_bounds_seconds = 0.0;
+  double col_bounds_seconds = 0.0;
+  NnzOffset unique_entries = 0;
+  NnzOffset kept_entries = 0;
+};
+
+struct FoldingMapProfile {
+  double total_seconds = 0.0;
+  double copy_color_ids_seconds = 0.0;
+  double count_color_seconds = 0.0;
+  double scale_seconds = 0.0;
+};
+
+struct FoldingProfile {
+  double total_seconds = 0.0;
+  FoldingRefinementProfile refinement;
+  FoldingReduceProfile reduce;
+  FoldingMapProfile map;
+};
+
+struct FoldingMapDevice {
+  std::int32_t original_rows = 0;
+  std::int32_t original_cols = 0;
+  std::int32_t reduced_rows = 0;
+  std::int32_t reduced_cols = 0;
+  std::int32_t* row_color_id = nullptr;
+  std::int32_t* col_color_id = nullptr;
+  double* row_scale = nullptr;
+  double* col_scale = nullptr;
+  bool owns_buffers = false;
+};
+
+struct FoldingRunSummary {
+  bool applied = false;
+  std::int32_t original_rows = 0;
+  std::int32_t original_cols = 0;
+  std::int32_t folded_rows = 0;
+  std::int32_t folded_cols = 0;
+  FoldingProfile profile;
+  FoldingMapDevice map;
+  presolve::LPInfoGpu folded_lp;
+  bool owns_folded_lp = false;
+};
 
 // FoldingPipelineSummary (folding_structs.hpp:94-109).  Move-only: it owns
 // the reduced LP and the recovery record until free_summary().
@@ -121,7 +244,11 @@ struct FoldingPipelineSummary {
   std::int32_t original_cols = 0;
   std::int32_t folded_rows = 0;
   std::int32_t folded_cols = 0;
+  FoldingRunSummary folding;
   presolve::GpuPresolveSummary presolve;
+  // GP_STAGE_* bits of requested stages this engine does not run (no
+  // reference counterpart: the reference runs them).
+  std::int32_t unsupported_stages = 0;
 
   FoldingPipelineSummary() : raw_() {}
   FoldingPipelineSummary(const FoldingPipelineSummary&) = delete;
@@ -160,9 +287,18 @@ struct FoldingPipelineSummary {
     presolve.obj_constant_delta = raw_.obj_constant_delta;
     presolve.has_infeasible = raw_.has_infeasible != 0;
     presolve.has_unbounded = raw_.has_unbounded != 0;
-    presolve.record = raw_.record;
+    presolve.record.device = raw_.record ? &raw_ : nullptr;
+    presolve.record.m0 = raw_.original_rows;
+    presolve.record.n0 = raw_.original_cols;
+    presolve.record.m1 = raw_.reduced_rows;
+    presolve.record.n1 = raw_.reduced_cols;
+    presolve.record.obj_constant_new = raw_.reduced_lp.obj_constant;
     presolve.reduced_lp = *reinterpret_cast<const presolve::LPInfoGpu*>(&raw_.reduced_lp);
     presolve.owns_reduced_lp = raw_.owns_reduced_lp != 0;
+    folding = FoldingRunSummary{};
+    folding.original_rows = folding.folded_rows = raw_.original_rows;
+    folding.original_cols = folding.folded_cols = raw_.original_cols;
+    unsupported_stages = raw_.unsupported_stages;
   }
 
  private:
@@ -177,8 +313,12 @@ struct FoldingPipelineSummary {
     original_cols = o.original_cols;
     folded_rows = o.folded_rows;
     folded_cols = o.folded_cols;
-    presolve = o.presolve;
+    folding = o.folding;
+    presolve = std::move(o.presolve);
+    unsupported_stages = o.unsupported_stages;
     raw_ = o.raw_;
+    // the record view follows the summary it belongs to
+    if (presolve.record.device) presolve.record.device = &raw_;
     o.raw_ = gp_summary();
     o.presolve = presolve::GpuPresolveSummary();
   }
@@ -203,6 +343,7 @@ inline Summary preprocess(const DeviceLp& input, const Parameters& params = {}) 
   Summary s;
   detail::raise(gp_preprocess(reinterpret_cast<const gp_device_lp*>(&input), &params, s.raw_ptr()));
   s.sync();
+  s.presolve.record.obj_constant_old = input.obj_constant;
   return s;
 }
 

@@ -160,6 +160,38 @@ typedef struct gp_params {
   int32_t parallel_columns_scale_guard_max_group_size;
   int32_t parallel_columns_scale_guard_min_groups;
   int32_t enable_structure_specific_rules;
+  /* Startup-rule settings (presolve_structs.hpp:103-146): accepted, not built;
+   * gp_summary.unsupported_stages reports a request for these rules. */
+  int32_t enable_two_row_variable_elimination;
+  int32_t two_row_variable_elimination_max_row_nnz;
+  double two_row_variable_elimination_safety_tol;
+  int32_t two_row_variable_elimination_min_candidates;
+  double two_row_variable_elimination_min_candidate_ratio;
+  int32_t enable_zero_cost_product_variables;
+  int32_t zero_cost_product_variables_min_candidates;
+  double zero_cost_product_variables_min_candidate_ratio;
+  int32_t enable_piecewise_linear_reformulation_variable_pair;
+  int32_t piecewise_linear_reformulation_variable_pair_epigraph_probe_pairs;
+  int32_t piecewise_linear_reformulation_variable_pair_epigraph_min_pairs;
+  double piecewise_linear_reformulation_variable_pair_epigraph_min_potential_gain;
+  double piecewise_linear_reformulation_variable_pair_epigraph_min_actual_gain;
+  int32_t piecewise_linear_reformulation_variable_pair_epigraph_max_tail_cols;
+  int32_t piecewise_linear_reformulation_variable_pair_epigraph_max_quotient_terms_per_row;
+  int32_t piecewise_linear_reformulation_variable_pair_shifted_antipodal_probe_pairs;
+  int32_t piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_pairs;
+  double piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_potential_gain;
+  double piecewise_linear_reformulation_variable_pair_shifted_antipodal_min_actual_gain;
+  int32_t piecewise_linear_reformulation_variable_pair_shifted_antipodal_max_quotient_terms_per_row;
+  int32_t enable_column_domination;
+  int32_t column_domination_probe_rows;
+  int32_t column_domination_probe_cols;
+  int32_t column_domination_probe_nnz;
+  int32_t column_domination_min_fixed_cols;
+  double column_domination_min_candidate_ratio;
+  int32_t enable_piecewise_linear_reformulation_absolute_value;
+  int32_t piecewise_linear_reformulation_absolute_value_residual_bound_mode;
+  double piecewise_linear_reformulation_absolute_value_residual_bound_as_free_min;
+  int32_t enable_redundant_variable_bounds;
   int32_t enable_startup;
   int64_t bound_strengthening_bound_only_nnz_round_budget;
   /* Engine extensions (no reference counterpart). */
@@ -234,6 +266,13 @@ enum {
   GP_GROUP_PARALLEL = 6          /* parallel rows, parallel columns            */
 };
 
+/* Stages of the reference's preprocess() this engine does not build. */
+enum {
+  GP_STAGE_FOLDING = 1,                  /* enable_folding (folding/folding.cu)       */
+  GP_STAGE_STRUCTURE_SPECIFIC_RULES = 2  /* enable_structure_specific_rules with one of
+                                            its sub-rules on (gpu_presolve.cu:2361-2454) */
+};
+
 /* FoldingPipelineSummary + GpuPresolveSummary, flattened.  Value semantics:
  * gp_preprocess fills a caller-provided struct; gp_free_summary releases its
  * device memory and zeroes it (idempotent). */
@@ -263,6 +302,9 @@ typedef struct gp_summary {
   double kernel_ms[GP_KT_COUNT];
   int64_t kernel_launches[GP_KT_COUNT];
   double kernel_bytes[GP_KT_COUNT]; /* algorithmic bytes per family */
+  int32_t unsupported_stages; /* GP_STAGE_* bits: stages the parameters requested
+                                 that this engine does not run (the reduction
+                                 is then the one with those stages off) */
 } gp_summary;
 
 /* Batched presolve (no reference counterpart; the reference presolves one LP

@@ -31,6 +31,65 @@ def test_shim_compiles_as_host_cpp(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+ADAPTER = r"""
+// A solver adapter written against the drop-in names the way the reference's
+// callers use them (solver_interface/adapters/gpu.cpp:700-780,
+// cpp/tools/gpu_preprocessor_mps.cpp:30-62): parameters set field by field,
+// preprocess, folding and presolve timers and dimensions, the record's
+// objective constant and exported fixed columns, postprocess, free_summary.
+#include <cmath>
+#include <iostream>
+#include "gpu_preprocessor/gpu_preprocessor.hpp"
+
+namespace gp = gpu_preprocessor;
+
+struct Handle {
+  gp::DeviceLp original_lp;
+  gp::Summary folding_summary;
+  double presolve_seconds = 0.0;
+};
+
+bool run_adapter(Handle& h, bool folding, bool presolve, bool new_rules, double time_limit) {
+  gp::Parameters p;
+  p.verbose = false;
+  p.enable_folding = folding;
+  p.enable_presolve = presolve;
+  p.enable_structure_specific_rules = new_rules;
+  if (std::isfinite(time_limit) && time_limit > 0.0) p.max_time = time_limit;
+  h.folding_summary = gp::preprocess(h.original_lp, p);
+  gp::presolve::GpuPresolveSummary& s = h.folding_summary.presolve;
+  h.presolve_seconds = s.elapsed_seconds;
+  std::cout << h.folding_summary.folding.profile.total_seconds << " "
+            << h.folding_summary.original_rows << " " << h.folding_summary.folded_cols << " "
+            << h.folding_summary.folding.applied << " " << h.folding_summary.folding.owns_folded_lp << " "
+            << s.reduced_rows << " " << s.reduced_cols << " " << s.record.obj_constant_new << "\n";
+  if (s.has_infeasible || s.has_unbounded) return false;
+  gp::presolve::export_recovery_record(h.folding_summary.presolve.record);
+  const std::size_t fixed = s.record.fixed_idx.size() + s.record.tape.types.size();
+  const gp::presolve::LPInfoGpu& reduced = s.reduced_lp;
+  (void)reduced.A.nnz;
+  (void)fixed;
+  (void)h.folding_summary.workspace_setup_seconds;
+  (void)h.folding_summary.unsupported_stages;
+  return true;
+}
+
+gp::Solution finish(Handle& h, double* x, double* y, double* z) {
+  gp::Solution sol = gp::postprocess(x, y, z, h.folding_summary, h.original_lp);
+  gp::free_summary(h.folding_summary);
+  return sol;
+}
+"""
+
+
+def test_adapter_in_reference_style_compiles(tmp_path):
+    src = tmp_path / "adapter.cpp"
+    src.write_text(ADAPTER)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(REPO, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.gpu
 def test_example_caller_runs():
     exe = os.path.join(PKG, "presolve_example")

@@ -271,6 +271,13 @@ def test_public_pipeline_contract():
         s.free()
         s = G.preprocess(d, G.parity_params())
         assert s.owns_reduced_lp == 1
+        assert s.unsupported_stages == 0
+        s.free()
+        # reference defaults request folding and the Startup rules: reported, not run
+        s = G.preprocess(d, G.default_params())
+        assert s.unsupported_stages == 3 and s.folding_applied == 0
+        s.free()
+        s = G.preprocess(d, G.parity_params())
         assert s.preprocessing_seconds >= s.elapsed_seconds >= 0.0
         after = G.device_lp_to_host(d.lp)
         for k in ("row_ptr", "col_idx", "vals", "c", "l", "u", "AL", "AU"):
