@@ -24,11 +24,27 @@ HBM_PEAK_GBPS = 8000.0
 
 # kernel-timing family -> the one kernel it brackets (rocprofv3 kernel name)
 KERNEL_OF_FAMILY = {
+    "compact": "k_compact_tiles",
     "activity": "k_row_fold<ActivityFold, 256, 2048>",
     "sc_activity": "k_row_fold<ScActivityFold, 256, 2048>",
     "redundant": "k_row_fold<RedundantFold, 256, 2048>",
     "bound_candidates": "k_bs_candidates",
-    "col_max": "k_col_max_abs",
+    "col_max": "k_seg_reduce_tiles<AbsMaxOp>",
+}
+
+# what each bench workload is (the generator recipe in csrc/lpgen.c)
+RECIPE = ("Zipf(1.5) column degree capped at 500, 1% empty columns; planted 5% singleton rows, 5% doubleton "
+          "equalities, 2% parallel rows, 2% duplicate and 3% fixed columns; 40% equality rows; 10% free columns; "
+          "int32 indices, FP64 values")
+WORKLOAD_TEXT = {
+    "large": "5M x 8M synthetic LP, " + RECIPE + "; integer-valued data: coefficients +-1..4, integer bounds, "
+             "planted point and slacks, parallel-row factors in {+-1, +-2, +-3}, so every activity and shift is "
+             "exact in binary64",
+    "large_fp64": "5M x 8M synthetic LP, " + RECIPE + "; coefficients N(0,1) clipped away from |a| < 1e-3, "
+                  "parallel-row factors U[0.5, 4]",
+    "large10": "500k x 800k, the large recipe at 1/10 scale, integer-valued data",
+    "medium": "200k x 300k synthetic LP, ~2M nnz, " + RECIPE,
+    "small": "2k x 3k synthetic LP, uniform degree 6, FP64 N(0,1) coefficients",
 }
 
 
@@ -54,6 +70,106 @@ def pct(v, q):
     import numpy as np
 
     return round(float(np.percentile(np.asarray(v) * 1e3, q)), 3)
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def pslp_pinned(lp, cores=2):
+    """PSLP 0.0.11 on `lp`, its threads pinned to the first `cores` CPUs this
+    process may use (taskset -c 0-1 in the reference's setup); PSLP's prints
+    go to stderr so stdout keeps the one JSON line."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pslp
+
+    if not pslp.available():
+        return None
+    saved_aff = os.sched_getaffinity(0)
+    use = set(sorted(saved_aff)[:cores])
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        os.sched_setaffinity(0, use)
+        r = pslp.presolve(lp)
+    finally:
+        os.sched_setaffinity(0, saved_aff)
+        import ctypes
+
+        ctypes.CDLL(None).fflush(None)  # PSLP's buffered C stdio goes out while fd 1 is still stderr
+        os.dup2(saved, 1)
+        os.close(saved)
+    st = r["status"]
+    r["outcome"] = ("infeasible" if st >= 0 and st & 2 else "unbounded" if st >= 0 and st & 4 else
+                    "reduced" if st >= 0 and st & 1 else "unchanged" if st == 0 else "error")
+    r["cores"] = len(use)
+    return r
+
+
+def gpu_presolve_ms(lp, params, reps):
+    """p50 and p99 wall time of gp_preprocess + gp_free_summary on a resident LP."""
+    import gpupre as G
+
+    with G.DeviceLP(lp) as d:
+        G.preprocess(d, params).free()
+        times, red = [], None
+        for _ in range(reps):
+            G.check(G.load().gp_synchronize())
+            t = time.perf_counter()
+            s = G.preprocess(d, params)
+            red = (s.reduced_rows, s.reduced_cols, s.reduced_nnz, s.iterations,
+                   "infeasible" if s.has_infeasible else "unbounded" if s.has_unbounded else "ok")
+            s.free()
+            times.append(time.perf_counter() - t)
+    return pct(times, 50), pct(times, 99), red
+
+
+def configs_leg(seed):
+    """Presolve time of the benchmark configurations that fit on the host
+    baseline as well: GPU p50 over 10 runs against PSLP on two pinned cores."""
+    import gpupre as G
+
+    cases = [("config1 small 2000x3000", G.generate_workload("small", seed)),
+             ("config2 medium 200000x300000", G.generate_workload("medium", seed)),
+             ("config5 QAP n=12", G.generate_qap(12, seed)),
+             ("config5 QAP n=24", G.generate_qap(24, seed)),
+             ("config5 generator 200000x300000", G.generate(200_000, 300_000, seed, degree_mode=0, mean_degree=6.0))]
+    out = {}
+    for name, lp in cases:
+        p50, p99, red = gpu_presolve_ms(lp, G.parity_params(), 10)
+        row = {"rows": lp.m, "cols": lp.n, "nnz": lp.nnz, "gpu_ms_p50": p50, "gpu_ms_p99": p99,
+               "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3], "status": red[4]}}
+        r = pslp_pinned(lp)
+        if r is not None:
+            row["pslp"] = {"time_presolve_ms": round(r["time_presolve"] * 1e3, 3),
+                           "wall_ms": round(r["wall_seconds"] * 1e3, 3), "outcome": r["outcome"],
+                           "reduced": [r["rows"], r["cols"], r["nnz"]], "cores": r["cores"]}
+        out[name] = row
+        log(f"{name}: {json.dumps(row)}")
+    return out
+
+
+def fp64_verdict_leg(seed):
+    """The large recipe with FP64 N(0,1) data: time for the presolve to reach
+    its verdict (the tolerance policy ends it as infeasible)."""
+    import gpupre as G
+
+    lp = G.generate_workload("large_fp64", seed)
+    p50, p99, red = gpu_presolve_ms(lp, G.parity_params(), 3)
+    out = {"workload": WORKLOAD_TEXT["large_fp64"], "rows": lp.m, "cols": lp.n, "nnz": lp.nnz, "ms_p50": p50,
+           "iterations": red[3], "status": red[4], "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2]}}
+    log(f"large_fp64 verdict: {json.dumps(out)}")
+    return out
 
 
 def batched_leg(count, seed, serial=True):
@@ -173,6 +289,9 @@ def main():
     ap.add_argument("--no-serial", action="store_true",
                     help="batched leg without the one-LP-after-another comparison (profiling runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the per-config GPU vs PSLP table")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-recipe verdict leg")
+    ap.add_argument("--no-pslp-full", action="store_true", help="skip PSLP on the whole workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
@@ -202,6 +321,7 @@ def main():
         sys.exit(1)
     t = time.time()
     lp = G.generate_workload(args.workload, seed=args.seed)
+    m0, n0, nnz = lp.m, lp.n, lp.nnz
     log(f"generated {args.workload}: {lp.m} x {lp.n}, nnz={lp.nnz} in {time.time() - t:.1f}s")
     t = time.time()
     dlp = G.DeviceLP(lp)
@@ -252,9 +372,10 @@ def main():
     if not args.no_kernel_timing:
         for name, v in ks.items():
             if v["launches"]:
-                families[name] = dict(ms=round(v["ms"], 3), launches=v["launches"],
-                                      gbps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["bytes"] > 0 else None)
-        # dominant single-kernel sweep family (the mixed families carry no byte model)
+                gbps = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["bytes"] > 0 and v["ms"] > 0 else None
+                families[name] = dict(ms=round(v["ms"], 3), launches=v["launches"], gbps=gbps,
+                                      frac=round(gbps / HBM_PEAK_GBPS, 4) if gbps is not None else None)
+        # dominant family that brackets one kernel (roofline of that kernel)
         sweep = [n for n in ks if ks[n]["launches"] and ks[n]["bytes"] > 0 and n in KERNEL_OF_FAMILY]
         dom = max(sweep, key=lambda n: ks[n]["ms"])
         v = ks[dom]
@@ -270,33 +391,40 @@ def main():
 
     cpu = None
     restatement = None
+    pslp_full = None
+    host = host_info()
     if not args.no_cpu_baseline:
+        # PSLP, the reference's bundled CPU presolver, on a bounded sample of the
+        # same recipe (the largest fraction on which it completes a presolve)
+        frac = {"large": 4, "large_fp64": 4, "large10": 1, "medium": 1, "small": 1}.get(args.workload, 1)
+        w = dict(G.WORKLOADS[args.workload])
+        m, n = w.pop("m") // frac, w.pop("n") // frac
+        sample = lp if frac == 1 else G.generate(m, n, args.seed, **w)
+        r = pslp_pinned(sample)
+        if r is not None:
+            cpu = {"value": round(sample.nnz / r["time_presolve"] / 1e6, 3) if r["time_presolve"] > 0 else None,
+                   "unit": "Mnnz/s", "cores": r["cores"], "kind": "reference",
+                   "sample": f"PSLP 0.0.11 (the reference's bundled CPU presolver, default settings) on a "
+                             f"{sample.m}x{sample.n} LP ({sample.nnz} nnz) from the same recipe ("
+                             f"{'1/' + str(frac) + ' of the rows and columns' if frac > 1 else 'the workload itself'}"
+                             f"): time_presolve {r['time_presolve']:.3f} s (the figure the reference reports), "
+                             f"new_presolver + run_presolver {r['wall_seconds']:.3f} s, outcome {r['outcome']} -> "
+                             f"{r['rows']}x{r['cols']} nnz={r['nnz']}; pinned to {r['cores']} CPUs (one thread plus "
+                             f"one helper during init) of {host['cpu_model']} ({host['nproc']} CPUs on the host)",
+                   "ms": round(r["time_presolve"] * 1e3, 1), "wall_ms": round(r["wall_seconds"] * 1e3, 1),
+                   "host": host}
+            log(f"cpu baseline (PSLP sample): {cpu}")
+        if frac > 1 and not args.no_pslp_full:
+            r = pslp_pinned(lp)
+            if r is not None:
+                pslp_full = {"rows": lp.m, "cols": lp.n, "nnz": lp.nnz, "outcome": r["outcome"],
+                             "wall_ms": round(r["wall_seconds"] * 1e3, 1),
+                             "time_presolve_ms": round(r["time_presolve"] * 1e3, 1),
+                             "note": "PSLP on the whole workload; an 'infeasible' outcome is a tolerance artefact "
+                                     "(the generator's planted point satisfies every row and bound, "
+                                     "tests/test_generator_feasibility.py)"}
+                log(f"PSLP on the full LP: {pslp_full}")
         sys.path.insert(0, os.path.join(REPO, "oracle"))
-        import pslp
-
-        if pslp.available():
-            # the reference's bundled CPU presolver (PSLP 0.0.11) on the full LP;
-            # its C-level prints go to stderr so stdout keeps the one JSON line
-            sys.stdout.flush()
-            saved = os.dup(1)
-            os.dup2(2, 1)
-            try:
-                r = pslp.presolve(lp)
-            finally:
-                os.dup2(saved, 1)
-                os.close(saved)
-            st = r["status"]
-            outcome = ("declared the LP infeasible (a false positive: the generator's planted point satisfies "
-                       "every row and bound)" if st >= 0 and st & 2 else
-                       f"reduced it to {r['rows']}x{r['cols']} nnz={r['nnz']}")
-            cpu = {"value": round(lp.nnz / r["wall_seconds"] / 1e6, 3), "unit": "Mnnz/s", "cores": 2,
-                   "kind": "reference",
-                   "sample": f"PSLP 0.0.11 (the reference's bundled CPU presolver, default settings) on the full "
-                             f"{lp.m}x{lp.n} LP: new_presolver+run_presolver {r['wall_seconds']:.2f} s "
-                             f"(time_presolve {r['time_presolve']:.2f} s); PSLP {outcome}; one thread plus one "
-                             f"helper thread during init",
-                   "ms": round(r["wall_seconds"] * 1e3, 1)}
-            log(f"cpu baseline (PSLP): {cpu}")
         import oracle as O
 
         # the CPU restatement of this engine (the parity oracle) on a bounded
@@ -317,22 +445,27 @@ def main():
             restatement = None
 
     dlp.free()
+    del lp
     batched = None if args.no_batched else batched_leg(args.batch_count, args.seed, not args.no_serial)
     learned = None if args.no_learned else learned_leg(args.seed)
+    configs = None if args.no_configs else configs_leg(args.seed)
+    fp64 = None if (args.no_fp64 or args.workload != "large") else fp64_verdict_leg(args.seed)
 
-    value = lp.nnz / (ms_per_step * 1e-3) / 1e6
+    value = nnz / (ms_per_step * 1e-3) / 1e6
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mnnz/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.workload} synthetic LP presolve", "rows": lp.m, "cols": lp.n,
-                   "nnz": lp.nnz, "seed": args.seed, "folding": False, "structure_specific_rules": False,
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload), "name": args.workload,
+                   "rows": m0, "cols": n0, "nnz": nnz, "seed": args.seed, "folding": False,
+                   "structure_specific_rules": False,
                    "reduced": {"rows": reduced[0], "cols": reduced[1], "nnz": reduced[2], "iterations": reduced[3],
                                "status": reduced[4]}},
         "presolve_core_ms_p50": round(float(np.median(core)) * 1e3, 3),
         "postsolve_ms": round(post_ms, 2), "postsolve_records": int(rec_count),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
-        "batched": batched, "learned_vs_plain": learned,
+        "batched": batched, "learned_vs_plain": learned, "configs": configs, "large_fp64_verdict": fp64,
+        "pslp_full_lp": pslp_full,
     }
     print(json.dumps(out), flush=True)
 

@@ -204,6 +204,10 @@ struct KernelClock {
   void stop_segment();
   int begin(int family);                // returns a token (or -1)
   void end(int token, double algorithmic_bytes);
+  // bytes of a family whose size is known only after its scope closed
+  void add_bytes(int family, double b) {
+    if (enabled) bytes[family] += b;
+  }
   void flush();                          // sync + accumulate
   ~KernelClock();
 };

@@ -197,7 +197,8 @@ void apply_plan(Working& cur, gp_plan& plan, Record& r, bool captured) {
     compact_csr(cur.lp.AT, plan.keep_col_mask, plan.keep_row_mask, col_o2n.data(), n_new, m_new,
                 next.AT);
   {
-    TimedScope ts(GP_KT_COMPACT, 2.0 * (24.0 * n + 16.0 * m) / 2.0 + 24.0 * n_new + 16.0 * m_new + 4.0 * (m + n));
+    // c, l, u, AL, AU read through the old -> new maps, written compacted
+    TimedScope ts(GP_KT_VECTORS, 24.0 * n + 16.0 * m + 24.0 * n_new + 16.0 * m_new + 4.0 * (m + n));
     next.c = pool_alloc<double>(n_new);
     next.l = pool_alloc<double>(n_new);
     next.u = pool_alloc<double>(n_new);

@@ -1008,17 +1008,22 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   DVec<int32_t> tile_row(static_cast<size_t>(tiles));
   DVec<unsigned long long> ctl(static_cast<size_t>(tiles) + 3);
   unsigned long long res[3] = {};
+  GP_CHECK(hipMemsetAsync(ctl.data(), 0, sizeof(unsigned long long) * (static_cast<size_t>(tiles) + 3), 0));
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
   {
-    TimedScope ts(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + 12.0 * A.nnz + 4.0 * (m_new + 1));
-    GP_CHECK(hipMemsetAsync(ctl.data(), 0, sizeof(unsigned long long) * (static_cast<size_t>(tiles) + 3), 0));
-    hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
+    // the tile kernel alone
+    TimedScope ts(GP_KT_COMPACT, 0.0);
     const unsigned grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
     hipLaunchKernelGGL(k_compact_tiles, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
                        static_cast<int64_t>(A.nnz), tiles, static_cast<const int32_t*>(tile_row.data()), keep_row,
                        row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
     GP_LAUNCH_CHECK("compact");
-    d2h(res, ctl.data(), 3);
   }
+  d2h(res, ctl.data(), 3);
+  // read A and the masks, write the kept entries and the new row pointers
+  // (the kept count is known after the pass)
+  kclock().add_bytes(GP_KT_COMPACT, 4.0 * (m + 1) + 12.0 * A.nnz + (static_cast<double>(m) + A.cols) +
+                                        12.0 * static_cast<double>(res[2]) + 4.0 * (m_new + 1));
   if (res[1] != 0) throw std::runtime_error("compaction look-back did not complete");
   if (res[2] > static_cast<unsigned long long>(INT32_MAX)) throw std::length_error("compacted nnz exceeds int32 offsets");
   out.nnz = static_cast<gp_offset>(res[2]);
@@ -1600,10 +1605,11 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const int32_t* row_nnz, double2* entry_lu) {
   if (A.rows <= 0) return;
   const PackedMask cols(keep_col, A.cols);
-  // row_ptr, col_idx, vals; one (l, u) pair and one mask bit per column; keep_row; 24 bytes out per
-  // row; with the optional outputs, 16 bytes per entry (its (l, u) pair) and a 64-byte record per row
-  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + A.cols / 8.0 + A.rows + 24.0 * A.rows +
-                       (entry_lu ? 16.0 * A.nnz : 0.0) + (rec ? 64.0 * A.rows : 0.0);
+  // compulsory bytes of the sweep: row_ptr, col_idx, vals; one (l, u) pair per column; the row and
+  // column masks (one byte each); 24 bytes out per row.  The optional stored outputs (16 bytes per
+  // entry, a 64-byte record per row) are intermediates for later rules, not counted.
+  const double bytes = 4.0 * (A.rows + 1) + 12.0 * A.nnz + 16.0 * A.cols + (static_cast<double>(A.rows) + A.cols) +
+                       24.0 * A.rows;
   TimedScope ts(GP_KT_ACTIVITY, bytes);
   launch_row_fold(A, lu, cols.view(),
                   ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz,

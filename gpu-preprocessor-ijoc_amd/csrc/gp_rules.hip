@@ -84,7 +84,8 @@ void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
   if (n == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // l, u, c and keep_col per column; AL, AU per row
+  TimedScope ts(GP_KT_CLEANUP, 25.0 * n + 16.0 * m);
   DVec<uint8_t> fixed(n);
   DVec<double> val(n), radius(n), row_error(m ? m : 1);
   Flags flags;
@@ -155,7 +156,8 @@ void rule_empty_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols;
   if (n == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // col_nnz, keep_col, c, l, u per column
+  TimedScope ts(GP_KT_CLEANUP, 29.0 * n);
   DVec<uint8_t> fixed(n);
   DVec<double> val(n);
   Flags flags;
@@ -202,7 +204,8 @@ void rule_empty_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, c
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
   if (m == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // row_nnz, keep_row, AL, AU per row
+  TimedScope ts(GP_KT_CLEANUP, 21.0 * m);
   Flags flags;
   hipLaunchKernelGGL(k_empty_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, plan.keep_row_mask, plan.new_AL,
                      plan.new_AU, s.row_nnz, p.feasibility_tol, flags.d.data());
@@ -306,7 +309,8 @@ void rule_singleton_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // singleton mask, column and value, keep_row, AL, AU per row; l, u per column
+  TimedScope ts(GP_KT_CLEANUP, 30.0 * m + 16.0 * n);
   DVec<double> cl(n ? n : 1), cu(n ? n : 1);
   DVec<uint8_t> remove(m);
   d2d(cl.data(), plan.new_l, n);
@@ -503,7 +507,8 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
   if (m == 0) return;
-  TimedScope outer(GP_KT_RULES_OTHER, 0.0);
+  // outside the fold: the three decision flags and AL, AU per row
+  TimedScope outer(GP_KT_BOUNDS, 19.0 * m);
   DVec<uint8_t> full(m), dl(m), du(m);
   Flags flags;
   const PackedMask live_cols(s.live_masks ? static_cast<const uint8_t*>(plan.keep_col_mask) : nullptr, lp.A.cols);
@@ -994,7 +999,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   }
   Flags flags;
   {
-    TimedScope outer(GP_KT_RULES_OTHER, 0.0);
+    // outside the sweeps: candidates, bounds, cost and column max per column
+    TimedScope outer(GP_KT_BOUNDS, 48.0 * n);
     const bool masked = s.live_masks != 0;
     const PackedMask live_rows(masked ? static_cast<const uint8_t*>(plan.keep_row_mask) : nullptr, m);
     RuleCaches& rc = rule_caches(plan);
@@ -1059,7 +1065,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     return;
   }
   const bool f2 = f[2] != 0;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // support rows, records and fixings: the new bounds per column
+  TimedScope ts(GP_KT_BOUNDS, 16.0 * n);
   if (tape_on(plan, p) && f[1]) {
     const double support_tol = cmax(10.0 * p.bound_tol, 1.0e-10);
     DVec<int32_t> sl(n), su(n);

@@ -1680,7 +1680,8 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   if (plan.has_infeasible || plan.has_unbounded || !p.enable_singleton_columns) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // singleton statistics, c, l, u per column; row activity cache, AL, AU per row
+  TimedScope ts(GP_KT_SINGLETON_COLUMNS, 37.0 * n + 56.0 * m);
   Flags flags;
   hipLaunchKernelGGL(k_sc_count, dim3(grid_capped(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask,
                      s.singleton_column_mask, flags.d.data());
@@ -1829,7 +1830,8 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
   if (n == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // one pass over AT; c, l, u and keep_col per column; AL, AU and keep_row per row
+  TimedScope ts(GP_KT_DUAL_FIX, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + 25.0 * n + 17.0 * m);
   Flags flags;
   DVec<uint8_t> fixed(n), inf(n), ffix(n);
   DVec<double> val(n), contrib(n);
@@ -1931,7 +1933,8 @@ void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
   DVec<uint64_t> hash(m);
   DVec<int32_t> ids(m);
   {
-    TimedScope ts(GP_KT_HASH_SORT, 0.0);
+    // hash of every row (one pass over A, 8 bytes out per row) and the sort of the (hash, row) pairs
+    TimedScope ts(GP_KT_HASH_SORT, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 8.0 * m + 24.0 * m);
     DVec<int32_t> len(m);
     DVec<gp_offset> first(m);
     first_live(lp.A, LivePr{keep_col}, len.data(), first.data());
@@ -1939,7 +1942,8 @@ void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
                                     p.zero_tol, use_xor, hash.data(), ids.data()});
     sort_pairs_u64_i32(hash.data(), ids.data(), m);
   }
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // sorted (hash, row) pairs, one pass over A for the ratios, AL, AU per row
+  TimedScope ts(GP_KT_PARALLEL, 12.0 * m + 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * m);
   Flags flags;
   DVec<uint8_t> row_delete(m);
   fill_u8(row_delete.data(), 0, m);
@@ -1989,7 +1993,8 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
   DVec<uint64_t> hash(n);
   DVec<int32_t> ids(n);
   {
-    TimedScope ts(GP_KT_HASH_SORT, 0.0);
+    // hash of every column (one pass over AT, 8 bytes out per column) and the sort of the pairs
+    TimedScope ts(GP_KT_HASH_SORT, 4.0 * (n + 1) + 12.0 * lp.AT.nnz + 8.0 * n + 24.0 * n);
     DVec<int32_t> len(n);
     DVec<gp_offset> first(n);
     first_live(lp.AT, LivePc{plan.keep_row_mask, zt}, len.data(), first.data());
@@ -1997,7 +2002,8 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
                                      lp.AT.vals, hash.data(), ids.data()});
     sort_pairs_u64_i32(hash.data(), ids.data(), n);
   }
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // sorted (hash, column) pairs, one pass over AT for the ratios, c, l, u per column
+  TimedScope ts(GP_KT_PARALLEL, 12.0 * n + 4.0 * (n + 1) + 12.0 * lp.AT.nnz + 24.0 * n);
   Flags flags;
   double active_max = p.parallel_columns_max_abs_ratio;
   if (std::isfinite(p.parallel_columns_max_abs_ratio) && p.parallel_columns_scale_guard_min_groups > 1) {
@@ -2065,7 +2071,8 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return;
-  TimedScope ts(GP_KT_RULES_OTHER, 0.0);
+  // one pass over A; c, l, u per column; AL, AU per row
+  TimedScope ts(GP_KT_DOUBLETON, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 24.0 * n + 16.0 * m);
   const gp_csr A = plan.has_new_A ? plan.new_A : lp.A;
   const gp_csr& AT = lp.AT;
   const double zt = p.zero_tol, bt = p.bound_tol;

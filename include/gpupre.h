@@ -230,13 +230,20 @@ enum {
   GP_KT_BOUND_CANDIDATES = 2, /* bound-strengthening candidates                */
   GP_KT_COMPACT = 3,          /* CSR compaction at materialize                 */
   GP_KT_TRANSPOSE = 4,        /* CSR transpose                                 */
-  GP_KT_RULES_OTHER = 5,      /* everything else inside the rules              */
+  GP_KT_RULES_OTHER = 5,      /* rule work outside the families below          */
   GP_KT_HASH_SORT = 6,        /* parallel rows / columns hashing and sorts     */
   GP_KT_SC_ACTIVITY = 7,      /* singleton-column row activity cache (row fold)*/
   GP_KT_REDUNDANT = 8,        /* redundant-row activity test (row fold)        */
   GP_KT_COL_MAX = 9,          /* column max |a|                                */
   GP_KT_LIVE_STATS_INC = 10,  /* live statistics from deleted lines            */
-  GP_KT_COUNT = 11
+  GP_KT_CLEANUP = 11,         /* fixed variables, empty rows / columns, singleton rows */
+  GP_KT_SINGLETON_COLUMNS = 12, /* singleton columns outside the activity cache */
+  GP_KT_BOUNDS = 13,          /* bound strengthening / redundant rows outside the folds */
+  GP_KT_DUAL_FIX = 14,        /* dual fix                                      */
+  GP_KT_DOUBLETON = 15,       /* doubleton equations                           */
+  GP_KT_PARALLEL = 16,        /* parallel rows / columns outside the hashing   */
+  GP_KT_VECTORS = 17,         /* vector compaction at materialize              */
+  GP_KT_COUNT = 18
 };
 
 /* ---- learned pass scheduler (no reference counterpart) --------------------
