@@ -350,6 +350,7 @@ struct Flags {
 void inclusive_scan_i32(int32_t* data, int64_t n);
 void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total);
 int64_t exclusive_scan_i32_total(int32_t* data, int64_t n);  // returns total
+void exclusive_scan_i32(int32_t* data, int64_t n);            // no host read
 // exclusive scan whose total is written to *dev_total (device), no host read
 void exclusive_scan_i64_device_total(int64_t* data, int64_t n, int64_t* dev_total);
 // the same for three arrays of length n at once; totals to dev_totals[0..2]

@@ -923,6 +923,15 @@ int gp_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint8
   });
 }
 
+int gp_radix_sort_pairs(uint64_t* keys, int32_t* vals, int64_t n, int32_t end_bit) {
+  return guarded([&] {
+    if (n < 0 || end_bit < 0 || end_bit > 64) throw std::invalid_argument("bad sort arguments");
+    if (n > 0 && (!keys || !vals)) throw std::invalid_argument("null argument");
+    sort_pairs_u64_i32(keys, vals, n, end_bit);
+    GP_CHECK(hipDeviceSynchronize());
+  });
+}
+
 // ---- matrix primitives ----------------------------------------------------------------------
 int gp_transpose_csr(const gp_csr* A, gp_csr* AT) {
   return guarded([&] {

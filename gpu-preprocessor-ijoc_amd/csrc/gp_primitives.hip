@@ -284,6 +284,9 @@ __global__ void k_old_to_new(const uint8_t* keep, int32_t n, int32_t* map) {
 }  // namespace
 
 void inclusive_scan_i32(int32_t* data, int64_t n) { scan_impl<int32_t>(data, n, true); }
+void exclusive_scan_i32(int32_t* data, int64_t n) {
+  if (n > 0) scan_impl<int32_t>(data, n, false);
+}
 
 void exclusive_scan_i64(int64_t* data, int64_t n, int64_t* total) {
   if (n <= 0) {

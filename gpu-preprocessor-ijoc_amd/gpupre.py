@@ -308,6 +308,7 @@ def load() -> C.CDLL:
                                                       vp]),
         "gp_transpose_csr": (C.c_int, [C.POINTER(GpCsr), C.POINTER(GpCsr)]),
         "gp_parallel_row_hashes": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.c_double, vp]),
+        "gp_radix_sort_pairs": (C.c_int, [vp, vp, C.c_int64, C.c_int32]),
         "gp_compact_csr": (C.c_int, [C.POINTER(GpCsr), vp, vp, C.POINTER(GpCsr)]),
         "gp_csr_free": (None, [C.POINTER(GpCsr)]),
         "gp_device_count": (C.c_int, [C.POINTER(C.c_int32)]),

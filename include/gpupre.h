@@ -597,6 +597,13 @@ int gp_row_activity_summary_noalloc(const gp_csr* A, const double* l, const doub
                                     const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
                                     double* max_fin, int32_t* min_ninf, int32_t* max_pinf);
 
+/* Stable sort of (key, value) pairs by key bits [0, end_bit), in place on the
+ * device; the engine's replacement for the reference's library sorts
+ * (cub::DeviceRadixSort in postsolve/parallel_replay.cuh, thrust::sort_by_key
+ * in rule_doubleton_equations.cu:126-148 and rule_parallel_columns.cu:746-766,
+ * the thrust sorts of rule_parallel_rows.cu:643-888). */
+int gp_radix_sort_pairs(uint64_t* keys, int32_t* vals, int64_t n, int32_t end_bit);
+
 /* compute_parallel_row_hashes (rule_parallel_rows.cu:141-196,583-627): per row
  * the FNV-1a chain over (column, bits of a / first live a) of its live
  * entries, XOR-aggregated per entry when nnz > 32 rows; `row` for rows not

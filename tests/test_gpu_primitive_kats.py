@@ -192,3 +192,29 @@ def test_compaction_stable_filter_long_rows():
     assert_same(red.row_ptr, e["row_ptr"], "row_ptr")
     assert_same(red.col_idx, e["col_idx"], "col_idx")
     assert np.array_equal(red.vals.view(np.uint64), e["vals"].view(np.uint64))
+
+
+@pytest.mark.parametrize("n,end_bit,distinct", [(1, 64, 0), (2, 64, 0), (4095, 64, 0), (4096, 23, 0),
+                                                (4097, 64, 3), (100_003, 64, 0), (100_003, 64, 17),
+                                                (1_000_000, 40, 1000), (3_000_001, 24, 0)])
+def test_radix_sort_pairs_is_a_stable_sort(n, end_bit, distinct):
+    """Keys on bits [0, end_bit) -- uniform, or drawn from `distinct` values
+    to make long runs of equal keys -- with values recording the input
+    order: the device sort equals numpy's stable argsort, bits above end_bit
+    ignored."""
+    print(f"[radix sort] n={n} end_bit={end_bit} distinct={distinct}", flush=True)
+    rng = np.random.default_rng(n + end_bit)
+    if distinct:
+        pool = rng.integers(0, 1 << 63, size=distinct, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        keys = pool[rng.integers(0, distinct, size=n)]
+    else:
+        keys = rng.integers(0, 1 << 63, size=n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, size=n,
+                                                                                             dtype=np.uint64)
+    vals = np.arange(n, dtype=np.int32)
+    mask = np.uint64((1 << end_bit) - 1) if end_bit < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    order = np.argsort(keys & mask, kind="stable")
+    dk, dv = G.DeviceArray.from_host(keys), G.DeviceArray.from_host(vals)
+    G.check(G.load().gp_radix_sort_pairs(dk.ptr, dv.ptr, n, end_bit))
+    gk, gv = G.to_host(dk.ptr, n, np.uint64), G.to_host(dv.ptr, n, np.int32)
+    assert np.array_equal(gv, order.astype(np.int32))
+    assert np.array_equal(gk, keys[order])
