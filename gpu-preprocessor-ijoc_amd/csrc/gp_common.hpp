@@ -607,6 +607,7 @@ void seg_reduce_tiles(const gp_csr& M, const Op& op, typename Op::Slot* out) {
 void col_max_abs_tiles(const gp_csr& AT, Bits keep_row, double* vmax);
 // Matrices.
 void transpose_csr(const gp_csr& A, gp_csr& AT);  // AT arrays from the pool
+int bits_for(int64_t v);  // bits needed for the values 0..v-1 (at least 1)
 // the histogram / atomic-scatter / segment-sort transpose (for matrices with more
 // entries than one radix sort takes, 2^31 - 1)
 void transpose_csr_scatter(const gp_csr& A, gp_csr& AT);

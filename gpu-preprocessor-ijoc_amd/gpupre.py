@@ -62,6 +62,13 @@ class GpDeviceLp(C.Structure):
                 ("obj_constant", C.c_double)]
 
 
+class GpMpsModel(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("entries", C.c_int64), ("entry_row", C.c_void_p),
+                ("entry_col", C.c_void_p), ("entry_val", C.c_void_p), ("c", C.c_void_p), ("AL", C.c_void_p),
+                ("AU", C.c_void_p), ("l", C.c_void_p), ("u", C.c_void_p), ("obj_constant", C.c_double),
+                ("name", C.c_void_p)]
+
+
 class GpParams(C.Structure):
     _fields_ = [
         ("struct_size", C.c_int32), ("max_iters", C.c_int32), ("max_time", C.c_double),
@@ -321,6 +328,12 @@ def load() -> C.CDLL:
         "gp_lp_upload": (C.c_int, [C.c_int32, C.c_int32, OFFSET, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                     C.POINTER(GpDeviceLp)]),
         "gp_lp_free": (None, [C.POINTER(GpDeviceLp)]),
+        "gp_lp_upload_entries": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
+                                           C.c_double, C.POINTER(GpDeviceLp)]),
+        "gp_mps_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(GpMpsModel)]),
+        "gp_mps_read": (C.c_int, [C.c_char_p, C.POINTER(GpMpsModel)]),
+        "gp_mps_free": (None, [C.POINTER(GpMpsModel)]),
+        "gp_mps_upload": (C.c_int, [C.POINTER(GpMpsModel), C.POINTER(GpDeviceLp)]),
         "gp_preprocess_batch": (C.c_int, [C.POINTER(GpDeviceLp), C.c_int32, C.POINTER(GpParams),
                                           C.POINTER(GpBatchSummary)]),
         "gp_postprocess_batch": (C.c_int, [vp, vp, vp, C.POINTER(GpBatchSummary), vp, vp, vp]),
@@ -429,6 +442,28 @@ class DeviceLP:
                                   _ptr(lp.c), _ptr(lp.AL), _ptr(lp.AU), _ptr(lp.l), _ptr(lp.u),
                                   float(lp.obj_constant), C.byref(self.lp)))
 
+    @classmethod
+    def from_mps(cls, model: MpsModel) -> "DeviceLP":
+        """gp_mps_upload: duplicates summed and A, A^T assembled on the device."""
+        self = cls.__new__(cls)
+        self.lp = GpDeviceLp()
+        self.m, self.n = model.m, model.n
+        check(load().gp_mps_upload(C.byref(model.raw), C.byref(self.lp)))
+        return self
+
+    @classmethod
+    def from_entries(cls, m, n, rows, cols, vals, c, AL, AU, l, u, obj_constant=0.0) -> "DeviceLP":
+        """gp_lp_upload_entries on host (row, column, value) entries in any order."""
+        self = cls.__new__(cls)
+        self.lp = GpDeviceLp()
+        self.m, self.n = m, n
+        rows, cols = np.ascontiguousarray(rows, np.int32), np.ascontiguousarray(cols, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        vecs = [np.ascontiguousarray(v, np.float64) for v in (c, AL, AU, l, u)]
+        check(load().gp_lp_upload_entries(m, n, len(vals), _ptr(rows), _ptr(cols), _ptr(vals),
+                                          *[_ptr(v) for v in vecs], float(obj_constant), C.byref(self.lp)))
+        return self
+
     def free(self):
         if self.lp.A.row_ptr:
             load().gp_lp_free(C.byref(self.lp))
@@ -444,6 +479,60 @@ class DeviceLP:
             self.free()
         except Exception:
             pass
+
+
+class MpsModel:
+    """An MPS file read by gp_mps_parse / gp_mps_read (host memory of the
+    library): dimensions, vectors and the matrix entries in file order."""
+
+    def __init__(self, raw: GpMpsModel):
+        self.raw = raw
+        self.m, self.n, self.entries = raw.rows, raw.cols, raw.entries
+        self.name = C.string_at(raw.name) if raw.name else b""
+        self.obj_constant = float(raw.obj_constant)
+
+    @staticmethod
+    def _host(ptr, count, ctype, dtype):
+        if count == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), (count,)).copy()
+
+    def arrays(self) -> dict:
+        r, i32, f64 = self.raw, (C.c_int32, np.int32), (C.c_double, np.float64)
+        return {"entry_row": self._host(r.entry_row, self.entries, *i32),
+                "entry_col": self._host(r.entry_col, self.entries, *i32),
+                "entry_val": self._host(r.entry_val, self.entries, *f64),
+                "c": self._host(r.c, self.n, *f64), "l": self._host(r.l, self.n, *f64),
+                "u": self._host(r.u, self.n, *f64), "AL": self._host(r.AL, self.m, *f64),
+                "AU": self._host(r.AU, self.m, *f64)}
+
+    def upload(self) -> "DeviceLP":
+        return DeviceLP.from_mps(self)
+
+    def free(self):
+        load().gp_mps_free(C.byref(self.raw))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
+def parse_mps(text) -> MpsModel:
+    """gp_mps_parse on an in-memory MPS text (bytes or str)."""
+    if isinstance(text, str):
+        text = text.encode("latin-1")
+    raw = GpMpsModel()
+    check(load().gp_mps_parse(text, len(text), C.byref(raw)))
+    return MpsModel(raw)
+
+
+def read_mps(path) -> MpsModel:
+    """gp_mps_read on a file."""
+    raw = GpMpsModel()
+    check(load().gp_mps_read(os.fsencode(path), C.byref(raw)))
+    return MpsModel(raw)
 
 
 def device_lp_to_host(d: GpDeviceLp) -> HostLP:

@@ -364,4 +364,79 @@ inline void free_summary(Summary& summary) {
   summary = Summary{};
 }
 
+namespace tools {
+
+// An MPS file in host memory (mps_reader::MpsModel, mpsreader.hpp:14-17): its
+// name, dimensions, vectors and matrix entries in file order.  Move-only.
+class MpsModel {
+ public:
+  MpsModel() = default;
+  MpsModel(const MpsModel&) = delete;
+  MpsModel& operator=(const MpsModel&) = delete;
+  MpsModel(MpsModel&& o) noexcept : raw_(std::exchange(o.raw_, gp_mps_model{})) {}
+  MpsModel& operator=(MpsModel&& o) noexcept {
+    if (this != &o) {
+      gp_mps_free(&raw_);
+      raw_ = std::exchange(o.raw_, gp_mps_model{});
+    }
+    return *this;
+  }
+  ~MpsModel() { gp_mps_free(&raw_); }
+
+  std::string name() const { return raw_.name ? raw_.name : ""; }
+  int num_rows() const noexcept { return raw_.rows; }
+  int num_cols() const noexcept { return raw_.cols; }
+  double obj_constant() const noexcept { return raw_.obj_constant; }
+  const gp_mps_model& raw() const noexcept { return raw_; }
+  gp_mps_model* raw_ptr() noexcept { return &raw_; }
+
+ private:
+  gp_mps_model raw_{};
+};
+
+// read_mps_file (gpu_preprocessor_tool_common.hpp:143-154) and read_mps on a
+// text already in memory (mpsreader.hpp:19).
+inline MpsModel read_mps_file(const std::string& path) {
+  MpsModel model;
+  detail::raise(gp_mps_read(path.c_str(), model.raw_ptr()));
+  return model;
+}
+inline MpsModel read_mps_text(const std::string& text) {
+  MpsModel model;
+  detail::raise(gp_mps_parse(text.data(), text.size(), model.raw_ptr()));
+  return model;
+}
+
+// DeviceLpOwner (gpu_preprocessor_tool_common.hpp:77-106): a device LP that
+// frees its arrays on destruction.  Move-only.
+struct DeviceLpOwner {
+  DeviceLp lp;
+  DeviceLpOwner() = default;
+  DeviceLpOwner(const DeviceLpOwner&) = delete;
+  DeviceLpOwner& operator=(const DeviceLpOwner&) = delete;
+  DeviceLpOwner(DeviceLpOwner&& o) noexcept : lp(std::exchange(o.lp, DeviceLp{})) {}
+  DeviceLpOwner& operator=(DeviceLpOwner&& o) noexcept {
+    if (this != &o) {
+      reset();
+      lp = std::exchange(o.lp, DeviceLp{});
+    }
+    return *this;
+  }
+  ~DeviceLpOwner() { reset(); }
+  void reset() noexcept {
+    gp_lp_free(reinterpret_cast<gp_device_lp*>(&lp));
+    lp = DeviceLp{};
+  }
+};
+
+// upload_lp (gpu_preprocessor_tool_common.hpp:108-141): the model in HBM,
+// duplicates summed and A, A^T assembled on the device.
+inline DeviceLpOwner upload_lp(const MpsModel& model) {
+  DeviceLpOwner owner;
+  detail::raise(gp_mps_upload(&model.raw(), reinterpret_cast<gp_device_lp*>(&owner.lp)));
+  return owner;
+}
+
+}  // namespace tools
+
 }  // namespace gpu_preprocessor

@@ -661,6 +661,54 @@ int gp_lp_upload(int32_t m, int32_t n, gp_offset nnz, const gp_offset* row_ptr,
                  double obj_constant, gp_device_lp* out);
 void gp_lp_free(gp_device_lp* lp);
 
+/* ---- MPS ingest -------------------------------------------------------------
+ * An MPS file as mps_reader::read_mps reads it
+ * (cpp/mps_reader/include/mps_reader/mpsreader.hpp:14-19, LpModel of
+ * lp_model.hpp:12-81): rows x cols, costs, row sides AL/AU, column bounds
+ * l/u and the objective constant, with the matrix as its entries in file
+ * order (duplicates of one (row, column) not yet summed).  Host memory owned
+ * by the library: gp_mps_free. */
+typedef struct gp_mps_model {
+  int32_t rows;
+  int32_t cols;
+  int64_t entries;
+  int32_t* entry_row;
+  int32_t* entry_col;
+  double* entry_val;
+  double* c;
+  double* AL;
+  double* AU;
+  double* l;
+  double* u;
+  double obj_constant;
+  char* name; /* NAME card, NUL-terminated ("" when absent) */
+} gp_mps_model;
+
+/* read_mps (mpsreader.hpp:19; cpp/mps_reader/src/io/mpsreader.cpp:686-709) on
+ * an in-memory text: free format, then fixed format; GP_ERR_INVALID_ARGUMENT
+ * with the reference's message (the free-format one) when both fail.  Host
+ * only: no GPU call. */
+int gp_mps_parse(const char* text, size_t length, gp_mps_model* out);
+/* The same on a file (read_mps_file, cpp/tools/gpu_preprocessor_tool_common.hpp:143-154;
+ * GP_ERR_RUNTIME "failed to open <path>" when it cannot be opened). */
+int gp_mps_read(const char* path, gp_mps_model* out);
+void gp_mps_free(gp_mps_model* model);
+/* upload_lp (cpp/tools/gpu_preprocessor_tool_common.hpp:108-141): the model in
+ * HBM as a device LP.  Duplicates are summed on the device in file order
+ * (the reference's per-column stable sort and left-to-right sum,
+ * mpsreader.cpp:625-648), A^T comes out with increasing row indices per
+ * column and A with increasing column indices per row (CscMatrix::to_csr,
+ * sparse_matrix.cpp:111-135), explicit zeros kept.  Free with gp_lp_free. */
+int gp_mps_upload(const gp_mps_model* model, gp_device_lp* out);
+/* The device assembly of gp_mps_upload for any caller holding matrix entries
+ * (row, column, value) in its own order, duplicates allowed: same result as
+ * building the CSC on the host and calling upload_lp.  GP_ERR_INVALID_ARGUMENT
+ * when an entry lies outside m x n; GP_ERR_LENGTH beyond 2^31 - 1 entries. */
+int gp_lp_upload_entries(int32_t m, int32_t n, int64_t entries, const int32_t* entry_row,
+                         const int32_t* entry_col, const double* entry_val, const double* c,
+                         const double* AL, const double* AU, const double* l, const double* u,
+                         double obj_constant, gp_device_lp* out);
+
 #ifdef __cplusplus
 }
 #endif

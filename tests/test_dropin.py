@@ -3,7 +3,8 @@ gpu_preprocessor.hpp): it compiles as plain host C++ (layout checks against
 the C ABI are static_asserts), and on a GPU the example caller
 (examples/presolve_example.cpp) runs preprocess / postprocess / free_summary
 on an LP that presolve solves completely and checks the recovered point's
-KKT conditions."""
+KKT conditions; the MPS tool (examples/mps_presolve.cpp) reads a file,
+uploads it and presolves it with the same outcome as the Python path."""
 import os
 import subprocess
 
@@ -24,6 +25,9 @@ def test_shim_compiles_as_host_cpp(tmp_path):
         "  lp.A = {2, 2, 2, nullptr, nullptr, nullptr};\n"
         "  gpu_preprocessor::Summary s;\n"
         "  gpu_preprocessor::free_summary(s);\n"
+        "  gpu_preprocessor::tools::MpsModel model;\n"
+        "  gpu_preprocessor::tools::DeviceLpOwner owner;\n"
+        "  if (model.num_rows() != 0 || owner.lp.A.rowPtr) return 2;\n"
         "  return p.max_iters == 512 && lp.A.rows == 2 ? 0 : 1;\n"
         "}\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(REPO, "include"), str(src)],
@@ -99,3 +103,38 @@ def test_example_caller_runs():
     assert r.returncode == 0, r.stderr
     assert "x = (2, 3, 1, 1.5), y = (1, -2, 0.5), objective 1: KKT conditions hold" in r.stdout
     assert "unchanged run borrows the input LP" in r.stdout
+
+
+@pytest.mark.gpu
+def test_mps_tool_reads_and_presolves(tmp_path):
+    import numpy as np
+
+    import gpupre as G
+    import mps_texts as T
+
+    exe = os.path.join(PKG, "mps_presolve")
+    assert os.path.exists(exe), "build the package first"
+    r = subprocess.run([exe, os.path.join(REPO, "tests", "golden", "model.mps")], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    out = dict(line.split(" ", 1) for line in r.stdout.splitlines())
+    assert r.returncode == 0 and out["status"] == "ok"
+    assert (out["original_rows"], out["original_cols"], out["original_nnz"]) == ("2", "2", "4")
+
+    text, _ = T.write_integer_lp(np.random.default_rng(21), 2000, 3000, 14_000)
+    path = tmp_path / "intlp.mps"
+    path.write_bytes(text)
+    r = subprocess.run([exe, str(path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    out = dict(line.split(" ", 1) for line in r.stdout.splitlines())
+    with G.read_mps(str(path)) as model, model.upload() as d:
+        s = G.preprocess(d, G.default_params(enable_folding=0, enable_structure_specific_rules=0))
+        try:
+            want = {"original_rows": model.m, "original_cols": model.n, "reduced_rows": s.reduced_rows,
+                    "reduced_cols": s.reduced_cols, "reduced_nnz": s.reduced_nnz, "iterations": s.iterations}
+            status = "infeasible" if s.has_infeasible else "unbounded" if s.has_unbounded else "ok"
+        finally:
+            s.free()
+    assert out["status"] == status and r.returncode == (0 if status == "ok" else 1)
+    for k, v in want.items():
+        assert out[k] == str(v), k
