@@ -69,6 +69,26 @@ class GpMpsModel(C.Structure):
                 ("name", C.c_void_p)]
 
 
+class GpLpSolveParams(C.Structure):
+    _fields_ = [("struct_size", C.c_int32), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
+                ("time_limit", C.c_double), ("check_every", C.c_int32), ("ruiz_iterations", C.c_int32),
+                ("pock_chambolle", C.c_int32), ("bound_cost_scaling", C.c_int32), ("power_iterations", C.c_int32),
+                ("reserved", C.c_int32), ("power_tolerance", C.c_double), ("restart_sufficient", C.c_double),
+                ("restart_necessary", C.c_double), ("restart_artificial", C.c_double),
+                ("residual_balance", C.c_double), ("residual_balance_band", C.c_double)]
+
+
+class GpLpSolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("restarts", C.c_int32), ("reserved", C.c_int32),
+                ("primal_objective", C.c_double), ("dual_objective", C.c_double),
+                ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("gap", C.c_double),
+                ("seconds", C.c_double), ("sigma", C.c_double), ("lambda_", C.c_double),
+                ("bytes_per_iteration", C.c_double)]
+
+
+LP_STATUS = {0: "OPTIMAL", 1: "ITER_LIMIT", 2: "TIME_LIMIT", 3: "NUMERICAL_ERROR", 4: "UNBOUNDED", 5: "INFEASIBLE"}
+
+
 class GpParams(C.Structure):
     _fields_ = [
         ("struct_size", C.c_int32), ("max_iters", C.c_int32), ("max_time", C.c_double),
@@ -334,6 +354,9 @@ def load() -> C.CDLL:
         "gp_mps_read": (C.c_int, [C.c_char_p, C.POINTER(GpMpsModel)]),
         "gp_mps_free": (None, [C.POINTER(GpMpsModel)]),
         "gp_mps_upload": (C.c_int, [C.POINTER(GpMpsModel), C.POINTER(GpDeviceLp)]),
+        "gp_lp_solve_default_params": (None, [C.POINTER(GpLpSolveParams)]),
+        "gp_lp_solve": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpLpSolveParams), vp, vp, vp,
+                                  C.POINTER(GpLpSolveInfo)]),
         "gp_preprocess_batch": (C.c_int, [C.POINTER(GpDeviceLp), C.c_int32, C.POINTER(GpParams),
                                           C.POINTER(GpBatchSummary)]),
         "gp_postprocess_batch": (C.c_int, [vp, vp, vp, C.POINTER(GpBatchSummary), vp, vp, vp]),
@@ -533,6 +556,25 @@ def read_mps(path) -> MpsModel:
     raw = GpMpsModel()
     check(load().gp_mps_read(os.fsencode(path), C.byref(raw)))
     return MpsModel(raw)
+
+
+def lp_solve_params(**overrides) -> GpLpSolveParams:
+    p = GpLpSolveParams()
+    load().gp_lp_solve_default_params(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def solve_lp(lp, params: GpLpSolveParams | None = None, **overrides):
+    """gp_lp_solve on a device LP (DeviceLP or GpDeviceLp): host x, y, z and the info struct."""
+    d = lp.lp if isinstance(lp, DeviceLP) else lp
+    m, n = d.A.rows, d.A.cols
+    p = params if params is not None else lp_solve_params(**overrides)
+    dx, dy, dz = DeviceArray(8 * n), DeviceArray(8 * m), DeviceArray(8 * n)
+    info = GpLpSolveInfo()
+    check(load().gp_lp_solve(C.byref(d), C.byref(p), dx.ptr, dy.ptr, dz.ptr, C.byref(info)))
+    return to_host(dx.ptr, n, np.float64), to_host(dy.ptr, m, np.float64), to_host(dz.ptr, n, np.float64), info
 
 
 def device_lp_to_host(d: GpDeviceLp) -> HostLP:

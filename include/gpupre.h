@@ -709,6 +709,72 @@ int gp_lp_upload_entries(int32_t m, int32_t n, int64_t entries, const int32_t* e
                          const double* AL, const double* AU, const double* l, const double* u,
                          double obj_constant, gp_device_lp* out);
 
+/* ---- FP64 first-order LP solver ---------------------------------------------
+ * The solver the reference hands the reduced LP to: HPR-LP, the restarted
+ * Halpern Peaceman-Rachford method (third_party/hprlp: HPRLP_main_solve /
+ * solve, include/HPRLP.h; parameters and results of include/api/structs.h:
+ * 256-291, 334-358), called from solver_interface/integration.cpp:44-119.
+ * Tolerance, iteration and time limits keep HPR-LP's meaning and defaults
+ * (stop_tol 1e-6, time_limit 1000 s); termination when the relative primal
+ * residual, dual residual and duality gap are all <= tolerance. */
+enum {
+  GP_LP_OPTIMAL = 0,
+  GP_LP_ITERATION_LIMIT = 1,
+  GP_LP_TIME_LIMIT = 2,
+  GP_LP_NUMERICAL_ERROR = 3,
+  GP_LP_UNBOUNDED = 4,  /* only reported for an LP without matrix entries */
+  GP_LP_INFEASIBLE = 5  /* only reported for an LP without matrix entries */
+};
+
+typedef struct gp_lp_solve_params {
+  int32_t struct_size;      /* set by gp_lp_solve_default_params */
+  int32_t max_iterations;   /* max_iter */
+  double tolerance;         /* stop_tol: relative KKT tolerance */
+  double time_limit;        /* seconds */
+  int32_t check_every;      /* check_iter: iterations between termination / restart checks */
+  int32_t ruiz_iterations;  /* use_Ruiz_scaling: equilibration passes (0 = off) */
+  int32_t pock_chambolle;   /* use_Pock_Chambolle_scaling */
+  int32_t bound_cost_scaling; /* use_bc_scaling */
+  int32_t power_iterations; /* limit of the ||A||^2 power iteration */
+  int32_t reserved;
+  double power_tolerance;
+  /* restarts: when the fixed-point residual falls to `restart_sufficient` of
+   * its value at the last restart, or below `restart_necessary` of it while
+   * rising, or after `restart_artificial` of all iterations so far */
+  double restart_sufficient;
+  double restart_necessary;
+  double restart_artificial;
+  /* at a restart sigma = (primal move / dual move) / ||A||, then, when the
+   * relative primal and dual residuals differ by more than the factor
+   * residual_balance_band, times clamp((primal / dual)^residual_balance,
+   * 0.1, 10): a larger sigma (primal step) when the dual residual lags */
+  double residual_balance;
+  double residual_balance_band;
+} gp_lp_solve_params;
+
+typedef struct gp_lp_solve_info {
+  int32_t status;           /* GP_LP_* */
+  int32_t iterations;
+  int32_t restarts;
+  int32_t reserved;
+  double primal_objective;  /* c'x + obj_constant */
+  double dual_objective;
+  double primal_residual;   /* ||Ax - proj_[AL,AU](Ax)|| / (1 + ||b||) */
+  double dual_residual;     /* ||c - A'y - z|| / (1 + ||c||) */
+  double gap;               /* |primal - dual| / (1 + |primal| + |dual|) */
+  double seconds;
+  double sigma;             /* final primal step */
+  double lambda;            /* estimate of ||A||_2^2 of the scaled matrix */
+  double bytes_per_iteration; /* algorithmic HBM bytes of one iteration */
+} gp_lp_solve_info;
+
+void gp_lp_solve_default_params(gp_lp_solve_params* params);
+/* Solve the device LP; x (cols), y (rows), z (cols) are caller-owned device
+ * arrays that receive the solution (any may be NULL).  Convention
+ * c - A'y - z = 0, as postprocess expects. */
+int gp_lp_solve(const gp_device_lp* lp, const gp_lp_solve_params* params, double* x, double* y, double* z,
+                gp_lp_solve_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,3 +52,13 @@ def test_errors_are_status_codes_without_device():
     lib.gp_free_summary(C.byref(s))
     lib.gp_free_summary(C.byref(s))  # idempotent
     lib.gp_free_summary(None)
+
+
+def test_lp_solver_defaults_and_layouts():
+    p = G.lp_solve_params()
+    # HPR-LP's defaults (third_party/hprlp/include/api/structs.h:256-291): stop_tol 1e-6, time_limit 1000 s
+    assert p.struct_size == C.sizeof(G.GpLpSolveParams) == 96
+    assert p.tolerance == 1e-6 and p.time_limit == 1000.0 and p.check_every == 64
+    assert p.ruiz_iterations == 10 and p.pock_chambolle == 1 and p.bound_cost_scaling == 1
+    assert C.sizeof(G.GpLpSolveInfo) == 16 + 9 * 8
+    assert C.sizeof(G.GpMpsModel) == 4 + 4 + 8 + 8 * 8 + 8 + 8
