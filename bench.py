@@ -275,6 +275,104 @@ def learned_leg(seed):
     return out
 
 
+def highs_objective(lp):
+    """FP64 CPU reference solve (HiGHS through SciPy) of min c'x + k, AL <= Ax <= AU, l <= x <= u."""
+    import numpy as np
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+
+    A = sp.csr_matrix((lp.vals, lp.col_idx, lp.row_ptr), shape=(lp.m, lp.n))
+    AL, AU = np.asarray(lp.AL), np.asarray(lp.AU)
+    eq = np.isfinite(AL) & np.isfinite(AU) & (AL == AU)
+    up, lo = np.isfinite(AU) & ~eq, np.isfinite(AL) & ~eq
+    A_ub = sp.vstack([A[up], -A[lo]]).tocsr() if (up.any() or lo.any()) else None
+    b_ub = np.concatenate([AU[up], -AL[lo]]) if A_ub is not None else None
+    A_eq, b_eq = (A[eq], AU[eq]) if eq.any() else (None, None)
+    res = linprog(lp.c, A_ub=A_ub, b_ub=b_ub, A_eq=A_eq, b_eq=b_eq, bounds=np.column_stack([lp.l, lp.u]),
+                  method="highs", options={"primal_feasibility_tolerance": 1e-9, "dual_feasibility_tolerance": 1e-9})
+    return float(res.fun) + float(lp.obj_constant) if res.status == 0 else None
+
+
+def lp_solver_leg(seed, medium=False):
+    """The FP64 first-order solver (gp_lp_solve) and the north star's gate: the
+    presolved LP solved, postsolved, and its objective set against HiGHS on the
+    original LP (config 1); with `medium`, the 200k x 300k LP (config 2) against
+    the solver's own solve of the original."""
+    import numpy as np
+    import gpupre as G
+
+    def one(name, tol):
+        lp = G.generate_workload(name, seed=seed)
+        t = time.perf_counter()
+        ref = highs_objective(lp) if name == "small" else None
+        highs_s = time.perf_counter() - t
+        with G.DeviceLP(lp) as d:
+            x, y, z, info = G.solve_lp(d, tolerance=tol, time_limit=120.0)
+            s = G.preprocess(d, G.parity_params())
+            try:
+                xr, yr, zr, rinfo = G.solve_lp(s.s.reduced_lp, tolerance=tol, time_limit=120.0)
+                xo, _, _ = G.postprocess(s, xr, yr, zr, d)
+                red = (s.reduced_rows, s.reduced_cols)
+            finally:
+                s.free()
+        post = float(np.dot(lp.c, xo)) + float(lp.obj_constant)
+        target = ref if ref is not None else info.primal_objective
+
+        def leg(i):
+            return {"status": G.LP_STATUS[i.status], "iterations": i.iterations, "restarts": i.restarts,
+                    "seconds": round(i.seconds, 3), "objective": i.primal_objective,
+                    "us_per_iteration": round(1e6 * i.seconds / max(1, i.iterations), 1),
+                    "gbps": round(i.bytes_per_iteration * i.iterations / i.seconds / 1e9, 1) if i.seconds else None}
+
+        out = {"rows": lp.m, "cols": lp.n, "nnz": lp.nnz, "kkt_tolerance": tol, "original": leg(info),
+               "reduced": dict(leg(rinfo), rows=red[0], cols=red[1]), "postsolved_objective": post,
+               "objective_error": abs(post - target) / max(1.0, abs(target)),
+               "error_against": "HiGHS on the original LP" if ref is not None else "this solver on the original LP"}
+        if ref is not None:
+            out.update(highs_objective=ref, highs_seconds=round(highs_s, 2),
+                       original_error=abs(info.primal_objective - ref) / max(1.0, abs(ref)))
+        log(f"lp solver {name}: {out}")
+        return out
+
+    res = {"config1_small": one("small", 1e-8)}
+    if medium:
+        res["config2_medium"] = one("medium", 1e-6)
+    return res
+
+
+def mps_ingest_leg(seed):
+    """MPS ingest of the 200k x 300k LP (config 2) written as free-format MPS:
+    host parse (gp_mps_parse) and device assembly (gp_mps_upload: entries to
+    HBM, duplicates summed, A and A^T built), against the host-CSR upload."""
+    import gpupre as G
+
+    lp = G.generate_workload("medium", seed=seed)
+    text = G.write_mps(lp, "medium")
+    t = time.perf_counter()
+    model = G.parse_mps(text)
+    parse_s = time.perf_counter() - t
+    try:
+        G.check(G.load().gp_synchronize())
+        t = time.perf_counter()
+        d = model.upload()
+        G.check(G.load().gp_synchronize())
+        up_s = time.perf_counter() - t
+        entries = model.entries
+    finally:
+        model.free()
+    d.free()
+    t = time.perf_counter()
+    d2 = G.DeviceLP(lp)
+    G.check(G.load().gp_synchronize())
+    csr_s = time.perf_counter() - t
+    d2.free()
+    out = {"rows": lp.m, "cols": lp.n, "entries": int(entries), "text_mb": round(len(text) / 1e6, 1),
+           "parse_s": round(parse_s, 3), "parse_mb_per_s": round(len(text) / parse_s / 1e6, 1),
+           "device_assembly_ms": round(up_s * 1e3, 2), "host_csr_upload_ms": round(csr_s * 1e3, 2)}
+    log(f"mps ingest: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +390,9 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the per-config GPU vs PSLP table")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-recipe verdict leg")
     ap.add_argument("--no-pslp-full", action="store_true", help="skip PSLP on the whole workload")
+    ap.add_argument("--no-lp-solver", action="store_true", help="skip the LP-solver gate leg")
+    ap.add_argument("--lp-medium", action="store_true", help="add the 200k x 300k LP to the solver leg (~1 min)")
+    ap.add_argument("--no-mps", action="store_true", help="skip the MPS ingest leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
@@ -450,6 +551,8 @@ def main():
     learned = None if args.no_learned else learned_leg(args.seed)
     configs = None if args.no_configs else configs_leg(args.seed)
     fp64 = None if (args.no_fp64 or args.workload != "large") else fp64_verdict_leg(args.seed)
+    lp_solver = None if args.no_lp_solver else lp_solver_leg(args.seed, args.lp_medium)
+    mps = None if args.no_mps else mps_ingest_leg(args.seed)
 
     value = nnz / (ms_per_step * 1e-3) / 1e6
     out = {
@@ -465,7 +568,7 @@ def main():
         "postsolve_ms": round(post_ms, 2), "postsolve_records": int(rec_count),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
         "batched": batched, "learned_vs_plain": learned, "configs": configs, "large_fp64_verdict": fp64,
-        "pslp_full_lp": pslp_full,
+        "pslp_full_lp": pslp_full, "lp_solver": lp_solver, "mps_ingest": mps,
     }
     print(json.dumps(out), flush=True)
 

@@ -542,6 +542,58 @@ class MpsModel:
         self.free()
 
 
+def write_mps(lp: HostLP, name: str = "lp") -> bytes:
+    """Free-format MPS text of a host LP (rows R<i>, columns X<j>, numbers in
+    repr form; a column without entries or cost gets a zero-cost card so
+    that it exists for a reader).  Rows are E / L / G with RANGES for the
+    two-sided ones; a free row is an L row with an infinite right-hand side."""
+    import io
+
+    out = io.StringIO()
+    AL, AU = np.asarray(lp.AL), np.asarray(lp.AU)
+    kind = np.where(AL == AU, "E", np.where(np.isfinite(AU), "L", "G"))
+    out.write(f"NAME {name}\nROWS\n N obj\n")
+    out.writelines(f" {k} R{i}\n" for i, k in enumerate(kind))
+    out.write("COLUMNS\n")
+    rows = np.repeat(np.arange(lp.m, dtype=np.int64), np.diff(np.asarray(lp.row_ptr, np.int64)))
+    order = np.lexsort((rows, np.asarray(lp.col_idx)))
+    cols = np.asarray(lp.col_idx)[order]
+    ptr = np.searchsorted(cols, np.arange(lp.n + 1))
+    r_sorted, v_sorted = rows[order], np.asarray(lp.vals)[order]
+    c = np.asarray(lp.c)
+    for j in range(lp.n):
+        if c[j] != 0.0 or ptr[j] == ptr[j + 1]:
+            out.write(f" X{j} obj {float(c[j])!r}\n")
+        out.writelines(f" X{j} R{r} {float(v)!r}\n" for r, v in zip(r_sorted[ptr[j]:ptr[j + 1]],
+                                                                   v_sorted[ptr[j]:ptr[j + 1]]))
+    out.write("RHS\n")
+    for i in range(lp.m):
+        side = AU[i] if kind[i] in ("E", "L") else AL[i]
+        if side != 0.0:
+            out.write(f" rhs R{i} {float(side)!r}\n")
+    two = np.flatnonzero((kind == "L") & np.isfinite(AL))
+    if two.size:
+        out.write("RANGES\n")
+        out.writelines(f" rng R{i} {float(AU[i] - AL[i])!r}\n" for i in two)
+    out.write("BOUNDS\n")
+    l, u = np.asarray(lp.l), np.asarray(lp.u)
+    for j in range(lp.n):
+        if np.isfinite(l[j]) and l[j] == u[j]:
+            out.write(f" FX bnd X{j} {float(l[j])!r}\n")
+            continue
+        if not np.isfinite(l[j]) and not np.isfinite(u[j]):
+            out.write(f" FR bnd X{j}\n")
+            continue
+        if not np.isfinite(l[j]):
+            out.write(f" MI bnd X{j}\n")
+        elif l[j] != 0.0:
+            out.write(f" LO bnd X{j} {float(l[j])!r}\n")
+        if np.isfinite(u[j]):
+            out.write(f" UP bnd X{j} {float(u[j])!r}\n")
+    out.write("ENDATA\n")
+    return out.getvalue().encode()
+
+
 def parse_mps(text) -> MpsModel:
     """gp_mps_parse on an in-memory MPS text (bytes or str)."""
     if isinstance(text, str):

@@ -191,3 +191,23 @@ def test_integer_writer_reads_back_exactly():
     assert np.array_equal(ref.csr_val[order], exp["vals"])
     for k in ("c", "AL", "AU", "l", "u"):
         assert np.array_equal(getattr(ref, k), exp[k]), k
+
+
+def test_package_writer_reads_back():
+    # gpupre.write_mps: structure and infinities exactly; FP64 values to the
+    # digit path's rounding (a few ulp); integer-valued matrix and sides exactly
+    lp = G.generate_workload("small", seed=0)
+    r = M.read_mps(G.write_mps(lp))
+    assert (r.m, r.n) == (lp.m, lp.n)
+    assert np.array_equal(r.row_ptr, lp.row_ptr) and np.array_equal(r.col_idx, lp.col_idx)
+    assert np.allclose(r.csr_val, lp.vals, rtol=1e-14, atol=0)
+    for k in ("AL", "AU", "l", "u", "c"):
+        a, b = getattr(r, k), np.asarray(getattr(lp, k))
+        assert np.array_equal(np.isfinite(a), np.isfinite(b)), k
+        f = np.isfinite(b)
+        assert np.allclose(a[f], b[f], rtol=1e-14, atol=0), k
+    ilp = G.generate(1500, 2000, 3, degree_mode=0, mean_degree=5.0, integral=1)
+    r = M.read_mps(G.write_mps(ilp))
+    assert np.array_equal(r.csr_val, ilp.vals)
+    for k in ("AL", "AU", "l", "u"):  # the recipe's costs are not integers
+        assert np.array_equal(getattr(r, k), np.asarray(getattr(ilp, k))), k
