@@ -2,6 +2,7 @@
 // positions, restore fixed columns, replay the tape in reverse (primal pass,
 // then dual pass) and project column duals from the original model
 // (gpu_postsolve.cu:535-670; record handlers postsolve/postsolve_<name>.cuh).
+#include "gp_fold.hpp"
 #include "gp_tape.hpp"
 
 namespace gp {
@@ -980,9 +981,15 @@ struct PostsolveInputs {
 
 // postsolve_gpu (gpu_postsolve.cu:535-670): scatter, fixed restore, primal
 // replay between structural checkpoints in reverse, dual replay, projection.
+// Device copies of a postsolved point (filled instead of host arrays when
+// the caller continues on the device, e.g. to unfold).
+struct PostsolvedDevice {
+  DVec<double> x, y, z;
+};
+
 void postsolve_run(const PostsolveInputs& in, const double* x, const double* y, const double* z,
                    const gp_device_lp* original, const double* original_l, double* x_out, double* y_out,
-                   double* z_out) {
+                   double* z_out, PostsolvedDevice* keep = nullptr) {
   const int32_t m0 = in.m0, n0 = in.n0;
   if (!original_l && original) original_l = original->l;
   DVec<double> xo(n0 ? n0 : 1), yo(m0 ? m0 : 1), zo(n0 ? n0 : 1);
@@ -1035,6 +1042,12 @@ void postsolve_run(const PostsolveInputs& in, const double* x, const double* y, 
                          in.tape.idx, in.count, n0, protect.data());
     hipLaunchKernelGGL(k_project, dim3(grid_for(n0)), dim3(kBlock), 0, 0, s, protect.data(), n0);
     GP_LAUNCH_CHECK("postsolve projection");
+  }
+  if (keep) {
+    keep->x = std::move(xo);
+    keep->y = std::move(yo);
+    keep->z = std::move(zo);
+    return;
   }
   if (x_out) d2h(x_out, xo.data(), static_cast<size_t>(n0));
   if (y_out) d2h(y_out, yo.data(), static_cast<size_t>(m0));
@@ -1273,6 +1286,40 @@ extern "C" int gp_postprocess(const double* x, const double* y, const double* z,
   return guarded([&] {
     if (!summary || !summary->record) throw std::invalid_argument("summary has no record");
     const RecordView r = record_view(summary->record);
+    const FoldState* fold = static_cast<const FoldState*>(r.fold);
+    if (fold) {
+      // postsolve against the folded LP, then unfold
+      // (postsolve_and_unfold_to_host, folding.cu:211-262)
+      PostsolvedDevice folded;
+      const double *fx = x, *fy = y, *fz = z;
+      if (summary->presolve_enabled) {
+        const DeviceTape& t = *r.tape;
+        PostsolveInputs in{};
+        in.m0 = r.m0;
+        in.n0 = r.n0;
+        in.m1 = r.m1;
+        in.n1 = r.n1;
+        in.rows = r.rows;
+        in.cols = r.cols;
+        in.fixed_idx = r.fixed_idx;
+        in.fixed_val = r.fixed_val;
+        in.fixed_count = r.fixed_count;
+        in.tape = PsTape{t.types.data(), t.istarts.data(), t.vstarts.data(), t.idx.data(), t.vals.data()};
+        in.count = t.count;
+        in.timeline = nullptr;
+        postsolve_run(in, x, y, z, &fold->lp, nullptr, nullptr, nullptr, nullptr, &folded);
+        fx = folded.x.data();
+        fy = folded.y.data();
+        fz = folded.z.data();
+      }
+      DVec<double> ux(static_cast<size_t>(fold->n ? fold->n : 1)), uy(static_cast<size_t>(fold->m ? fold->m : 1)),
+          uz(static_cast<size_t>(fold->n ? fold->n : 1));
+      unfold(*fold, fx, fy, fz, ux.data(), uy.data(), uz.data());
+      if (x_out && fx) d2h(x_out, ux.data(), static_cast<size_t>(fold->n));
+      if (y_out && fy) d2h(y_out, uy.data(), static_cast<size_t>(fold->m));
+      if (z_out && fz) d2h(z_out, uz.data(), static_cast<size_t>(fold->n));
+      return;
+    }
     if (!summary->presolve_enabled) {
       if (x && x_out) d2h(x_out, x, static_cast<size_t>(r.n0));
       if (y && y_out) d2h(y_out, y, static_cast<size_t>(r.m0));

@@ -9,8 +9,10 @@
 // Rules write into one deferred plan over the physical LP; the LP is
 // compacted (rows/cols dropped, A and A^T rebuilt) only at materialize points.
 #include <chrono>
+#include <memory>
 #include <cstdio>
 
+#include "gp_fold.hpp"
 #include "gp_tape.hpp"
 
 namespace gp {
@@ -24,6 +26,7 @@ double now_s() {
 
 // ---- recovery record ---------------------------------------------------------------------
 struct Record {
+  std::unique_ptr<FoldState> fold;  // the folding stage, when it folded the LP
   int32_t m0 = 0, n0 = 0, m1 = 0, n1 = 0;
   DVec<int32_t> rows, cols;  // physical -> original maps
   DVec<int32_t> fixed_idx;
@@ -583,8 +586,17 @@ void validate_lp(const gp_device_lp* lp) {
 
 RecordView record_view(const void* record) {
   const Record* r = static_cast<const Record*>(record);
-  return RecordView{r->m0, r->n0, r->m1, r->n1, r->rows.data(), r->cols.data(), r->fixed_idx.data(),
-                    r->fixed_val.data(), r->fixed_count, &r->tape};
+  return RecordView{r->m0,
+                    r->n0,
+                    r->m1,
+                    r->n1,
+                    r->rows.data(),
+                    r->cols.data(),
+                    r->fixed_idx.data(),
+                    r->fixed_val.data(),
+                    r->fixed_count,
+                    &r->tape,
+                    r->fold && r->fold->applied ? r->fold.get() : nullptr};
 }
 
 }  // namespace gp
@@ -693,10 +705,9 @@ int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary
     out->original_cols = input->A.cols;
     out->presolve_enabled = params->enable_presolve;
     out->folding_applied = 0;
-    // Requested stages this engine does not run: folding (folding.cu:185-209)
-    // and the structure-specific Startup rules (gpu_presolve.cu:2088-2094,
-    // 2361-2454).  The reduction is the one with those stages off.
-    if (params->enable_folding) out->unsupported_stages |= GP_STAGE_FOLDING;
+    // Requested stages this engine does not run: the structure-specific
+    // Startup rules (gpu_presolve.cu:2088-2094, 2361-2454).  The reduction is
+    // the one with those stages off.
     if (params->enable_presolve && params->enable_startup && params->max_iters > 0 &&
         params->enable_structure_specific_rules &&
         (params->enable_column_domination || params->enable_two_row_variable_elimination ||
@@ -706,25 +717,39 @@ int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary
     const double setup_end = now_s();
     out->workspace_setup_seconds = setup_end - t0;
     try {
+      // folding first (run_gpu_presolve_with_folding, folding.cu:173-209):
+      // presolve then runs on the folded LP
+      std::unique_ptr<FoldState> fold;
+      if (params->enable_folding) fold = fold_lp(*input, params->folding_tolerance);
+      if (fold) fold->fill(out->folding);
+      else {
+        out->folding.original_rows = out->folding.folded_rows = input->A.rows;
+        out->folding.original_cols = out->folding.folded_cols = input->A.cols;
+      }
+      out->folding_applied = fold && fold->applied ? 1 : 0;
+      const gp_device_lp* work = out->folding_applied ? &fold->lp : input;
       if (params->enable_presolve) {
-        run_presolve(*input, *params, *out);
+        run_presolve(*work, *params, *out);
       } else {
         // identity record, reduced LP borrows the input (folding.cu:145-170)
         Record* rec = new Record();
         out->record = rec;
-        rec->m0 = rec->m1 = input->A.rows;
-        rec->n0 = rec->n1 = input->A.cols;
-        rec->rows.resize(input->A.rows ? input->A.rows : 1);
-        rec->cols.resize(input->A.cols ? input->A.cols : 1);
-        if (input->A.rows)
-          hipLaunchKernelGGL(k_iota, dim3(grid_for(input->A.rows)), dim3(kBlock), 0, 0, rec->rows.data(), input->A.rows);
-        if (input->A.cols)
-          hipLaunchKernelGGL(k_iota, dim3(grid_for(input->A.cols)), dim3(kBlock), 0, 0, rec->cols.data(), input->A.cols);
-        out->reduced_lp = *input;
-        out->reduced_rows = input->A.rows;
-        out->reduced_cols = input->A.cols;
-        out->reduced_nnz = input->A.nnz;
+        rec->m0 = rec->m1 = work->A.rows;
+        rec->n0 = rec->n1 = work->A.cols;
+        rec->rows.resize(work->A.rows ? work->A.rows : 1);
+        rec->cols.resize(work->A.cols ? work->A.cols : 1);
+        if (work->A.rows)
+          hipLaunchKernelGGL(k_iota, dim3(grid_for(work->A.rows)), dim3(kBlock), 0, 0, rec->rows.data(), work->A.rows);
+        if (work->A.cols)
+          hipLaunchKernelGGL(k_iota, dim3(grid_for(work->A.cols)), dim3(kBlock), 0, 0, rec->cols.data(), work->A.cols);
+        out->reduced_lp = *work;
+        out->reduced_rows = work->A.rows;
+        out->reduced_cols = work->A.cols;
+        out->reduced_nnz = work->A.nnz;
       }
+      // the record keeps the folded LP and the map alive (the reduced LP may
+      // borrow the folded arrays)
+      if (out->folding_applied) static_cast<Record*>(out->record)->fold = std::move(fold);
       kc.flush();
     } catch (...) {
       kc.segments.clear();

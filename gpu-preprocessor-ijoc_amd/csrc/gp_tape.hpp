@@ -100,6 +100,7 @@ struct RecordView {
   const double* fixed_val;
   int64_t fixed_count;
   const DeviceTape* tape;
+  const void* fold;  // FoldState of the folding stage (applied), or null
 };
 RecordView record_view(const void* record);
 

@@ -89,6 +89,15 @@ class GpLpSolveInfo(C.Structure):
 LP_STATUS = {0: "OPTIMAL", 1: "ITER_LIMIT", 2: "TIME_LIMIT", 3: "NUMERICAL_ERROR", 4: "UNBOUNDED", 5: "INFEASIBLE"}
 
 
+class GpFoldResult(C.Structure):
+    _fields_ = [("applied", C.c_int32), ("original_rows", C.c_int32), ("original_cols", C.c_int32),
+                ("folded_rows", C.c_int32), ("folded_cols", C.c_int32), ("rounds", C.c_int32),
+                ("refinement_ok", C.c_int32), ("final_row_colors", C.c_int32), ("final_col_colors", C.c_int32),
+                ("reserved", C.c_int32), ("seconds", C.c_double), ("folded_lp", GpDeviceLp),
+                ("row_color_id", C.c_void_p), ("col_color_id", C.c_void_p), ("row_scale", C.c_void_p),
+                ("col_scale", C.c_void_p), ("state", C.c_void_p)]
+
+
 class GpParams(C.Structure):
     _fields_ = [
         ("struct_size", C.c_int32), ("max_iters", C.c_int32), ("max_time", C.c_double),
@@ -183,6 +192,7 @@ class GpSummary(C.Structure):
         ("record", C.c_void_p), ("trace_count", C.c_int32), ("trace", C.POINTER(GpRoundTrace)),
         ("kernel_ms", C.c_double * GP_KT_COUNT), ("kernel_launches", C.c_int64 * GP_KT_COUNT),
         ("kernel_bytes", C.c_double * GP_KT_COUNT), ("unsupported_stages", C.c_int32),
+        ("folding", GpFoldResult),
     ]
 
 
@@ -355,6 +365,9 @@ def load() -> C.CDLL:
         "gp_mps_free": (None, [C.POINTER(GpMpsModel)]),
         "gp_mps_upload": (C.c_int, [C.POINTER(GpMpsModel), C.POINTER(GpDeviceLp)]),
         "gp_lp_solve_default_params": (None, [C.POINTER(GpLpSolveParams)]),
+        "gp_fold": (C.c_int, [C.POINTER(GpDeviceLp), C.c_double, C.POINTER(GpFoldResult)]),
+        "gp_unfold": (C.c_int, [C.POINTER(GpFoldResult), vp, vp, vp, vp, vp, vp]),
+        "gp_free_fold": (None, [C.POINTER(GpFoldResult)]),
         "gp_lp_solve": (C.c_int, [C.POINTER(GpDeviceLp), C.POINTER(GpLpSolveParams), vp, vp, vp,
                                   C.POINTER(GpLpSolveInfo)]),
         "gp_preprocess_batch": (C.c_int, [C.POINTER(GpDeviceLp), C.c_int32, C.POINTER(GpParams),
@@ -610,6 +623,46 @@ def read_mps(path) -> MpsModel:
     return MpsModel(raw)
 
 
+class Fold:
+    """gp_fold: an LP folded by colour refinement (device memory owned here)."""
+
+    def __init__(self, lp, tolerance: float = 1e-8):
+        d = lp.lp if isinstance(lp, DeviceLP) else lp
+        self.r = GpFoldResult()
+        check(load().gp_fold(C.byref(d), float(tolerance), C.byref(self.r)))
+        self.m, self.n = d.A.rows, d.A.cols
+
+    @property
+    def applied(self) -> bool:
+        return bool(self.r.applied)
+
+    def classes(self):
+        """(row class, column class) host arrays, 1-based (applied folds only)."""
+        return to_host(self.r.row_color_id, self.m, np.int32), to_host(self.r.col_color_id, self.n, np.int32)
+
+    def scales(self):
+        return to_host(self.r.row_scale, self.m, np.float64), to_host(self.r.col_scale, self.n, np.float64)
+
+    def folded(self) -> HostLP:
+        return device_lp_to_host(self.r.folded_lp)
+
+    def unfold(self, x_red, y_red, z_red):
+        """Reduced host vectors in, original-order host vectors out."""
+        dx, dy, dz = (DeviceArray.from_host(np.asarray(v, np.float64)) for v in (x_red, y_red, z_red))
+        x, y, z = np.empty(self.n), np.empty(self.m), np.empty(self.n)
+        check(load().gp_unfold(C.byref(self.r), dx.ptr, dy.ptr, dz.ptr, _ptr(x), _ptr(y), _ptr(z)))
+        return x, y, z
+
+    def free(self):
+        load().gp_free_fold(C.byref(self.r))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
 def lp_solve_params(**overrides) -> GpLpSolveParams:
     p = GpLpSolveParams()
     load().gp_lp_solve_default_params(C.byref(p))
@@ -710,9 +763,12 @@ def postprocess(summary: Summary, x: np.ndarray, y: np.ndarray, z: np.ndarray, o
     rec = GpRecordSizes()
     check(load().gp_record_get_sizes(C.byref(summary.s), C.byref(rec)))
     dx, dy, dz = (DeviceArray.from_host(np.asarray(v, dtype=np.float64)) for v in (x, y, z))
-    xo = np.empty(rec.n0)
-    yo = np.empty(rec.m0)
-    zo = np.empty(rec.n0)
+    # original order: the folding stage's input when it folded, else the presolve's
+    f = summary.s.folding
+    m0, n0 = (f.original_rows, f.original_cols) if summary.s.folding_applied else (rec.m0, rec.n0)
+    xo = np.empty(n0)
+    yo = np.empty(m0)
+    zo = np.empty(n0)
     check(load().gp_postprocess(dx.ptr, dy.ptr, dz.ptr, C.byref(summary.s),
                                 C.byref(original.lp) if original is not None else None,
                                 _ptr(xo), _ptr(yo), _ptr(zo)))

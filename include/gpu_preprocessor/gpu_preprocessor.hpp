@@ -20,10 +20,11 @@
 //    record; safe to call repeatedly.
 //  * Errors: std::invalid_argument, std::length_error, std::runtime_error as in
 //    the reference; infeasible / unbounded are flags, not errors.
-//  * Folding (enable_folding) and the structure-specific Startup rules are not
-//    part of this engine: the fields are accepted, the reduction is the one
-//    with those stages off, and Summary::unsupported_stages reports a request
-//    for them (GP_STAGE_* bits); folding.applied stays false.
+//  * Folding (enable_folding, on by default as in the reference) runs first;
+//    presolve then runs on the folded LP and postprocess unfolds.  The
+//    structure-specific Startup rules are not part of this engine: the fields
+//    are accepted, the reduction is the one with them off, and
+//    Summary::unsupported_stages reports a request for them.
 #pragma once
 
 #include <cstddef>
@@ -161,9 +162,10 @@ struct GpuPresolveSummary {
 
 namespace folding {
 
-// Folding profile and run summary (folding_structs.hpp:15-92).  This engine
-// does not fold: applied stays false, the folded dimensions equal the
-// original ones and every timer is zero.
+// Folding profile and run summary (folding_structs.hpp:15-92), filled from
+// the summary's folding stage: total time, rounds, refinement outcome and
+// class counts (the finer per-phase timers stay zero), and, when applied, the
+// map and the folded LP, both owned by the summary.
 struct FoldingRefinementProfile {
   double total_seconds = 0.0;
   double init_color_seconds = 0.0;
@@ -273,10 +275,13 @@ struct FoldingPipelineSummary {
     workspace_release_seconds = raw_.workspace_release_seconds;
     presolve_enabled = raw_.presolve_enabled != 0;
     folding_applied = raw_.folding_applied != 0;
-    original_rows = folded_rows = raw_.original_rows;
-    original_cols = folded_cols = raw_.original_cols;
-    presolve.original_rows = raw_.original_rows;
-    presolve.original_cols = raw_.original_cols;
+    original_rows = raw_.original_rows;
+    original_cols = raw_.original_cols;
+    // presolve ran on the folded LP when folding applied
+    folded_rows = folding_applied ? raw_.folding.folded_rows : raw_.original_rows;
+    folded_cols = folding_applied ? raw_.folding.folded_cols : raw_.original_cols;
+    presolve.original_rows = folded_rows;
+    presolve.original_cols = folded_cols;
     presolve.reduced_rows = raw_.reduced_rows;
     presolve.reduced_cols = raw_.reduced_cols;
     presolve.reduced_nnz = raw_.reduced_nnz;
@@ -288,16 +293,31 @@ struct FoldingPipelineSummary {
     presolve.has_infeasible = raw_.has_infeasible != 0;
     presolve.has_unbounded = raw_.has_unbounded != 0;
     presolve.record.device = raw_.record ? &raw_ : nullptr;
-    presolve.record.m0 = raw_.original_rows;
-    presolve.record.n0 = raw_.original_cols;
+    presolve.record.m0 = folded_rows;
+    presolve.record.n0 = folded_cols;
     presolve.record.m1 = raw_.reduced_rows;
     presolve.record.n1 = raw_.reduced_cols;
     presolve.record.obj_constant_new = raw_.reduced_lp.obj_constant;
     presolve.reduced_lp = *reinterpret_cast<const presolve::LPInfoGpu*>(&raw_.reduced_lp);
     presolve.owns_reduced_lp = raw_.owns_reduced_lp != 0;
     folding = FoldingRunSummary{};
-    folding.original_rows = folding.folded_rows = raw_.original_rows;
-    folding.original_cols = folding.folded_cols = raw_.original_cols;
+    const gp_fold_result& f = raw_.folding;
+    folding.applied = f.applied != 0;
+    folding.original_rows = raw_.original_rows;
+    folding.original_cols = raw_.original_cols;
+    folding.folded_rows = folded_rows;
+    folding.folded_cols = folded_cols;
+    folding.profile.total_seconds = f.seconds;
+    folding.profile.refinement.rounds = f.rounds;
+    folding.profile.refinement.refinement_ok = f.refinement_ok != 0;
+    folding.profile.refinement.final_row_colors = f.final_row_colors;
+    folding.profile.refinement.final_col_colors = f.final_col_colors;
+    if (folding.applied) {
+      // device arrays owned by the summary's record (freed by free_summary)
+      folding.map = FoldingMapDevice{f.original_rows, f.original_cols, f.folded_rows, f.folded_cols,
+                                     f.row_color_id,  f.col_color_id,  f.row_scale,   f.col_scale, false};
+      folding.folded_lp = *reinterpret_cast<const presolve::LPInfoGpu*>(&f.folded_lp);
+    }
     unsupported_stages = raw_.unsupported_stages;
   }
 

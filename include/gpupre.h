@@ -119,7 +119,7 @@ typedef struct gp_device_lp {
 } gp_device_lp;
 
 /* PresolveParams with the reference's field names and defaults.  Booleans are
- * int32 (0/1).  Fields of rules this engine does not build (folding and the
+ * int32 (0/1).  Fields of rules this engine does not build (the
  * structure-specific Startup rules) are accepted and ignored. */
 typedef struct gp_params {
   int32_t struct_size; /* sizeof(gp_params); set by gp_default_params */
@@ -273,9 +273,10 @@ enum {
   GP_GROUP_PARALLEL = 6          /* parallel rows, parallel columns            */
 };
 
-/* Stages of the reference's preprocess() this engine does not build. */
+/* Stages of the reference's preprocess() this engine does not build.  (Bit 1,
+ * folding, is built: it is never set.) */
 enum {
-  GP_STAGE_FOLDING = 1,                  /* enable_folding (folding/folding.cu)       */
+  GP_STAGE_FOLDING = 1,
   GP_STAGE_STRUCTURE_SPECIFIC_RULES = 2  /* enable_structure_specific_rules with one of
                                             its sub-rules on (gpu_presolve.cu:2361-2454) */
 };
@@ -283,6 +284,31 @@ enum {
 /* FoldingPipelineSummary + GpuPresolveSummary, flattened.  Value semantics:
  * gp_preprocess fills a caller-provided struct; gp_free_summary releases its
  * device memory and zeroes it (idempotent). */
+/* Folding (FoldingRunSummary, cpp/include/gpu_preprocessor/folding/folding_structs.hpp:
+ * 64-92; run_folding, cpp/src/folding/folding.cu:69-136): the LP folded by
+ * colour refinement -- one row per row class and one column per column
+ * class -- and the map back.  Arrays are device memory owned by the result
+ * (gp_free_fold) or, inside a gp_summary, by the summary. */
+typedef struct gp_fold_result {
+  int32_t applied;
+  int32_t original_rows;
+  int32_t original_cols;
+  int32_t folded_rows;
+  int32_t folded_cols;
+  int32_t rounds;           /* refinement rounds run */
+  int32_t refinement_ok;    /* 0: stopped because both sides kept > 80% of their lines */
+  int32_t final_row_colors;
+  int32_t final_col_colors;
+  int32_t reserved;
+  double seconds;
+  gp_device_lp folded_lp;   /* valid when applied */
+  int32_t* row_color_id;    /* [original_rows] 1-based class of each row (when applied) */
+  int32_t* col_color_id;    /* [original_cols] */
+  double* row_scale;        /* [original_rows] 1 / class size */
+  double* col_scale;        /* [original_cols] */
+  void* state;              /* owner handle (NULL inside a gp_summary) */
+} gp_fold_result;
+
 typedef struct gp_summary {
   double preprocessing_seconds;
   double workspace_setup_seconds;
@@ -312,6 +338,8 @@ typedef struct gp_summary {
   int32_t unsupported_stages; /* GP_STAGE_* bits: stages the parameters requested
                                  that this engine does not run (the reduction
                                  is then the one with those stages off) */
+  gp_fold_result folding;     /* the folding stage (enable_folding); presolve ran
+                                 on folding.folded_lp when folding.applied */
 } gp_summary;
 
 /* Batched presolve (no reference counterpart; the reference presolves one LP
@@ -660,6 +688,17 @@ int gp_lp_upload(int32_t m, int32_t n, gp_offset nnz, const gp_offset* row_ptr,
                  const double* AL, const double* AU, const double* l, const double* u,
                  double obj_constant, gp_device_lp* out);
 void gp_lp_free(gp_device_lp* lp);
+
+/* ---- folding ----------------------------------------------------------------
+ * run_folding (cpp/include/gpu_preprocessor/folding/folding.hpp:16-18) with
+ * the folding tolerance; gp_unfold is unfold_solution_device_to_host
+ * (folding.hpp:38-41): reduced x, y, z device arrays in, original-order host
+ * arrays out (x copied, y and z scaled by 1 / class size); gp_free_fold is
+ * free_folded_lp. */
+int gp_fold(const gp_device_lp* lp, double tolerance, gp_fold_result* out);
+int gp_unfold(const gp_fold_result* fold, const double* x_red, const double* y_red, const double* z_red,
+              double* x, double* y, double* z);
+void gp_free_fold(gp_fold_result* fold);
 
 /* ---- MPS ingest -------------------------------------------------------------
  * An MPS file as mps_reader::read_mps reads it

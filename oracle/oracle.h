@@ -118,6 +118,21 @@ void or_mlp_loss_grad(const float* params, const float* x, const float* y, const
 void or_mlp_adam_step(float* params, const float* grad, float* m, float* v, int32_t t, float lr, float beta1,
                       float beta2, float eps);
 
+/* LP folding by colour refinement (cpp/src/folding/, oracle_fold.c). */
+typedef struct or_fold {
+  int32_t applied, refinement_ok, rounds;
+  int32_t m, n, mf, nf;
+  int32_t* row_class; /* [m] 1-based */
+  int32_t* col_class; /* [n] 1-based */
+  double* row_scale;  /* [m] when applied */
+  double* col_scale;  /* [n] when applied */
+  gp_device_lp lp;    /* folded LP (host arrays) when applied */
+} or_fold;
+void or_fold_lp(const gp_device_lp* lp, double tol, or_fold* out);
+void or_fold_free(or_fold* f);
+void or_unfold(const or_fold* f, const double* x_red, const double* y_red, const double* z_red, double* x,
+               double* y, double* z);
+
 #ifdef __cplusplus
 }
 #endif

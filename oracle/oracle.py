@@ -77,6 +77,9 @@ def load():
             "or_mlp_forward": (None, [vp, vp, C.c_int32, vp]),
             "or_mlp_loss_grad": (None, [vp, vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]),
             "or_mlp_adam_step": (None, [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]),
+            "or_fold_lp": (None, [C.POINTER(G.GpDeviceLp), C.c_double, C.POINTER(OrFold)]),
+            "or_fold_free": (None, [C.POINTER(OrFold)]),
+            "or_unfold": (None, [C.POINTER(OrFold), vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -96,6 +99,13 @@ def _arr(ptr, count, dtype):
         return np.empty(count, dtype=dtype)
     ct = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
     return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+
+
+class OrFold(C.Structure):
+    _fields_ = [("applied", C.c_int32), ("refinement_ok", C.c_int32), ("rounds", C.c_int32), ("m", C.c_int32),
+                ("n", C.c_int32), ("mf", C.c_int32), ("nf", C.c_int32), ("row_class", C.c_void_p),
+                ("col_class", C.c_void_p), ("row_scale", C.c_void_p), ("col_scale", C.c_void_p),
+                ("lp", G.GpDeviceLp)]
 
 
 class HostLPView:
@@ -154,6 +164,49 @@ class Result:
             self.free()
         except Exception:
             pass
+
+
+class FoldResult:
+    """or_fold_lp: the restatement of the reference's folding."""
+
+    def __init__(self, lp: "G.HostLP", tolerance: float = 1e-8):
+        view = HostLPView(lp)
+        self._f = OrFold()
+        load().or_fold_lp(C.byref(view.lp), float(tolerance), C.byref(self._f))
+        f = self._f
+        self.applied, self.refinement_ok, self.rounds = bool(f.applied), bool(f.refinement_ok), f.rounds
+        self.m, self.n, self.mf, self.nf = f.m, f.n, f.mf, f.nf
+        have = f.m > 0 and f.n > 0
+        self.row_class = _arr(f.row_class, f.m if have else 0, np.int32)
+        self.col_class = _arr(f.col_class, f.n if have else 0, np.int32)
+        self.row_scale = _arr(f.row_scale, f.m if self.applied else 0, np.float64)
+        self.col_scale = _arr(f.col_scale, f.n if self.applied else 0, np.float64)
+        self.folded = None
+        if self.applied:
+            d = f.lp
+            mf, nf, nnz = d.A.rows, d.A.cols, d.A.nnz
+            self.folded = G.HostLP(mf, nf, _arr(d.A.row_ptr, mf + 1, G.NP_OFFSET), _arr(d.A.col_idx, nnz, np.int32),
+                                   _arr(d.A.vals, nnz, np.float64), _arr(d.c, nf, np.float64),
+                                   _arr(d.l, nf, np.float64), _arr(d.u, nf, np.float64), _arr(d.AL, mf, np.float64),
+                                   _arr(d.AU, mf, np.float64), float(d.obj_constant))
+            self.folded.at = (_arr(d.AT.row_ptr, nf + 1, G.NP_OFFSET), _arr(d.AT.col_idx, nnz, np.int32),
+                              _arr(d.AT.vals, nnz, np.float64))
+
+    def unfold(self, x_red, y_red, z_red):
+        x, y, z = np.empty(self.n), np.empty(self.m), np.empty(self.n)
+        xr, yr, zr = (np.ascontiguousarray(v, dtype=np.float64) for v in (x_red, y_red, z_red))
+        load().or_unfold(C.byref(self._f), _p(xr), _p(yr), _p(zr), _p(x), _p(y), _p(z))
+        return x, y, z
+
+    def __del__(self):
+        try:
+            load().or_fold_free(C.byref(self._f))
+        except Exception:
+            pass
+
+
+def fold(lp: "G.HostLP", tolerance: float = 1e-8) -> FoldResult:
+    return FoldResult(lp, tolerance)
 
 
 def presolve(lp: "G.HostLP", params) -> Result:
