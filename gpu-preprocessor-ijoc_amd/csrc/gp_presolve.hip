@@ -13,6 +13,7 @@
 #include <cstdio>
 
 #include "gp_fold.hpp"
+#include "gp_startup.hpp"
 #include "gp_tape.hpp"
 
 namespace gp {
@@ -392,6 +393,43 @@ struct Scheduler {
   }
 };
 
+// The Startup projection batch (gpu_presolve.cu:2361-2410): column
+// domination, two-row elimination and zero-cost products share one
+// deletion-only plan, created when the first of them applies; each analysis
+// after that reads the plan's masks.  The plan is applied once.
+void startup_batch(Scheduler& s) {
+  static const int32_t order[3] = {GP_STARTUP_COLUMN_DOMINATION, GP_STARTUP_TWO_ROW_ELIMINATION,
+                                   GP_STARTUP_ZERO_COST_PRODUCTS};
+  const gp_params& p = *s.p;
+  const int enabled[3] = {p.enable_column_domination, p.enable_two_row_variable_elimination,
+                          p.enable_zero_cost_product_variables};
+  gp_plan plan;
+  bool allocated = false;
+  int32_t flags = 0;
+  auto get_plan = [&]() -> gp_plan* {
+    if (!allocated) {
+      plan_create(s.cur.lp, plan);
+      if (p.record_postsolve_tape) plan.tape = new DeviceTape();
+      allocated = true;
+    }
+    return &plan;
+  };
+  try {
+    for (int k = 0; k < 3; ++k) {
+      if (!enabled[k] || s.terminal || s.time_exceeded()) continue;
+      flags |= startup_rule(order[k], s.cur.lp, p, allocated ? &plan : nullptr, get_plan).primal_only;
+    }
+    if (allocated && plan.has_change) {
+      apply_plan(s.cur, plan, *s.r, false);
+      s.out->primal_only_reductions |= flags;
+    }
+    if (allocated) plan_destroy(plan);
+  } catch (...) {
+    if (allocated) plan_destroy(plan);
+    throw;
+  }
+}
+
 bool good_progress(int64_t before, int64_t after, int64_t minimum) {
   return before >= minimum && after <= before - minimum;
 }
@@ -420,7 +458,11 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
   s.out = &out;
   s.start = now_s();
   try {
-    if (p.enable_startup && p.max_iters > 0 && !s.time_exceeded()) out.iterations++;
+    // Startup (gpu_presolve.cu:2361-2454): the structural rules, one iteration
+    if (p.enable_startup && p.max_iters > 0 && !s.time_exceeded()) {
+      if (p.enable_structure_specific_rules) startup_batch(s);
+      out.iterations++;
+    }
     int phase = GP_PHASE_CLEANUP;
     const int64_t min_dec = 100;
     int bound_only_rounds = 0;
@@ -705,13 +747,12 @@ int gp_preprocess(const gp_device_lp* input, const gp_params* params, gp_summary
     out->original_cols = input->A.cols;
     out->presolve_enabled = params->enable_presolve;
     out->folding_applied = 0;
-    // Requested stages this engine does not run: the structure-specific
-    // Startup rules (gpu_presolve.cu:2088-2094, 2361-2454).  The reduction is
-    // the one with those stages off.
+    // Requested stages this engine does not run: the piecewise-linear
+    // reformulation branches of Startup (gpu_presolve.cu:2412-2449).  The
+    // reduction is the one with those branches off.
     if (params->enable_presolve && params->enable_startup && params->max_iters > 0 &&
         params->enable_structure_specific_rules &&
-        (params->enable_column_domination || params->enable_two_row_variable_elimination ||
-         params->enable_zero_cost_product_variables || params->enable_piecewise_linear_reformulation_variable_pair ||
+        (params->enable_piecewise_linear_reformulation_variable_pair ||
          params->enable_piecewise_linear_reformulation_absolute_value))
       out->unsupported_stages |= GP_STAGE_STRUCTURE_SPECIFIC_RULES;
     const double setup_end = now_s();

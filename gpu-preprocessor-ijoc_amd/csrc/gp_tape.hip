@@ -140,6 +140,20 @@ __global__ void k_append_global(TapeView out, const int32_t* types, const gp_off
         for (int32_t t = 0; t < support && 4 + t < size; ++t) o[4 + t] = map_row(o[4 + t], rows, m);
       }
       break;
+    case GP_RED_FME_COL:
+      /* [K, column, K rows, N, L, (orient, len) x L, N columns] */
+      if (size >= 4) {
+        const int32_t K = o[0];
+        o[1] = map_col(o[1], cols, n);
+        for (int32_t t = 0; t < K && 2 + t < size; ++t) o[2 + t] = map_row(o[2 + t], rows, m);
+        const int64_t sp = 2 + (int64_t)K;
+        if (sp + 1 < size) {
+          const int32_t N = o[sp], L = o[sp + 1];
+          const int64_t first = sp + 2 + 2 * (int64_t)L;
+          for (int32_t t = 0; t < N && first + t < size; ++t) o[first + t] = map_col(o[first + t], cols, n);
+        }
+      }
+      break;
     default:
       break;
   }

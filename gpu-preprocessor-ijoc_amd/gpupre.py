@@ -28,6 +28,13 @@ RULES = [
     "parallel_rows", "parallel_columns",
 ]
 RULE = {name: i for i, name in enumerate(RULES)}
+# structural Startup rules (GP_STARTUP_*), in the order Startup runs them
+STARTUP_RULES = ["column_domination", "two_row_elimination", "zero_cost_products"]
+STARTUP_RULE = {name: i for i, name in enumerate(STARTUP_RULES)}
+# GP_PRIMAL_ONLY_* bits of gp_summary.primal_only_reductions
+PRIMAL_ONLY_COLUMN_DOMINATION = 1
+PRIMAL_ONLY_PROJECTED_AUXILIARY = 2
+PRIMAL_ONLY_VARIABLE_PAIR = 4
 # postsolve record types (the GP_RED_* enum of gpupre.h)
 RED_FIXED_VARIABLE = 0
 RED_INFINITE_BOUND_DUAL_FIX = 1
@@ -192,7 +199,7 @@ class GpSummary(C.Structure):
         ("record", C.c_void_p), ("trace_count", C.c_int32), ("trace", C.POINTER(GpRoundTrace)),
         ("kernel_ms", C.c_double * GP_KT_COUNT), ("kernel_launches", C.c_int64 * GP_KT_COUNT),
         ("kernel_bytes", C.c_double * GP_KT_COUNT), ("unsupported_stages", C.c_int32),
-        ("folding", GpFoldResult),
+        ("folding", GpFoldResult), ("primal_only_reductions", C.c_int32),
     ]
 
 
@@ -335,6 +342,8 @@ def load() -> C.CDLL:
                                            C.POINTER(C.c_int32)]),
         "gp_rule_apply": (C.c_int, [C.c_int32, C.POINTER(GpPlan), C.POINTER(GpDeviceLp), C.POINTER(GpStats),
                                      C.POINTER(GpParams)]),
+        "gp_startup_rule": (C.c_int, [C.c_int32, C.POINTER(GpDeviceLp), C.POINTER(GpParams), C.POINTER(GpPlan),
+                                       C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gp_plan_tape_sizes": (C.c_int, [C.POINTER(GpPlan), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64)]),
         "gp_plan_tape_copy": (C.c_int, [C.POINTER(GpPlan)] + [vp] * 6),

@@ -114,6 +114,10 @@ struct PresolveRecordGpu {
   std::vector<double> fixed_val;
   double obj_constant_old = 0.0;
   double obj_constant_new = 0.0;
+  // postsolve of these reductions restores the primal point only
+  bool has_primal_only_piecewise_linear_reformulation_variable_pair_reduction = false;
+  bool has_primal_only_column_domination_reduction = false;
+  bool has_primal_only_projected_auxiliary_reduction = false;
   PostsolveTape tape;
 };
 
@@ -298,6 +302,12 @@ struct FoldingPipelineSummary {
     presolve.record.m1 = raw_.reduced_rows;
     presolve.record.n1 = raw_.reduced_cols;
     presolve.record.obj_constant_new = raw_.reduced_lp.obj_constant;
+    presolve.record.has_primal_only_piecewise_linear_reformulation_variable_pair_reduction =
+        (raw_.primal_only_reductions & GP_PRIMAL_ONLY_VARIABLE_PAIR) != 0;
+    presolve.record.has_primal_only_column_domination_reduction =
+        (raw_.primal_only_reductions & GP_PRIMAL_ONLY_COLUMN_DOMINATION) != 0;
+    presolve.record.has_primal_only_projected_auxiliary_reduction =
+        (raw_.primal_only_reductions & GP_PRIMAL_ONLY_PROJECTED_AUXILIARY) != 0;
     presolve.reduced_lp = *reinterpret_cast<const presolve::LPInfoGpu*>(&raw_.reduced_lp);
     presolve.owns_reduced_lp = raw_.owns_reduced_lp != 0;
     folding = FoldingRunSummary{};

@@ -340,7 +340,17 @@ typedef struct gp_summary {
                                  is then the one with those stages off) */
   gp_fold_result folding;     /* the folding stage (enable_folding); presolve ran
                                  on folding.folded_lp when folding.applied */
+  int32_t primal_only_reductions; /* GP_PRIMAL_ONLY_* bits: the record's
+                                 has_primal_only_*_reduction flags
+                                 (presolve_structs.hpp:362-365) */
 } gp_summary;
+
+/* Reductions whose postsolve restores the primal point only. */
+enum {
+  GP_PRIMAL_ONLY_COLUMN_DOMINATION = 1,
+  GP_PRIMAL_ONLY_PROJECTED_AUXILIARY = 2,
+  GP_PRIMAL_ONLY_VARIABLE_PAIR = 4
+};
 
 /* Batched presolve (no reference counterpart; the reference presolves one LP
  * per call).  The instances are presolved together as their block-diagonal
@@ -592,6 +602,35 @@ int gp_update_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan*
                          const uint8_t* old_keep_row, const uint8_t* old_keep_col, int32_t* accepted);
 int gp_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp,
                   const gp_stats* stats, const gp_params* params);
+
+/* Structural Startup rules that share one deletion-only plan, in the order
+ * Startup runs them (gpu_presolve.cu:2361-2410):
+ *   GP_STARTUP_COLUMN_DOMINATION   analyze_column_domination +
+ *     build_column_domination_plan (rule_column_domination.cu:244-377):
+ *     on a unit set-covering LP (rows 1 <= a x, entries 1, [0, 1] columns
+ *     with positive integer costs, a cost-one column in every row), columns
+ *     with c_j > nnz(A[:, j]) are fixed to 0 (keep_col 0, new_l = new_u = 0);
+ *   GP_STARTUP_TWO_ROW_ELIMINATION analyze_ / build_two_row_variable_elimination
+ *     (rule_two_row_variable_elimination.cu:520-641): a zero-cost bounded
+ *     column in exactly two short one-sided rows whose envelopes certify it
+ *     is projected out with both rows (FmeCol record);
+ *   GP_STARTUP_ZERO_COST_PRODUCTS  analyze_ / build_zero_cost_product_variables
+ *     (rule_zero_cost_product_variables.cu:436-567): a zero-cost [0, 1]
+ *     column z in exactly the rows z >= x + y - 1, z <= x, z <= y of binary
+ *     boxes x, y is projected out with them (FmeCol record).
+ * With pending = 1 the analysis reads the plan's masks and vectors (the
+ * `pending` plan of the reference); with pending = 0 it reads `lp` alone.
+ * When the rule is applicable and `plan` is given (device plan), the records
+ * go to the plan's tape (if any) and the deletions into its masks.
+ * *candidate_count and *applicable receive the analysis result (either may
+ * be NULL).  The rules run only when their enable_* field is set. */
+enum {
+  GP_STARTUP_COLUMN_DOMINATION = 0,
+  GP_STARTUP_TWO_ROW_ELIMINATION = 1,
+  GP_STARTUP_ZERO_COST_PRODUCTS = 2
+};
+int gp_startup_rule(int32_t rule, const gp_device_lp* lp, const gp_params* params, gp_plan* plan,
+                    int32_t pending, int32_t* candidate_count, int32_t* applicable);
 /* Export the plan tape (local indices). */
 /* append_compact_fixed_variable_tape_from_device (fixed_variable_tape.cu:141-327):
  * FixedVariable records (idx [col, kept rows...], vals [v, c, coefficients...],

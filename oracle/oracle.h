@@ -64,6 +64,11 @@ void or_parallel_row_hashes(const gp_csr* A, const uint8_t* keep_row, const uint
                             uint64_t* out);
 
 /* Rules. */
+/* Structural Startup rules (oracle_startup.c): gp_startup_rule on a host
+ * plan.  Returns 1 when applicable, 0 when not, -1 on a bad argument. */
+int or_startup_rule(int32_t rule, const gp_device_lp* lp, const gp_params* p, gp_plan* plan, int pending,
+                    int32_t* count);
+
 int or_rule_apply(int32_t rule, gp_plan* plan, const gp_device_lp* lp, const gp_stats* stats,
                   const gp_params* params);
 
@@ -92,6 +97,7 @@ typedef struct or_result {
   double obj_constant_delta;
   int32_t trace_count, trace_cap;
   gp_round_trace* trace;
+  int32_t primal_only; /* GP_PRIMAL_ONLY_* bits */
 } or_result;
 
 int or_presolve(const gp_device_lp* lp, const gp_params* params, or_result* out);

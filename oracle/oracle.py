@@ -34,7 +34,7 @@ class OrResult(C.Structure):
                 ("stopped_at_iteration_limit", C.c_int32), ("has_change", C.c_int32),
                 ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32),
                 ("obj_constant_delta", C.c_double), ("trace_count", C.c_int32), ("trace_cap", C.c_int32),
-                ("trace", C.POINTER(G.GpRoundTrace))]
+                ("trace", C.POINTER(G.GpRoundTrace)), ("primal_only", C.c_int32)]
 
 
 _lib = None
@@ -79,6 +79,8 @@ def load():
             "or_mlp_adam_step": (None, [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]),
             "or_fold_lp": (None, [C.POINTER(G.GpDeviceLp), C.c_double, C.POINTER(OrFold)]),
             "or_fold_free": (None, [C.POINTER(OrFold)]),
+            "or_startup_rule": (C.c_int, [C.c_int32, C.POINTER(G.GpDeviceLp), C.POINTER(G.GpParams),
+                                          C.POINTER(G.GpPlan), C.c_int, C.POINTER(C.c_int32)]),
             "or_unfold": (None, [C.POINTER(OrFold), vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -143,6 +145,7 @@ class Result:
         self.has_infeasible = r.has_infeasible
         self.has_unbounded = r.has_unbounded
         self.obj_constant_delta = r.obj_constant_delta
+        self.primal_only = r.primal_only
         self.trace = [G.GpRoundTrace.from_buffer_copy(r.trace[i]) for i in range(r.trace_count)]
         d = r.reduced
         m, n, nnz = d.A.rows, d.A.cols, d.A.nnz
@@ -207,6 +210,16 @@ class FoldResult:
 
 def fold(lp: "G.HostLP", tolerance: float = 1e-8) -> FoldResult:
     return FoldResult(lp, tolerance)
+
+
+def startup_analyze(lp: "G.HostLP", rule, params):
+    """or_startup_rule without a plan (analysis only): (applicable, count)."""
+    view = HostLPView(lp)
+    rid = G.STARTUP_RULE[rule] if isinstance(rule, str) else int(rule)
+    count = C.c_int32()
+    rc = load().or_startup_rule(rid, C.byref(view.lp), C.byref(params), None, 0, C.byref(count))
+    assert rc >= 0
+    return bool(rc), count.value
 
 
 def presolve(lp: "G.HostLP", params) -> Result:
@@ -351,6 +364,15 @@ class Plan:
 
     def flags(self):
         return {k: int(getattr(self.plan, k)) for k in G.PLAN_FLAGS}
+
+    def startup(self, rule, params, pending=True):
+        """or_startup_rule into this plan; returns (applicable, count)."""
+        rid = G.STARTUP_RULE[rule] if isinstance(rule, str) else int(rule)
+        count = C.c_int32()
+        rc = load().or_startup_rule(rid, C.byref(self.view.lp), C.byref(params), C.byref(self.plan),
+                                    1 if pending else 0, C.byref(count))
+        assert rc >= 0
+        return bool(rc), count.value
 
     def tape(self):
         if not self.plan.tape:

@@ -183,6 +183,20 @@ void or_globalize_record(int32_t type, int32_t* ix, int64_t size, const int32_t*
         for (int32_t t = 0; t < support && 4 + t < size; ++t) ix[4 + t] = map_row(ix[4 + t], rows, m);
       }
       break;
+    case GP_RED_FME_COL:
+      /* [K, column, K rows, N, L, (orient, len) x L, N columns] */
+      if (size >= 4) {
+        const int32_t K = ix[0];
+        ix[1] = map_col(ix[1], cols, n);
+        for (int32_t t = 0; t < K && 2 + t < size; ++t) ix[2 + t] = map_row(ix[2 + t], rows, m);
+        const int64_t sp = 2 + (int64_t)K;
+        if (sp + 1 < size) {
+          const int32_t N = ix[sp], L = ix[sp + 1];
+          const int64_t first = sp + 2 + 2 * (int64_t)L;
+          for (int32_t t = 0; t < N && first + t < size; ++t) ix[first + t] = map_col(ix[first + t], cols, n);
+        }
+      }
+      break;
     default:
       break;
   }

@@ -265,6 +265,38 @@ static int gate_on(int enabled, int32_t rule, uint32_t run, gp_round_trace* tr) 
   return 0;
 }
 
+/* The Startup projection batch (gpu_presolve.cu:2361-2410): column
+ * domination, two-row elimination and zero-cost products share one
+ * deletion-only plan, allocated when the first of them applies; each
+ * analysis after that sees the plan's masks.  The plan is applied once. */
+static void startup_batch(or_sched* s) {
+  static const int32_t order[3] = {GP_STARTUP_COLUMN_DOMINATION, GP_STARTUP_TWO_ROW_ELIMINATION,
+                                   GP_STARTUP_ZERO_COST_PRODUCTS};
+  const int enabled[3] = {s->p->enable_column_domination, s->p->enable_two_row_variable_elimination,
+                          s->p->enable_zero_cost_product_variables};
+  gp_plan plan;
+  int allocated = 0, flags = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (!enabled[k] || s->terminal || time_exceeded(s)) continue;
+    int32_t count = 0;
+    if (!or_startup_rule(order[k], &s->cur.lp, s->p, allocated ? &plan : NULL, allocated, &count)) continue;
+    if (!allocated) {
+      or_plan_create(&s->cur.lp, &plan);
+      if (s->p->record_postsolve_tape) or_plan_enable_tape(&plan);
+      allocated = 1;
+      or_startup_rule(order[k], &s->cur.lp, s->p, &plan, 0, &count);
+    }
+    flags |= order[k] == GP_STARTUP_COLUMN_DOMINATION ? GP_PRIMAL_ONLY_COLUMN_DOMINATION
+                                                       : GP_PRIMAL_ONLY_PROJECTED_AUXILIARY;
+  }
+  if (!allocated) return;
+  if (plan.has_change) {
+    apply_plan(&s->cur, &plan, s->r, 0);
+    s->r->primal_only |= flags;
+  }
+  or_plan_destroy(&plan);
+}
+
 static int good_progress(int64_t before, int64_t after, int64_t minimum) {
   return before >= minimum && after <= before - minimum;
 }
@@ -288,9 +320,12 @@ int or_presolve(const gp_device_lp* lp, const gp_params* p, or_result* out) {
   s.p = p;
   s.start = now_seconds();
 
-  /* Startup: the structure-specific rules are not part of this engine; the
-   * phase still counts one iteration (gpu_presolve.cu:2361-2454). */
-  if (p->enable_startup && p->max_iters > 0 && !time_exceeded(&s)) out->iterations++;
+  /* Startup (gpu_presolve.cu:2361-2454): the structural rules, then one
+   * iteration counted. */
+  if (p->enable_startup && p->max_iters > 0 && !time_exceeded(&s)) {
+    if (p->enable_structure_specific_rules) startup_batch(&s);
+    out->iterations++;
+  }
 
   int phase = GP_PHASE_CLEANUP;
   const int64_t min_dec = 100;

@@ -78,6 +78,9 @@ class OracleBackend:
     def flags(self):
         return self.p.flags()
 
+    def startup(self, rule, params, pending=True):
+        return self.p.startup(rule, params, pending)
+
     def obj(self):
         return float(self.p.plan.obj_constant_delta)
 
@@ -142,6 +145,14 @@ class GpuBackend:
 
     def flags(self):
         return {k: int(getattr(self.plan, k)) for k in G.PLAN_FLAGS}
+
+    def startup(self, rule, params, pending=True):
+        """gp_startup_rule into this plan; returns (applicable, count)."""
+        rid = G.STARTUP_RULE[rule] if isinstance(rule, str) else int(rule)
+        count, ok = C.c_int32(), C.c_int32()
+        G.check(G.load().gp_startup_rule(rid, C.byref(self.dlp.lp), C.byref(params), C.byref(self.plan),
+                                         1 if pending else 0, C.byref(count), C.byref(ok)))
+        return bool(ok.value), count.value
 
     def obj(self):
         return float(self.plan.obj_constant_delta)
@@ -238,3 +249,13 @@ def assert_close(got, exp, tol, what):
         fin = ~(np.isinf(exp) | np.isnan(exp))
         ok = same_inf | (fin & (np.abs(got - exp) <= tol))
     assert ok.all(), f"{what}: got {got.tolist()}, expected {exp.tolist()}"
+
+
+def gpu_startup_analyze(lp: G.HostLP, rule, params):
+    """gp_startup_rule without a plan (analysis only): (applicable, count)."""
+    rid = G.STARTUP_RULE[rule] if isinstance(rule, str) else int(rule)
+    count, ok = C.c_int32(), C.c_int32()
+    with G.DeviceLP(lp) as d:
+        G.check(G.load().gp_startup_rule(rid, C.byref(d.lp), C.byref(params), None, 0, C.byref(count),
+                                         C.byref(ok)))
+    return bool(ok.value), count.value
