@@ -293,6 +293,34 @@ def highs_objective(lp):
     return float(res.fun) + float(lp.obj_constant) if res.status == 0 else None
 
 
+def defaults_leg(dlp, runs=2):
+    """The same resident LP through gp_preprocess with the reference's default
+    parameters: folding first (colour refinement; it finds no symmetry in the
+    generator LP), then presolve with the structural Startup rules on.  The
+    piecewise-linear reformulation branches are not built; a request for them
+    is reported in unsupported_stages."""
+    import gpupre as G
+
+    out = None
+    for k in range(runs):
+        G.check(G.load().gp_synchronize())
+        t = time.perf_counter()
+        s = G.preprocess(dlp, G.default_params())
+        G.check(G.load().gp_synchronize())
+        wall = time.perf_counter() - t
+        f = s.folding
+        out = {"wall_ms": round(wall * 1e3, 3), "preprocessing_ms": round(s.preprocessing_seconds * 1e3, 3),
+               "folding_ms": round(f.seconds * 1e3, 3), "folding_applied": bool(s.folding_applied),
+               "folding_rounds": int(f.rounds), "presolve_core_ms": round(s.elapsed_seconds * 1e3, 3),
+               "reduced": {"rows": s.reduced_rows, "cols": s.reduced_cols, "nnz": s.reduced_nnz,
+                           "iterations": s.iterations},
+               "unsupported_stages": int(s.unsupported_stages), "primal_only_reductions": int(s.primal_only_reductions),
+               "runs": k + 1, "note": "second run reported (the first warms the folding kernels)"}
+        s.free()
+    log(f"reference defaults (folding + structural rules): {out}")
+    return out
+
+
 def lp_solver_leg(seed, medium=False):
     """The FP64 first-order solver (gp_lp_solve) and the north star's gate: the
     presolved LP solved, postsolved, and its objective set against HiGHS on the
@@ -393,6 +421,8 @@ def main():
     ap.add_argument("--no-lp-solver", action="store_true", help="skip the LP-solver gate leg")
     ap.add_argument("--lp-medium", action="store_true", help="add the 200k x 300k LP to the solver leg (~1 min)")
     ap.add_argument("--no-mps", action="store_true", help="skip the MPS ingest leg")
+    ap.add_argument("--no-defaults", action="store_true",
+                    help="skip the reference-default pipeline leg (folding + structural rules)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings, and exit")
@@ -467,6 +497,7 @@ def main():
     post_ms = (time.perf_counter() - t) * 1e3
     s.free()
     log(f"postsolve: {post_ms:.1f} ms for {rec_count} tape records")
+    defaults = None if args.no_defaults else defaults_leg(dlp)
 
     roofline = None
     families = {}
@@ -568,7 +599,7 @@ def main():
         "postsolve_ms": round(post_ms, 2), "postsolve_records": int(rec_count),
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
         "batched": batched, "learned_vs_plain": learned, "configs": configs, "large_fp64_verdict": fp64,
-        "pslp_full_lp": pslp_full, "lp_solver": lp_solver, "mps_ingest": mps,
+        "pslp_full_lp": pslp_full, "lp_solver": lp_solver, "mps_ingest": mps, "reference_defaults": defaults,
     }
     print(json.dumps(out), flush=True)
 

@@ -183,3 +183,21 @@ def test_default_thresholds_hold_small_counts_back():
     lp = S.planted_projections(0, blocks=24)
     assert not O.startup_analyze(lp, "two_row_elimination", G.default_params())[0]
     assert not O.startup_analyze(lp, "zero_cost_products", G.default_params())[0]
+
+
+@pytest.mark.parametrize("n", [6, 8])
+def test_qap_reduction_matches_reference_formula(n):
+    """The reference's QAP20 logs (results/raw/benchmarks/QAP20/03_GPU-Preprocessor/
+    logs/*.log, e.g. tai60b: 26,355,720 x 12,963,600 -> 432,120 x 6,485,400 with
+    26,359,200 nonzeros) follow 2n^3 + 2n rows, n^2 + (n^4 - n^2)/2 + n^2
+    columns and n^2 (2n + 2) + 2n^4 nonzeros; the general rules alone give
+    them on the generator's QAP LPs."""
+    assert (2 * 60**3 + 2 * 60, 60**2 + (60**4 - 60**2) // 2 + 60**2,
+            60**2 * (2 * 60 + 2) + 2 * 60**4) == (432120, 6485400, 26359200)
+    lp = G.generate_qap(n, seed=1)
+    assert (lp.m, lp.n) == (2 * n**4 + 2 * n**3 + n**2 + 2 * n, n**4 + n**2)
+    r = O.presolve(lp, G.parity_params())
+    assert (r.m1, r.n1, r.reduced.nnz) == (2 * n**3 + 2 * n, n * n + (n**4 - n * n) // 2 + n * n,
+                                           n * n * (2 * n + 2) + 2 * n**4)
+    on = O.presolve(lp, G.parity_params(enable_structure_specific_rules=1))
+    assert (on.m1, on.n1, on.reduced.nnz) == (r.m1, r.n1, r.reduced.nnz) and on.primal_only == 0
