@@ -321,6 +321,60 @@ def defaults_leg(dlp, runs=2):
     return out
 
 
+# The reference's published presolve and preprocessing times on H100 PCIe for
+# the two largest QAP20 instances, and their reduced dimensions
+# (results/raw/benchmarks/QAP20/03_GPU-Preprocessor/logs/
+#  tables_3_4_5_qap20_gpu_preprocessor_{tai60b,lipa70a}.log:16-22,27)
+PUBLISHED_QAP = {
+    60: dict(instance="tai60b", presolve_ms=202.03673, preprocessing_ms=335.16832, rows=26355720, cols=12963600,
+             nnz=65235600, reduced=(432120, 6485400, 26359200)),
+    70: dict(instance="lipa70a", presolve_ms=366.32473, preprocessing_ms=619.87047, rows=48711040, cols=24014900,
+             nnz=120740900, reduced=(686140, 12012350, 48715800)),
+}
+
+
+def published_qap_leg(seed, runs=3):
+    """QAP LPs of the shape of the reference's two largest QAP20 instances
+    (generate_qap20.jl's layout at n = 60 and 70, seeded flow / distance
+    data), presolved with the reference's defaults: folding, then presolve
+    with the structural Startup rules.  Set beside the reference's published
+    H100 PCIe times; the reduced dimensions must equal the published ones."""
+    import numpy as np
+    import gpupre as G
+
+    out = {}
+    for n, pub in PUBLISHED_QAP.items():
+        lp = G.generate_qap(n, seed=seed + n)
+        assert (lp.m, lp.n, lp.nnz) == (pub["rows"], pub["cols"], pub["nnz"])
+        with G.DeviceLP(lp) as d:
+            core, prep, fold = [], [], []
+            for k in range(runs + 1):
+                G.check(G.load().gp_synchronize())
+                s = G.preprocess(d, G.default_params())
+                G.check(G.load().gp_synchronize())
+                red = (s.reduced_rows, s.reduced_cols, s.reduced_nnz)
+                if k > 0:  # the first run warms the kernels
+                    core.append(s.elapsed_seconds * 1e3)
+                    prep.append(s.preprocessing_seconds * 1e3)
+                    fold.append(s.folding.seconds * 1e3)
+                s.free()
+        want = pub["reduced"]
+        same = red[0] == want[0] and red[1] == want[1] and (want[2] is None or red[2] == want[2])
+        c, p = float(np.median(core)), float(np.median(prep))
+        out[pub["instance"] + "_shape"] = {
+            "n": n, "rows": lp.m, "cols": lp.n, "nnz": lp.nnz,
+            "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2]}, "reduced_equals_published": same,
+            "presolve_ms_p50": round(c, 3), "preprocessing_ms_p50": round(p, 3),
+            "folding_ms_p50": round(float(np.median(fold)), 3),
+            "published_presolve_ms": pub["presolve_ms"], "published_preprocessing_ms": pub["preprocessing_ms"],
+            "presolve_speedup": round(pub["presolve_ms"] / c, 3),
+            "preprocessing_speedup": round(pub["preprocessing_ms"] / p, 3),
+            "published_hardware": "NVIDIA H100 PCIe"}
+        log(f"QAP n={n} ({pub['instance']} shape): {out[pub['instance'] + '_shape']}")
+        del lp
+    return out
+
+
 def lp_solver_leg(seed, medium=False):
     """The FP64 first-order solver (gp_lp_solve) and the north star's gate: the
     presolved LP solved, postsolved, and its objective set against HiGHS on the
@@ -421,6 +475,8 @@ def main():
     ap.add_argument("--no-lp-solver", action="store_true", help="skip the LP-solver gate leg")
     ap.add_argument("--lp-medium", action="store_true", help="add the 200k x 300k LP to the solver leg (~1 min)")
     ap.add_argument("--no-mps", action="store_true", help="skip the MPS ingest leg")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the QAP legs of the reference's published tai60b / lipa70a shapes")
     ap.add_argument("--no-defaults", action="store_true",
                     help="skip the reference-default pipeline leg (folding + structural rules)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -584,6 +640,7 @@ def main():
     fp64 = None if (args.no_fp64 or args.workload != "large") else fp64_verdict_leg(args.seed)
     lp_solver = None if args.no_lp_solver else lp_solver_leg(args.seed, args.lp_medium)
     mps = None if args.no_mps else mps_ingest_leg(args.seed)
+    published = None if args.no_published else published_qap_leg(args.seed)
 
     value = nnz / (ms_per_step * 1e-3) / 1e6
     out = {
@@ -600,6 +657,7 @@ def main():
         "kernel_families": families, "roofline": roofline, "cpu_baseline": cpu, "cpu_restatement": restatement,
         "batched": batched, "learned_vs_plain": learned, "configs": configs, "large_fp64_verdict": fp64,
         "pslp_full_lp": pslp_full, "lp_solver": lp_solver, "mps_ingest": mps, "reference_defaults": defaults,
+        "published_qap": published,
     }
     print(json.dumps(out), flush=True)
 

@@ -491,7 +491,7 @@ int lpgen_generate(const lpgen_spec* spec, lpgen_lp* out) {
  * zero with probability 1/2.  x0 = 1/n, y0 = 1/n^2 is a feasible point. */
 int lpgen_qap(int32_t nq, uint64_t seed, lpgen_lp* out) {
   memset(out, 0, sizeof(*out));
-  if (nq < 2 || nq > 40) return 1;
+  if (nq < 2 || nq > 100) return 1;
   const int64_t n = nq, n2 = n * n, n3 = n2 * n, n4 = n3 * n;
   const int64_t m64 = 2 * n4 + 2 * n3 + n2 + 2 * n, ncols64 = n4 + n2;
   const int64_t nnz64 = n2 * (2 + 2 * n + 1) + 3 * n4 + 2 * (n4 - n2);

@@ -134,3 +134,22 @@ def test_config5_learned_schedule_parity(name):
     assert obj == pobj
     c = lp.c
     assert float(c @ out["x_learned"]) == pytest.approx(float(c @ out["x_plain"]), rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("n,published", [(60, (432120, 6485400, 26359200)), (70, (686140, 12012350, 48715800))])
+def test_qap_reduction_equals_published(n, published):
+    """QAP LPs of the shape of the reference's tai60b (n = 60) and lipa70a
+    (n = 70) reduce, under the reference's default parameters, to exactly the
+    dimensions of its published logs
+    (results/raw/benchmarks/QAP20/03_GPU-Preprocessor/logs/
+    tables_3_4_5_qap20_gpu_preprocessor_{tai60b,lipa70a}.log:22,27)."""
+    lp = G.generate_qap(n, seed=n)
+    print(f"[qap published] n={n}: {lp.m} x {lp.n} nnz={lp.nnz}", flush=True)
+    with G.DeviceLP(lp) as d:
+        s = G.preprocess(d, G.default_params())
+        try:
+            assert not s.has_infeasible and not s.has_unbounded
+            assert (s.reduced_rows, s.reduced_cols, s.reduced_nnz) == published
+            assert s.folding_applied == 0
+        finally:
+            s.free()
