@@ -431,9 +431,11 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
 
 // Launch-schedule knobs; every choice gives identical results.  lanes_*: lanes
 // per row of the grouped sweeps (64 / lanes rows per wavefront); fold_rows:
-// rows per workgroup of the row folds.  0 = derived from the matrix.
+// rows per workgroup of the row folds; compact_per_lane: entries per lane of
+// the compaction tiles (4, 8 or 16 with 1,024 lanes per tile).  0 = derived
+// from the matrix (compaction: 8).
 struct Tuning {
-  int32_t lanes_candidates = 0, lanes_col_max = 0, lanes_compact = 0, lanes_live_stats = 0, fold_rows = 0;
+  int32_t lanes_candidates = 0, lanes_col_max = 0, compact_per_lane = 0, lanes_live_stats = 0, fold_rows = 0;
 };
 Tuning& tuning();  // process-wide (gp_primitives.hip)
 

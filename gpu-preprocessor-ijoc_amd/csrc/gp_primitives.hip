@@ -768,233 +768,251 @@ void csr_pool_free(gp_csr& A) {
 
 // ---- compaction ------------------------------------------------------------------------------
 namespace {
-// New index of column j: its rank among the kept columns, from the packed
-// keep bits and a per-word prefix (both L2-resident), instead of an HBM
-// gather into an n-entry old-to-new map.  -1 for a dropped column.
+// New index of column j: its rank among the kept columns, from one 8-byte
+// record per 32 columns (x: the kept bits, y: kept columns before the word),
+// so an entry costs one gather into an L2-resident table (2 MB at 8M
+// columns) instead of an HBM gather into an n-entry old-to-new map or two
+// gathers into separate bit and prefix arrays.  -1 for a dropped column.
 struct ColRank {
-  const uint32_t* words;  // null: keep every column, identity map
-  const int32_t* prefix;
-  __device__ int32_t operator()(int32_t j) const {
-    if (!words) return j;
-    const uint32_t w = words[j >> 5], bit = 1u << (j & 31);
-    return (w & bit) ? prefix[j >> 5] + __popc(w & (bit - 1u)) : -1;
-  }
+  const uint2* words;  // null: keep every column, identity map
 };
+__global__ void k_rank_words(const uint32_t* bits, const int32_t* prefix, int64_t nw, uint2* out) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w < nw) out[w] = make_uint2(bits[w], static_cast<uint32_t>(prefix[w]));
+}
+__global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w < nw) out[w] = __popc(words[w]);
+}
 
 // Entry-parallel stable compaction.  The output is the kept entries in input
-// order, so a kept entry's position is the number of kept entries before it:
-// each workgroup takes a tile of kCompactTile consecutive entries, starts from
-// the kept count before the tile (the row offset of the tile's first row plus
-// that row's kept entries before the tile) and adds a block prefix of the keep
-// flags.  Reads and writes are contiguous per pass; row ids come from LDS.
-constexpr int kCompactTile = 4096;
-// tile_row[t] = the row holding entry t * kCompactTile (one lane per row)
-__global__ void k_tile_rows(const gp_offset* rp, int32_t m, int32_t* tile_row) {
+// order, so a kept entry's position is the number of kept entries before it.
+// One workgroup per tile of kCompactThreads * P consecutive entries; a
+// lane's k-th entry is t0 + k * kCompactThreads + lane id, so every load and
+// store stays coalesced.  tile_row[t] = the row holding the tile's first
+// entry (one lane per row).
+constexpr int kCompactThreads = 1024;
+__global__ void k_tile_rows(const gp_offset* rp, int32_t m, int64_t tile, int32_t* tile_row) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= m) return;
   const int64_t a = rp[r], b = rp[r + 1];
   if (a >= b) return;
-  for (int64_t t = (a + kCompactTile - 1) / kCompactTile; t * kCompactTile < b; ++t)
-    tile_row[t] = static_cast<int32_t>(r);
-}
-// Row ids of the tile's entries into LDS: rows from i0 (the row holding t0) on
-// that start before t1.
-__device__ inline void fill_row_of(const gp_offset* rp, int32_t m, int32_t i0, int64_t t0, int64_t t1,
-                                   int32_t* row_of) {
-  for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && rp[r] < t1; r += 256) {
-    const int64_t a = rp[r] > t0 ? rp[r] : t0, b = rp[r + 1] < t1 ? rp[r + 1] : t1;
-    for (int64_t p = a; p < b; ++p) row_of[p - t0] = r;
-  }
+  for (int64_t t = (a + tile - 1) / tile; t * tile < b; ++t) tile_row[t] = static_cast<int32_t>(r);
 }
 
-// Entry t0 + k * 256 + threadIdx.x of a tile is the lane's k-th entry, so
-// every load stays coalesced; all kPer loads of a lane are issued before any
-// is used (branch-free: out-of-tile lanes read index 0 and are masked after).
-constexpr int kPer = kCompactTile / 256;
-struct TileEntries {
-  int32_t j[kPer];  // column index, then its new index (-1 when dropped or past the tile)
-  // column indices of the lane's entries (issued before the row map is built)
-  __device__ void load_cols(const int32_t* col, int64_t t0, int64_t t1) {
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      j[k] = ld_stream(col + (p < t1 ? p : t0));
-    }
-  }
-  __device__ void resolve(const int32_t* row_of, const uint8_t* keep_row, ColRank rank, int64_t t0, int64_t t1) {
-    uint32_t w[kPer];
-    int32_t pre[kPer];
-    uint8_t kr[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      const int32_t r = row_of[p < t1 ? static_cast<int32_t>(p - t0) : 0];
-      kr[k] = keep_row ? keep_row[r] : 1;
-      w[k] = rank.words ? rank.words[j[k] >> 5] : ~0u;
-      pre[k] = rank.words ? rank.prefix[j[k] >> 5] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      const uint32_t bit = 1u << (j[k] & 31);
-      const int32_t nj = rank.words ? pre[k] + __popc(w[k] & (bit - 1u)) : j[k];
-      j[k] = (p < t1 && kr[k] && (w[k] & bit)) ? nj : -1;
-    }
-  }
-};
-
-// Single pass with decoupled look-back.  Workgroups take tiles in ticket
-// order from an atomic counter, so every tile a workgroup waits for belongs to
-// a workgroup that is already running and publishes its count before it waits
-// on anything.  A tile's status word holds a flag in the top two bits (1: the
-// tile's own count, 2: the inclusive count of tiles 0..t) and the count below.
-// A look-back that spins past kSpinLimit polls gives up, marks the error word
-// and lets the kernel finish; the host then raises an error.
+// Tile status words for the decoupled look-back: a flag in the top two bits
+// (1: the tile's own kept count, 2: the inclusive count of tiles 0..t) and
+// the count below.  A look-back that spins past kSpinLimit polls gives up,
+// marks the error word and lets the kernel finish; the host then raises an
+// error.
 constexpr unsigned long long kStatAgg = 1ull << 62, kStatIncl = 2ull << 62, kStatMask = (1ull << 62) - 1;
 constexpr int64_t kSpinLimit = int64_t{1} << 26;
 __device__ inline void publish(unsigned long long* st, unsigned long long v) {
   __hip_atomic_store(st, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// kept entries before tile t (thread-local; called by one lane)
-__device__ inline int64_t look_back(unsigned long long* status, int64_t t, int32_t* err) {
+
+// Kept entries before tile t, looked back over by a whole wavefront: lane L
+// reads the status word of tile q - L, so one poll covers 64 predecessors.
+// The window's entries up to the nearest inclusive one are summed when none
+// of them is still unpublished; without an inclusive entry the whole window
+// is summed and the window moves 64 tiles back.  Called by every lane of one
+// wavefront; every lane returns the same value.
+__device__ inline int64_t wave_look_back(unsigned long long* status, int64_t t, int32_t* err) {
+  const int lane = static_cast<int>(__lane_id());
   int64_t excl = 0, spins = 0;
   for (int64_t q = t - 1; q >= 0;) {
-    const unsigned long long v = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((v >> 62) == 0) {
+    const int64_t idx = q - lane;
+    // positions before tile 0 read as an inclusive count of 0
+    const unsigned long long v =
+        idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kStatIncl;
+    const unsigned long long incl = __ballot((v >> 62) == 2);
+    const unsigned long long open = __ballot((v >> 62) == 0);
+    // lanes up to and including the nearest inclusive entry, else all
+    const unsigned long long upto = incl ? ((incl & (~incl + 1ull)) << 1) - 1ull : ~0ull;
+    if (open & upto) {
       if (++spins > kSpinLimit) {
-        atomicOr(err, 1);
+        if (lane == 0) atomicOr(err, 1);
         return 0;
       }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl += static_cast<int64_t>(v & kStatMask);
-    if ((v >> 62) == 2) break;
-    --q;
+    int64_t x = ((upto >> lane) & 1ull) ? static_cast<int64_t>(v & kStatMask) : 0;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    excl += x;
+    if (incl) break;
+    q -= 64;
   }
   return excl;
 }
 
-// Kept entries written at the tile's base + their prefix in the tile; the
-// output row pointer of every kept row that starts in the tile is the prefix
-// at its first position.  The tile prefix orders entries by position: the
-// count of (k', w') before (k, w) is all of k' < k plus waves w' < w of k.
+// Single pass.  Workgroups take tiles in ticket order from an atomic counter,
+// so every tile a workgroup waits for belongs to a workgroup that is already
+// running and publishes its count before it waits on anything.  A tile keeps
+// in LDS a bit per entry of a dropped row, the wavefront ballots of its keep
+// flags and their exclusive prefix per 64-entry group (groups in position
+// order g = k * waves + wave); an entry's output position is base +
+// prefix(g) + the kept lanes of g before it, and the row pointer of a kept
+// row starting at tile position a is the same expression at a.
 // ctl[0]: tile ticket, ctl[1]: error word, ctl[2]: total kept entries,
 // ctl[3..]: tile status words; all zero at launch.
-__global__ void __launch_bounds__(256) k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val,
-                                                       int32_t m, int64_t nnz, int64_t tiles, const int32_t* tile_row,
-                                                       const uint8_t* keep_row, const int32_t* row_o2n,
-                                                       ColRank col_rank, int32_t m_new, unsigned long long* ctl,
-                                                       gp_offset* out_rp, int32_t* out_col, double* out_val) {
-  __shared__ int32_t row_of[kCompactTile];
-  __shared__ int32_t pre[kCompactTile];
-  __shared__ int32_t s_cnt[kPer * 4];
+template <int P>
+__global__ void __launch_bounds__(kCompactThreads)
+    k_compact_tiles(const gp_offset* rp, const int32_t* col, const double* val, int32_t m, int64_t nnz, int64_t tiles,
+                    const int32_t* tile_row, const uint8_t* keep_row, const int32_t* row_o2n, ColRank col_rank,
+                    int32_t m_new, unsigned long long* ctl, gp_offset* out_rp, int32_t* out_col, double* out_val) {
+  constexpr int kT = kCompactThreads, kTile = kT * P, kWaves = kT / 64;
+  constexpr int kGroups = kTile / 64, kWords = kTile / 32;
+  constexpr int kPerLane = kGroups / 64;  // groups of the prefix scan per lane of wave 0
+  static_assert(kGroups % 64 == 0, "prefix scan layout");
+  __shared__ uint32_t s_drop[kWords];
+  __shared__ unsigned long long s_bal[kGroups];
+  __shared__ int32_t s_pre[kGroups + 1];
   __shared__ int64_t s_tile, s_base;
   const int lane = static_cast<int>(__lane_id()), wave = threadIdx.x / 64;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long* status = ctl + 3;
   int32_t* err = reinterpret_cast<int32_t*>(ctl + 1);
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = static_cast<int64_t>(atomicAdd(ctl, 1ull));
-    __syncthreads();
-    const int64_t tile = s_tile;
-    if (tile >= tiles) break;
-    const int64_t t0 = tile * kCompactTile;
-    const int64_t t1 = t0 + kCompactTile < nnz ? t0 + kCompactTile : nnz;
-    TileEntries e;
-    e.load_cols(col, t0, t1);
-    double v[kPer];
+  if (threadIdx.x == 0) s_tile = static_cast<int64_t>(atomicAdd(ctl, 1ull));
+  for (int w = threadIdx.x; w < kWords; w += kT) s_drop[w] = 0u;
+  __syncthreads();
+  const int64_t tile = s_tile;
+  if (tile >= tiles) return;
+  const int64_t t0 = tile * kTile;
+  const int64_t t1 = t0 + kTile < nnz ? t0 + kTile : nnz;
+  // every load of the lane issued before any is used (out-of-tile lanes read
+  // entry t0 and are masked below)
+  int32_t j[P];
+  double v[P];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      v[k] = ld_stream(val + (p < t1 ? p : t0));
-    }
-    const int32_t i0 = tile_row[tile];
-    fill_row_of(rp, m, i0, t0, t1, row_of);
-    __syncthreads();
-    e.resolve(row_of, keep_row, col_rank, t0, t1);
-    unsigned long long bal[kPer];
+  for (int k = 0; k < P; ++k) {
+    const int64_t p = t0 + static_cast<int64_t>(k) * kT + threadIdx.x;
+    j[k] = ld_stream(col + (p < t1 ? p : t0));
+  }
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      bal[k] = __ballot(e.j[k] >= 0);
-      if (lane == 0) s_cnt[k * 4 + wave] = __popcll(bal[k]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t own = 0;
-      for (int q = 0; q < kPer * 4; ++q) own += s_cnt[q];
-      int64_t excl = 0;
-      if (tile == 0) {
-        publish(status, kStatIncl | static_cast<unsigned long long>(own));
-      } else {
-        publish(status + tile, kStatAgg | static_cast<unsigned long long>(own));
-        excl = look_back(status, tile, err);
-        publish(status + tile, kStatIncl | static_cast<unsigned long long>(excl + own));
+  for (int k = 0; k < P; ++k) {
+    const int64_t p = t0 + static_cast<int64_t>(k) * kT + threadIdx.x;
+    v[k] = ld_stream(val + (p < t1 ? p : t0));
+  }
+  const int32_t i0 = tile_row[tile];
+  // the tile's entries of dropped rows
+  if (keep_row) {
+    for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && rp[r] < t1; r += kT) {
+      if (keep_row[r]) continue;
+      const int64_t a = (rp[r] > t0 ? rp[r] : t0) - t0, b = (rp[r + 1] < t1 ? rp[r + 1] : t1) - t0;
+      for (int64_t q = a; q < b;) {
+        const int64_t w = q >> 5, e = ((w + 1) << 5) < b ? ((w + 1) << 5) : b;
+        const uint32_t below_e = (e - (w << 5)) == 32 ? ~0u : ((1u << (e - (w << 5))) - 1u);
+        atomicOr(&s_drop[w], below_e & ~((1u << (q & 31)) - 1u));
+        q = e;
       }
+    }
+  }
+  uint2 rank[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) rank[k] = col_rank.words ? col_rank.words[j[k] >> 5] : make_uint2(~0u, 0u);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int32_t pos = k * kT + static_cast<int32_t>(threadIdx.x);
+    const bool in = t0 + pos < t1 && !((s_drop[pos >> 5] >> (pos & 31)) & 1u);
+    const uint32_t bit = 1u << (j[k] & 31);
+    const int32_t nj = col_rank.words ? static_cast<int32_t>(rank[k].y) + __popc(rank[k].x & (bit - 1u)) : j[k];
+    j[k] = in && (rank[k].x & bit) ? nj : -1;
+    const unsigned long long b = __ballot(j[k] >= 0);
+    if (lane == 0) s_bal[k * kWaves + wave] = b;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // exclusive prefix of the group counts, kPerLane consecutive groups per lane
+    int32_t c[kPerLane];
+    int32_t sum = 0;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+      c[u] = __popcll(s_bal[lane * kPerLane + u]);
+      sum += c[u];
+    }
+    int32_t incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    int32_t run = incl - sum;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+      s_pre[lane * kPerLane + u] = run;
+      run += c[u];
+    }
+    const int64_t own = __shfl(incl, 63, 64);
+    int64_t excl = 0;
+    if (tile == 0) {
+      if (lane == 0) publish(status, kStatIncl | static_cast<unsigned long long>(own));
+    } else {
+      if (lane == 0) publish(status + tile, kStatAgg | static_cast<unsigned long long>(own));
+      excl = wave_look_back(status, tile, err);
+      if (lane == 0) publish(status + tile, kStatIncl | static_cast<unsigned long long>(excl + own));
+    }
+    if (lane == 0) {
+      s_pre[kGroups] = static_cast<int32_t>(own);
       if (tile == tiles - 1) {
         ctl[2] = static_cast<unsigned long long>(excl + own);
         out_rp[m_new] = static_cast<gp_offset>(excl + own);
       }
       s_base = excl;
     }
-    __syncthreads();
-    const int64_t base = s_base;
-    int32_t running = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      int32_t off = running;
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        const int32_t cnt = s_cnt[k * 4 + w];
-        off += w < wave ? cnt : 0;
-        running += cnt;
-      }
-      const int32_t at = off + __popcll(bal[k] & lt);
-      const int64_t p = t0 + static_cast<int64_t>(k) * 256 + threadIdx.x;
-      if (p < t1) pre[p - t0] = at;
-      if (e.j[k] >= 0) {
-        out_col[base + at] = e.j[k];
-        out_val[base + at] = v[k];
-      }
-    }
-    __syncthreads();
-    // row pointers of the kept rows starting in this tile (trailing empty rows
-    // at nnz belong to the last tile; empty rows just before i0 start at t0)
-    const bool last = t1 == nnz;
-    for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && (rp[r] < t1 || (last && rp[r] == nnz));
-         r += 256) {
-      const int64_t a = rp[r];
-      if (a < t0 || (keep_row && !keep_row[r])) continue;
-      out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + (a < t1 ? pre[a - t0] : running));
-    }
-    if (threadIdx.x == 0)
-      for (int32_t r = i0 - 1; r >= 0 && rp[r] == t0; --r)
-        if (!keep_row || keep_row[r]) out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base);
-    __syncthreads();
   }
-}
-static_assert(kBlock == 256, "k_compact_tiles assumes 256-lane workgroups");
-
-__global__ void k_word_popc(const uint32_t* words, int64_t nw, int32_t* out) {
-  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w < nw) out[w] = __popc(words[w]);
+  __syncthreads();
+  const int64_t base = s_base;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (j[k] < 0) continue;
+    const int g = k * kWaves + wave;
+    const int64_t at = base + s_pre[g] + __popcll(s_bal[g] & lt);
+    out_col[at] = j[k];
+    out_val[at] = v[k];
+  }
+  // row pointers of the kept rows starting in this tile (trailing empty rows
+  // at nnz belong to the last tile; empty rows just before i0 start at t0)
+  const bool last = t1 == nnz;
+  for (int32_t r = i0 + static_cast<int32_t>(threadIdx.x); r < m && (rp[r] < t1 || (last && rp[r] == nnz)); r += kT) {
+    const int64_t a = rp[r];
+    if (a < t0 || (keep_row && !keep_row[r])) continue;
+    int64_t at = s_pre[kGroups];
+    if (a < t1) {
+      const int32_t pos = static_cast<int32_t>(a - t0), g = pos >> 6;
+      at = s_pre[g] + __popcll(s_bal[g] & ((1ull << (pos & 63)) - 1ull));
+    }
+    out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base + at);
+  }
+  if (threadIdx.x == 0)
+    for (int32_t r = i0 - 1; r >= 0 && rp[r] == t0; --r)
+      if (!keep_row || keep_row[r]) out_rp[row_o2n ? row_o2n[r] : r] = static_cast<gp_offset>(base);
 }
 }  // namespace
 
 void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_col, const int32_t* row_o2n,
                  int32_t m_new, int32_t n_new, gp_csr& out) {
   const int32_t m = A.rows;
-  const PackedMask cols(keep_col, A.cols);
-  // column ranks: exclusive prefix of the kept bits per 32-column word
+  // column ranks: the kept bits and the kept count before each 32-column word
   const int64_t nw = (static_cast<int64_t>(A.cols) + 31) / 32;
-  DVec<int32_t> word_prefix(static_cast<size_t>(nw > 0 ? nw : 1));
+  DVec<uint2> rank_words(static_cast<size_t>(keep_col && nw > 0 ? nw : 1));
   if (keep_col && nw > 0) {
+    const PackedMask cols(keep_col, A.cols);
+    DVec<int32_t> word_prefix(static_cast<size_t>(nw));
     hipLaunchKernelGGL(k_word_popc, dim3(grid_for(nw)), dim3(kBlock), 0, 0, cols.words.data(), nw, word_prefix.data());
     GP_LAUNCH_CHECK("compact word counts");
-    exclusive_scan_i32_total(word_prefix.data(), nw);
+    exclusive_scan_i32(word_prefix.data(), nw);
+    hipLaunchKernelGGL(k_rank_words, dim3(grid_for(nw)), dim3(kBlock), 0, 0, cols.words.data(), word_prefix.data(), nw,
+                       rank_words.data());
+    GP_LAUNCH_CHECK("compact rank words");
   }
-  const ColRank rank{keep_col ? cols.words.data() : nullptr, word_prefix.data()};
-  const int64_t tiles = (static_cast<int64_t>(A.nnz) + kCompactTile - 1) / kCompactTile;
+  const ColRank rank{keep_col && nw > 0 ? rank_words.data() : nullptr};
+  // entries per lane: 8 by default (8,192-entry tiles), or the tuned value
+  const int32_t per_lane = tuning().compact_per_lane == 4 || tuning().compact_per_lane == 16
+                               ? tuning().compact_per_lane
+                               : 8;
+  const int64_t tile_e = static_cast<int64_t>(kCompactThreads) * per_lane;
+  const int64_t tiles = (static_cast<int64_t>(A.nnz) + tile_e - 1) / tile_e;
   out.rows = m_new;
   out.cols = n_new;
   out.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(m_new) + 1);
@@ -1012,14 +1030,22 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   DVec<unsigned long long> ctl(static_cast<size_t>(tiles) + 3);
   unsigned long long res[3] = {};
   GP_CHECK(hipMemsetAsync(ctl.data(), 0, sizeof(unsigned long long) * (static_cast<size_t>(tiles) + 3), 0));
-  hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_row.data());
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, A.row_ptr, m, tile_e, tile_row.data());
   {
     // the tile kernel alone
     TimedScope ts(GP_KT_COMPACT, 0.0);
-    const unsigned grid = static_cast<unsigned>(tiles < 2048 ? tiles : 2048);
-    hipLaunchKernelGGL(k_compact_tiles, dim3(grid), dim3(256), 0, 0, A.row_ptr, A.col_idx, A.vals, m,
-                       static_cast<int64_t>(A.nnz), tiles, static_cast<const int32_t*>(tile_row.data()), keep_row,
-                       row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
+    const dim3 grid(static_cast<unsigned>(tiles)), block(kCompactThreads);
+    const int32_t* tr = tile_row.data();
+    const int64_t nz = static_cast<int64_t>(A.nnz);
+    if (per_lane == 4)
+      hipLaunchKernelGGL(k_compact_tiles<4>, grid, block, 0, 0, A.row_ptr, A.col_idx, A.vals, m, nz, tiles, tr,
+                         keep_row, row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
+    else if (per_lane == 16)
+      hipLaunchKernelGGL(k_compact_tiles<16>, grid, block, 0, 0, A.row_ptr, A.col_idx, A.vals, m, nz, tiles, tr,
+                         keep_row, row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
+    else
+      hipLaunchKernelGGL(k_compact_tiles<8>, grid, block, 0, 0, A.row_ptr, A.col_idx, A.vals, m, nz, tiles, tr,
+                         keep_row, row_o2n, rank, m_new, ctl.data(), out.row_ptr, out.col_idx, out.vals);
     GP_LAUNCH_CHECK("compact");
   }
   d2h(res, ctl.data(), 3);

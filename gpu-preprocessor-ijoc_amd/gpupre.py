@@ -212,7 +212,7 @@ class GpBatchSummary(C.Structure):
 
 
 class GpTuning(C.Structure):
-    _fields_ = [("lanes_candidates", C.c_int32), ("lanes_col_max", C.c_int32), ("lanes_compact", C.c_int32),
+    _fields_ = [("lanes_candidates", C.c_int32), ("lanes_col_max", C.c_int32), ("compact_per_lane", C.c_int32),
                 ("lanes_live_stats", C.c_int32), ("fold_rows", C.c_int32)]
 
     def as_dict(self):

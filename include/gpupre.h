@@ -506,14 +506,15 @@ int gp_postsolve_record(const double* x_red, const double* y_red, const double* 
 /* ---- launch-schedule autotuner (no reference counterpart) ---------------- */
 
 /* Result-invariant launch schedule: lanes per row of the grouped sweeps
- * (0, 2, 4, 8, 16 or 64; 64 / lanes rows per wavefront) and rows per
+ * (0, 2, 4, 8, 16 or 64; 64 / lanes rows per wavefront), entries per lane of
+ * the compaction tiles (0, 4, 8 or 16; 1,024 lanes per tile) and rows per
  * workgroup of the row folds (0, 128, 256 or 512).  0 = derived from the
- * matrix's mean row length; for lanes_col_max, 0 = entry-parallel tiles
- * (no lanes per column).  Process-wide. */
+ * matrix's mean row length; for lanes_col_max, 0 = entry-parallel tiles (no
+ * lanes per column); for compact_per_lane, 0 = 8.  Process-wide. */
 typedef struct gp_tuning {
   int32_t lanes_candidates;
   int32_t lanes_col_max;
-  int32_t lanes_compact;   /* accepted, no effect: compaction is entry-parallel */
+  int32_t compact_per_lane;
   int32_t lanes_live_stats;
   int32_t fold_rows;
 } gp_tuning;

@@ -1,7 +1,7 @@
 """Launch-schedule autotuner: every schedule (lanes per row of the grouped
-sweeps, rows per workgroup of the row folds) presolves to bit-identical
-results, the tuner returns a valid schedule, and invalid schedules are
-rejected."""
+sweeps, entries per lane of the compaction tiles, rows per workgroup of the
+row folds) presolves to bit-identical results, the tuner returns a valid
+schedule, and invalid schedules are rejected."""
 import numpy as np
 import pytest
 
@@ -32,13 +32,13 @@ def same(a, b):
     assert np.array_equal(la.vals.view(np.uint64), lb.vals.view(np.uint64)) and oa == ob
 
 
-@pytest.mark.parametrize("lanes,rows", [(2, 128), (8, 512), (16, 256), (64, 128)])
-def test_schedules_give_identical_results(lanes, rows):
+@pytest.mark.parametrize("lanes,per_lane,rows", [(2, 4, 128), (8, 16, 512), (16, 8, 256), (64, 4, 128)])
+def test_schedules_give_identical_results(lanes, per_lane, rows):
     lp = lp_zipf()
     with G.DeviceLP(lp) as d:
         G.set_tuning()
         ref = run(d)
-        G.set_tuning(G.GpTuning(lanes, lanes, lanes, lanes, rows))
+        G.set_tuning(G.GpTuning(lanes, lanes, per_lane, lanes, rows))
         try:
             got = run(d)
         finally:
@@ -54,8 +54,9 @@ def test_autotune_returns_valid_schedule_and_keeps_current():
         cur = G.GpTuning()
         G.check(G.load().gp_get_tuning(cur))
         assert cur.as_dict() == G.GpTuning().as_dict()
-        for k in ("lanes_candidates", "lanes_col_max", "lanes_compact", "lanes_live_stats"):
+        for k in ("lanes_candidates", "lanes_col_max", "lanes_live_stats"):
             assert getattr(best, k) in (0, 2, 4, 8, 16, 64)
+        assert best.compact_per_lane in (0, 4, 8, 16)
         assert best.fold_rows in (0, 128, 256, 512)
         assert 0 < tuned <= default
         G.set_tuning(best)
@@ -71,3 +72,5 @@ def test_invalid_schedule_rejected():
         G.set_tuning(G.GpTuning(3, 0, 0, 0, 0))
     with pytest.raises(G.GpError):
         G.set_tuning(G.GpTuning(0, 0, 0, 0, 300))
+    with pytest.raises(G.GpError):
+        G.set_tuning(G.GpTuning(0, 0, 2, 0, 0))
