@@ -360,21 +360,32 @@ int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new);
 // Sum in the fixed pairwise order (chunks of 1024, stride halving, recursive).
 double det_sum(const double* x, int64_t n);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n);
+// What one row fold sees of the entry-bounds store (struct EntryBounds
+// below): an entry with |a| > zero_tol whose column is not marked changed
+// reads its stored pair; every other entry gathers lu[col], and a refreshing
+// (writer) fold stores the gathered pair of such an entry back.
+struct StoreView {
+  double2* pairs = nullptr;  // null: no store, every pair gathered
+  Bits changed{nullptr};     // columns whose pair differs from the store's snap (null: all)
+  double zero_tol = 0.0;
+  bool write = false;
+};
+
 // ---- LDS-staged row folds --------------------------------------------------------
 // A row fold visits every stored entry of a row in CSR order and folds the
 // entry's bound terms (a*l, a*u oriented by the sign of a) into per-row state
 // sequentially, so order-dependent sums come out as in a one-lane-per-row
 // sweep.  Rows go 256 to a block; their entries stream through LDS in chunks:
-// column indices, values and the columns' (l, u) pairs are loaded
-// entry-parallel (coalesced index/value loads, one 16-byte gather per entry),
-// then each lane folds its own row's staged terms.  The policy P supplies
+// column indices, values and the entries' (l, u) pairs are loaded
+// entry-parallel -- every load of a lane issued before any is used -- with
+// the pair read from the entry-bounds store where the store holds it
+// (coalesced) and gathered from lu[col] otherwise; then each lane folds its
+// own row's staged terms.  The policy P supplies
 //   bool active(row)                        -- fold this row?
 //   bool use(a)                             -- entry contributes (else "skipped")
 //   void init(State&)
 //   void add(State&, pos_in_row, lo, hi, skipped)   -- called for every stored entry
 //   void finish(row, const State&, active)  -- called for every row of the grid
-//   double2 bounds(entry, col, a, lu)       -- (l, u) of a used entry (normally lu[col])
-//   void gathered(entry, (l, u))            -- the bounds gathered for a used entry
 // Read-once matrix stream: the non-temporal hint asks L2 and the Infinity
 // Cache not to keep these lines, so the streamed CSR does not evict the
 // gathered bound and mask arrays.
@@ -388,7 +399,9 @@ __device__ inline T ld_stream(const T* p) {
 constexpr int kFoldRows = 256;
 
 template <class P, int kFoldRows, int kFoldChunk = 8 * kFoldRows>
-__global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2* lu, Bits keep_col, P pol) {
+__global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2* lu, Bits keep_col, P pol,
+                                                        StoreView store) {
+  constexpr int kPer = kFoldChunk / kFoldRows;  // staged entries per lane and chunk
   __shared__ double s_lo[kFoldChunk];
   __shared__ double s_hi[kFoldChunk];
   __shared__ uint8_t s_skip[kFoldChunk];
@@ -403,20 +416,40 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
     const gp_offset p0 = mine ? A.row_ptr[row] : 0, p1 = mine ? A.row_ptr[row + 1] : 0;
     for (gp_offset c0 = e0; c0 < e1; c0 += kFoldChunk) {
       const gp_offset c1 = e1 - c0 > kFoldChunk ? c0 + kFoldChunk : e1;
-      for (gp_offset q = c0 + threadIdx.x; q < c1; q += kFoldRows) {
-        const int32_t j = ld_stream(A.col_idx + q);
-        const double a = ld_stream(A.vals + q);
-        const bool skip = !keep_col[j] || !pol.use(a);
-        double lo = 0.0, hi = 0.0;
-        if (!skip) {
-          const double2 b = pol.bounds(q, j, a, lu);
-          pol.gathered(q, b);
-          lo = a >= 0.0 ? a * b.x : a * b.y;
-          hi = a >= 0.0 ? a * b.y : a * b.x;
+      int32_t j[kPer];
+      double a[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const gp_offset q = c0 + k * kFoldRows + static_cast<gp_offset>(threadIdx.x);
+        const gp_offset qq = q < c1 ? q : c0;
+        j[k] = ld_stream(A.col_idx + qq);
+        a[k] = ld_stream(A.vals + qq);
+      }
+      bool used[kPer], stored[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const gp_offset q = c0 + k * kFoldRows + static_cast<gp_offset>(threadIdx.x);
+        used[k] = q < c1 && keep_col[j[k]] && pol.use(a[k]);
+        stored[k] = store.pairs && fabs(a[k]) > store.zero_tol && !store.changed[j[k]];
+      }
+      double2 b[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const gp_offset q = c0 + k * kFoldRows + static_cast<gp_offset>(threadIdx.x);
+        const double2* src = stored[k] ? store.pairs + q : lu + j[k];
+        b[k] = used[k] ? *src : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const gp_offset q = c0 + k * kFoldRows + static_cast<gp_offset>(threadIdx.x);
+        if (store.write && used[k] && !stored[k] && store.pairs && fabs(a[k]) > store.zero_tol) store.pairs[q] = b[k];
+        if (q < c1) {
+          const double lo = used[k] ? (a[k] >= 0.0 ? a[k] * b[k].x : a[k] * b[k].y) : 0.0;
+          const double hi = used[k] ? (a[k] >= 0.0 ? a[k] * b[k].y : a[k] * b[k].x) : 0.0;
+          s_lo[q - c0] = lo;
+          s_hi[q - c0] = hi;
+          s_skip[q - c0] = used[k] ? 0 : 1;
         }
-        s_lo[q - c0] = lo;
-        s_hi[q - c0] = hi;
-        s_skip[q - c0] = skip ? 1 : 0;
       }
       __syncthreads();
       if (mine) {
@@ -440,18 +473,20 @@ struct Tuning {
 Tuning& tuning();  // process-wide (gp_primitives.hip)
 
 template <class P, int R>
-void launch_row_fold_rows(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol) {
+void launch_row_fold_rows(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol, const StoreView& store) {
   const int64_t blocks = (static_cast<int64_t>(A.rows) + R - 1) / R;
-  hipLaunchKernelGGL((k_row_fold<P, R>), dim3(static_cast<unsigned>(blocks)), dim3(R), 0, 0, A, lu, keep_col, pol);
+  hipLaunchKernelGGL((k_row_fold<P, R>), dim3(static_cast<unsigned>(blocks)), dim3(R), 0, 0, A, lu, keep_col, pol,
+                     store);
 }
 
 template <class P>
-void launch_row_fold(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol) {
+void launch_row_fold(const gp_csr& A, const double2* lu, Bits keep_col, const P& pol,
+                     const StoreView& store = StoreView{}) {
   if (A.rows <= 0) return;
   switch (tuning().fold_rows) {
-    case 128: launch_row_fold_rows<P, 128>(A, lu, keep_col, pol); break;
-    case 512: launch_row_fold_rows<P, 512>(A, lu, keep_col, pol); break;
-    default: launch_row_fold_rows<P, kFoldRows>(A, lu, keep_col, pol); break;
+    case 128: launch_row_fold_rows<P, 128>(A, lu, keep_col, pol, store); break;
+    case 512: launch_row_fold_rows<P, 512>(A, lu, keep_col, pol, store); break;
+    default: launch_row_fold_rows<P, kFoldRows>(A, lu, keep_col, pol, store); break;
   }
 }
 
@@ -461,6 +496,42 @@ struct PackedBounds {
   PackedBounds(const double* l, const double* u, int32_t n);  // gp_primitives.hip
   const double2* data() const { return lu.data(); }
 };
+
+// ---- the entry-bounds store ---------------------------------------------------------
+// The (l, u) pair of each stored entry's column of A, kept at the entry's
+// position across the rules of a plan, so the row folds read the pairs
+// coalesced instead of gathering them from the 16-byte-per-column array (a
+// gather pulls a whole cache line per entry).  snap[j] is the pair that
+// every stored entry of column j holds.  The store covers the entries with
+// |a| > zero_tol and a live column in every row that was live at the last
+// full-coverage fold (a fold whose active rows are all live rows).  Such a
+// fold refreshes the entries of the columns whose current pair differs from
+// snap, then takes the current pairs as snap.  Rows and columns only die
+// within a plan, so no fold reads an entry the store does not cover.  A new
+// matrix (materialize, rewritten A) starts the store over.
+struct EntryBounds {
+  DVec<double2> pairs;               // one per entry of the matrix it belongs to
+  DVec<double2> snap;                // one per column
+  const int32_t* col_idx = nullptr;  // identity of that matrix
+  int64_t nnz = -1;
+  int32_t n = -1;
+  double zero_tol = 0.0;
+  bool valid = false;
+  bool holds(const gp_csr& A, double zt) const {
+    return valid && col_idx == A.col_idx && nnz == static_cast<int64_t>(A.nnz) && n == A.cols && zero_tol == zt;
+  }
+  void drop() {
+    valid = false;
+    col_idx = nullptr;
+  }
+};
+// The view of `eb` (may be null) for one fold over A with the current pairs
+// lu; `changed` receives the changed-column bits.  A refreshing (writer) fold
+// starts the store over when it holds nothing for A; a reader then gathers.
+StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol, bool writer,
+                     DVec<uint32_t>& changed);  // gp_primitives.hip
+// After a writer fold: the current pairs become the snap.
+void store_commit(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol);
 
 // Stable ascending sort of (key, value) pairs on bits [0, end_bit) of the key,
 // in place (gp_sort.hip).
@@ -649,7 +720,7 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf, BsRow* rec = nullptr, const double* AL = nullptr,
                           const double* AU = nullptr, const int32_t* row_nnz = nullptr,
-                          double2* entry_lu = nullptr);
+                          const StoreView& store = StoreView{});
 
 // Kept rows of A with an entry in a column whose col_mark byte is set, appended
 // to list (any order); *count (device, caller-zeroed) receives their number.

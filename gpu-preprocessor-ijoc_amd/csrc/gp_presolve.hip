@@ -91,6 +91,7 @@ void flush_plan_tape(Record& r, gp_plan& plan) {
 struct Working {
   gp_device_lp lp{};
   bool owns = false;
+  EntryBounds store;  // per-entry column bounds of lp.A for the row folds
   void free_owned() {
     if (!owns) return;
     csr_pool_free(lp.A);
@@ -192,6 +193,8 @@ void apply_plan(Working& cur, gp_plan& plan, Record& r, bool captured) {
   const int32_t n_new = build_old_to_new(plan.keep_col_mask, n, col_o2n.data());
   const gp_csr& src = plan.has_new_A ? plan.new_A : cur.lp.A;
   gp_device_lp next{};
+  // a new matrix: the entry-bounds store starts over at its first full fold
+  cur.store.drop();
   compact_csr(src, plan.keep_row_mask, plan.keep_col_mask, row_o2n.data(), m_new, n_new, next.A);
   // Without a rewritten matrix the transpose of the compacted A is the compacted
   // AT: dropping dead entries keeps each column's increasing row order.
@@ -278,6 +281,7 @@ struct Scheduler {
   void prepare_deferred() {
     if (d.active) return;
     plan_create(cur.lp, d.plan);
+    rule_caches(d.plan).store = &cur.store;
     if (p->record_postsolve_tape) d.plan.tape = new DeviceTape();
     stats_create(cur.lp.A.rows, cur.lp.A.cols, d.stats);
     const int32_t n = cur.lp.A.cols;

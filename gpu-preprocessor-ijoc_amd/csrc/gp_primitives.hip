@@ -1058,6 +1058,7 @@ void compact_csr(const gp_csr& A, const uint8_t* keep_row, const uint8_t* keep_c
   out.nnz = static_cast<gp_offset>(res[2]);
 }
 
+
 // ---- entry-parallel segment reductions -------------------------------------------------------
 namespace {
 __global__ void k_seg_tile_first(const gp_offset* rp, int32_t m, int32_t* tile_seg) {
@@ -1576,6 +1577,61 @@ PackedBounds::PackedBounds(const double* l, const double* u, int32_t n) : lu(sta
   GP_LAUNCH_CHECK("pack bounds");
 }
 
+namespace {
+// bit j: column j's current pair differs from the snap (bitwise)
+__global__ void k_pairs_changed(const double2* lu, const double2* snap, int32_t n, uint32_t* words) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool c = false;
+  if (j < n) {
+    const double2 a = lu[j], b = snap[j];
+    c = __double_as_longlong(a.x) != __double_as_longlong(b.x) || __double_as_longlong(a.y) != __double_as_longlong(b.y);
+  }
+  const unsigned long long bal = __ballot(c);
+  const int lane = static_cast<int>(__lane_id());
+  if ((lane == 0 || lane == 32) && j < n) words[j >> 5] = static_cast<uint32_t>(lane == 0 ? bal : bal >> 32);
+}
+}  // namespace
+
+StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol, bool writer,
+                     DVec<uint32_t>& changed) {
+  StoreView v;
+  if (!eb || A.nnz <= 0 || A.cols <= 0) return v;
+  const int32_t n = A.cols;
+  if (eb->holds(A, zero_tol)) {
+    // current pairs, the snap, a bit out per column
+    TimedScope ts(GP_KT_ENTRY_BOUNDS, 32.0 * n + n / 8.0);
+    changed.resize(static_cast<size_t>((n + 31) / 32));
+    hipLaunchKernelGGL(k_pairs_changed, dim3(grid_for((static_cast<int64_t>(n) + 31) / 32 * 32)), dim3(kBlock), 0, 0,
+                       lu, eb->snap.data(), n, changed.data());
+    GP_LAUNCH_CHECK("changed pairs");
+    v.changed = Bits{changed.data()};
+  } else if (writer) {
+    eb->drop();
+    eb->pairs.resize(static_cast<size_t>(A.nnz));
+    v.changed = Bits{nullptr};  // every column: gather and store
+  } else {
+    return v;
+  }
+  v.pairs = eb->pairs.data();
+  v.zero_tol = zero_tol;
+  v.write = writer;
+  return v;
+}
+
+void store_commit(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol) {
+  if (!eb || A.nnz <= 0 || A.cols <= 0) return;
+  {
+    TimedScope ts(GP_KT_ENTRY_BOUNDS, 32.0 * A.cols);
+    eb->snap.resize(static_cast<size_t>(A.cols));
+    d2d(eb->snap.data(), lu, static_cast<size_t>(A.cols));
+  }
+  eb->col_idx = A.col_idx;
+  eb->nnz = static_cast<int64_t>(A.nnz);
+  eb->n = A.cols;
+  eb->zero_tol = zero_tol;
+  eb->valid = true;
+}
+
 // ---- row activity summary ------------------------------------------------------------------
 // Per live row: the finite parts of the minimum / maximum activity (blocked
 // Neumaier sums in CSR order) and the counts of -inf / +inf terms; entries
@@ -1592,17 +1648,12 @@ struct ActivityFold {
   const double* AL;
   const double* AU;
   const int32_t* row_nnz;
-  double2* entry_lu;  // optional: (l, u) of each used entry, at the entry's position
   struct State {
     RowAcc lo, hi;
     int32_t ni, pi;
   };
   __device__ bool active(int64_t i) const { return !keep_row || keep_row[i]; }
   __device__ bool use(double a) const { return fabs(a) > zero_tol; }
-  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
-  __device__ void gathered(int64_t q, double2 b) const {
-    if (entry_lu) entry_lu[q] = b;
-  }
   __device__ void init(State& s) const {
     s.lo = RowAcc{};
     s.hi = RowAcc{};
@@ -1631,7 +1682,7 @@ struct ActivityFold {
 void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, const uint8_t* keep_row,
                           const uint8_t* keep_col, double* min_fin, double* max_fin, int32_t* min_ninf,
                           int32_t* max_pinf, BsRow* rec, const double* AL, const double* AU,
-                          const int32_t* row_nnz, double2* entry_lu) {
+                          const int32_t* row_nnz, const StoreView& store) {
   if (A.rows <= 0) return;
   const PackedMask cols(keep_col, A.cols);
   // compulsory bytes of the sweep: row_ptr, col_idx, vals; one (l, u) pair per column; the row and
@@ -1641,8 +1692,8 @@ void row_activity_summary(const gp_csr& A, const double2* lu, double zero_tol, c
                        24.0 * A.rows;
   TimedScope ts(GP_KT_ACTIVITY, bytes);
   launch_row_fold(A, lu, cols.view(),
-                  ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz,
-                               entry_lu});
+                  ActivityFold{keep_row, zero_tol, min_fin, max_fin, min_ninf, max_pinf, rec, AL, AU, row_nnz},
+                  store);
   GP_LAUNCH_CHECK("activity summary");
 }
 

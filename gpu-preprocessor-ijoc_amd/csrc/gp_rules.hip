@@ -360,37 +360,10 @@ struct Act {
 };
 
 // Row fold of the redundancy test: full activity range of each live row
-// with two or more live entries that is not an equality.
-// The (l, u) pairs bound strengthening's activity sweep gathered, stored per
-// used entry, and the columns whose bounds the rule then changed.  Valid for
-// the rule applied right after it on the same plan (redundant rows); any
-// other rule invalidates it, and so does every call of the public rule entry
-// point, since its caller may rewrite the plan in between.
-// Invariant the reuse rests on: between the two rules keep_row and keep_col
-// only lose entries.  The sweep stores pairs only for entries it used (live
-// row, live column, |a| > zero_tol, in a workgroup with an active row); an
-// entry the redundancy fold reads is live then, so it was live in the sweep,
-// and its pair was stored unless its column's bounds changed in between,
-// which the changed bits route to a fresh gather.
-struct BsEntryCache {
-  DVec<double2> entry_lu;
-  DVec<uint32_t> changed;  // bit per column: tightened or fixed
-  int64_t nnz = -1;
-  int32_t m = -1, n = -1;
-  double zero_tol = 0.0;
-  bool masked = false, valid = false;
-};
-
-// bit j: column j's bounds changed (tightened or fixed); none when lch is null
-__global__ void k_changed_bits(int32_t n, const uint8_t* lch, const uint8_t* uch, const uint8_t* fixed,
-                               uint32_t* words) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool c = j < n && lch && (lch[j] || uch[j] || fixed[j]);
-  const unsigned long long b = __ballot(c);
-  const int lane = static_cast<int>(__lane_id());
-  if ((lane == 0 || lane == 32) && j < n) words[j >> 5] = static_cast<uint32_t>(lane == 0 ? b : b >> 32);
-}
-
+// with two or more live entries that is not an equality.  The entries' (l, u)
+// pairs come from the entry-bounds store where their column is unchanged
+// since the store's last refresh (normally bound strengthening's sweep right
+// before, on the same plan), else from the column arrays.
 struct RedundantFold {
   const uint8_t* keep_row;
   const int32_t* row_nnz;
@@ -401,11 +374,6 @@ struct RedundantFold {
   uint8_t* dl;
   uint8_t* du;
   int32_t* flags;
-  // optional: the (l, u) pairs the bound-strengthening sweep stored per used
-  // entry, valid for columns whose bounds it did not change (changed bits)
-  const double2* entry_lu;
-  Bits changed;
-  double entry_zero_tol;
   struct State {
     Act lo, hi;
   };
@@ -415,10 +383,6 @@ struct RedundantFold {
     return !(isfinite(al) && isfinite(au) && fabs(au - al) <= tol);
   }
   __device__ bool use(double) const { return true; }
-  __device__ void gathered(int64_t, double2) const {}
-  __device__ double2 bounds(int64_t q, int32_t j, double a, const double2* lu) const {
-    return entry_lu && fabs(a) > entry_zero_tol && !changed[j] ? entry_lu[q] : lu[j];
-  }
   __device__ void init(State& s) const {
     s.lo = Act{};
     s.hi = Act{};
@@ -516,24 +480,14 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   {
   // A, (l, u) and a mask bit per column, keep_row, row_nnz, AL, AU, 3 flags out per row
   TimedScope ts(GP_KT_REDUNDANT, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + lp.A.cols / 8.0 + 24.0 * m);
-  // bound strengthening ran right before on this plan and matrix: its stored
-  // per-entry bounds hold for every column it did not change
-  const double2* entry_lu = nullptr;
-  Bits changed{nullptr};
-  if (plan.rule_cache) {
-    if (void* c = static_cast<RuleCaches*>(plan.rule_cache)->bs_entry) {
-      BsEntryCache& ec = *static_cast<BsEntryCache*>(c);
-      if (ec.valid && ec.nnz == lp.A.nnz && ec.m == m && ec.n == lp.A.cols && ec.zero_tol == p.zero_tol &&
-          ec.masked == (s.live_masks != 0)) {
-        entry_lu = ec.entry_lu.data();
-        changed = Bits{ec.changed.data()};
-      }
-      ec.valid = false;
-    }
-  }
+  // the entry-bounds store (refreshed by bound strengthening's sweep right before) for unchanged columns
+  DVec<uint32_t> changed;
+  const StoreView store = store_view(s.live_masks ? plan_store(plan) : nullptr, lp.A, lu.data(), p.zero_tol, false,
+                                     changed);
   launch_row_fold(lp.A, lu.data(), live_cols.view(),
                   RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
-                                dl.data(), du.data(), flags.d.data(), entry_lu, changed, p.zero_tol});
+                                dl.data(), du.data(), flags.d.data()},
+                  store);
   GP_LAUNCH_CHECK("redundant rows");
   }
   const int32_t* f = flags.read();
@@ -918,8 +872,6 @@ struct FixedRowsFold {
   };
   __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
   __device__ bool use(double) const { return true; }
-  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
-  __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.acc = RowAcc{};
     s.fixed = 0;
@@ -962,28 +914,17 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   const PackedBounds lu(plan.new_l, plan.new_u, n);
   // the support-row search (tape only) reads each row's summary as one record
   DVec<BsRow> row_rec(tape_on(plan, p) && s.row_nnz ? static_cast<size_t>(m) : 0);
-  RuleCaches& rcache = rule_caches(plan);
-  if (!rcache.bs_entry) rcache.bs_entry = new BsEntryCache();
-  BsEntryCache& ec = *static_cast<BsEntryCache*>(rcache.bs_entry);
-  ec.valid = false;
-  ec.entry_lu.resize(static_cast<size_t>(lp.A.nnz > 0 ? lp.A.nnz : 1));
-  ec.changed.resize(static_cast<size_t>((n + 31) / 32 > 0 ? (n + 31) / 32 : 1));
-  ec.nnz = lp.A.nnz;
-  ec.m = m;
-  ec.n = n;
-  ec.zero_tol = p.zero_tol;
-  ec.masked = keep_col != nullptr;
-  // marks the stored pairs valid for the next rule, with the changed columns
-  auto publish = [&](const uint8_t* lc, const uint8_t* uc, const uint8_t* fx) {
-    hipLaunchKernelGGL(k_changed_bits, dim3(grid_for((static_cast<int64_t>(n) + 31) / 32 * 32)), dim3(kBlock), 0, 0,
-                       n, lc, uc, fx, ec.changed.data());
-    GP_LAUNCH_CHECK("changed bounds bits");
-    ec.valid = true;
-  };
-  const DVec<double2>& entry_lu = ec.entry_lu;
+  // the sweep refreshes the entry-bounds store, and the candidates read each entry's pair from it
+  EntryBounds* store = plan_store(plan);
+  // without live masks every column counts as live: start the store over, so it covers them all
+  if (!keep_col) store->drop();
+  DVec<uint32_t> changed;
+  const StoreView view = store_view(store, lp.A, lu.data(), p.zero_tol, true, changed);
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
-                       plan.new_AU, s.row_nnz, ec.entry_lu.data());
+                       plan.new_AU, s.row_nnz, view);
+  store_commit(store, lp.A, lu.data(), p.zero_tol);
+  const double2* entry_lu = store->pairs.data();
   {
     const PackedMask live_cols(keep_col, n);
     // A, the entries' (l, u) pairs, keep_row, row_nnz, AL, AU, the row summary (24 B), a mask bit
@@ -991,7 +932,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     TimedScope ts(GP_KT_BOUND_CANDIDATES, 4.0 * (m + 1) + 28.0 * lp.A.nnz + 45.0 * m + 16.0 * n + n / 8.0);
     launch_grouped<BsCandidatesK>(pick_lanes(tuning().lanes_candidates, m ? static_cast<double>(lp.A.nnz) / m : 0.0), m, lp.A,
                                   plan.keep_row_mask, live_cols.view(), s.row_nnz, plan.new_AL, plan.new_AU,
-                                  static_cast<const double2*>(entry_lu.data()), static_cast<const double*>(min_fin.data()),
+                                  entry_lu, static_cast<const double*>(min_fin.data()),
                                   static_cast<const double*>(max_fin.data()),
                                   static_cast<const int32_t*>(min_ninf.data()),
                                   static_cast<const int32_t*>(max_pinf.data()), p.zero_tol, cl.data(), cu.data());
@@ -1061,7 +1002,6 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
     return;
   }
   if (!(f[1] || f[2])) {
-    publish(nullptr, nullptr, nullptr);  // no bounds changed
     return;
   }
   const bool f2 = f[2] != 0;
@@ -1121,7 +1061,6 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   plan.has_row_action = plan.has_row_action || f2;
   plan.has_col_action = plan.has_col_action || f2;
   plan.has_change = 1;
-  publish(lch.data(), uch.data(), fixed.data());
 }
 
 RuleCaches& rule_caches(gp_plan& plan) {
@@ -1129,23 +1068,25 @@ RuleCaches& rule_caches(gp_plan& plan) {
   return *static_cast<RuleCaches*>(plan.rule_cache);
 }
 
+EntryBounds* plan_store(gp_plan& plan) {
+  RuleCaches& rc = rule_caches(plan);
+  if (rc.store) return rc.store;
+  if (!rc.own_store) rc.own_store = new EntryBounds();
+  return rc.own_store;
+}
+
 void col_max_cache_free(void* cache) { delete static_cast<ColMaxCache*>(cache); }
-void bs_entry_cache_free(void* cache) { delete static_cast<BsEntryCache*>(cache); }
 
 void rule_cache_free(void* cache) {
   if (!cache) return;
   RuleCaches* rc = static_cast<RuleCaches*>(cache);
   sc_cache_free(rc->sc);
   col_max_cache_free(rc->col_max);
-  bs_entry_cache_free(rc->bs_entry);
+  delete rc->own_store;
   delete rc;
 }
 
 void apply_rule(int32_t rule, gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
-  // bound strengthening's stored entry bounds serve only the rule right after it
-  if (rule != GP_RULE_REDUNDANT_ROWS && plan.rule_cache) {
-    if (void* c = static_cast<RuleCaches*>(plan.rule_cache)->bs_entry) static_cast<BsEntryCache*>(c)->valid = false;
-  }
   switch (rule) {
     case GP_RULE_FIXED_VARIABLES: rule_fixed_variables(plan, lp, p); break;
     case GP_RULE_EMPTY_COLUMNS: rule_empty_columns(plan, lp, s, p); break;

@@ -60,8 +60,6 @@ struct ScActivityFold {
   };
   __device__ bool active(int64_t i) const { return keep_row[i] != 0; }
   __device__ bool use(double) const { return true; }
-  __device__ double2 bounds(int64_t, int32_t j, double, const double2* lu) const { return lu[j]; }
-  __device__ void gathered(int64_t, double2) const {}
   __device__ void init(State& s) const {
     s.mn = RowAcc{};
     s.mx = RowAcc{};
@@ -1734,10 +1732,18 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
       }
     }
     if (full) {
-      // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
-      TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
-      launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, cache->act.data()});
-      GP_LAUNCH_CHECK("singleton columns row activity");
+      // every live row is folded: the sweep refreshes the entry-bounds store
+      EntryBounds* store = plan_store(plan);
+      DVec<uint32_t> changed;
+      const StoreView view = store_view(store, lp.A, lu.data(), p.zero_tol, true, changed);
+      {
+        // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
+        TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
+        launch_row_fold(lp.A, lu.data(), live_col.view(), ScActivityFold{plan.keep_row_mask, cache->act.data()},
+                        view);
+        GP_LAUNCH_CHECK("singleton columns row activity");
+      }
+      store_commit(store, lp.A, lu.data(), p.zero_tol);
     }
     std::swap(cache->lu, lu.lu);
     d2d(cache->keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));

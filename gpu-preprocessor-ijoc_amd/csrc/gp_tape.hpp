@@ -58,9 +58,13 @@ inline DeviceTape* plan_tape(const gp_plan& plan) { return static_cast<DeviceTap
 struct RuleCaches {
   void* sc = nullptr;        // singleton-column row activities (gp_rules2.hip)
   void* col_max = nullptr;   // column max |a| (gp_rules.hip)
-  void* bs_entry = nullptr;  // per-entry bounds of the last bound strengthening (gp_rules.hip)
+  EntryBounds* store = nullptr;      // the scheduler's entry-bounds store (not owned)
+  EntryBounds* own_store = nullptr;  // this plan's own store when the scheduler gives none
 };
 RuleCaches& rule_caches(gp_plan& plan);  // created on first use
+// The entry-bounds store the plan's row folds use (the scheduler's, else the
+// plan's own, created on first use).
+EntryBounds* plan_store(gp_plan& plan);
 void rule_cache_free(void* cache);
 void sc_cache_free(void* cache);         // gp_rules2.hip
 void col_max_cache_free(void* cache);    // gp_rules.hip

@@ -55,7 +55,8 @@ RECOVERY_VAR_PAIR = 1
 PHASES = ["cleanup", "propagation", "aggregation", "parallel", "end"]
 KERNEL_FAMILIES = ["live_stats", "activity", "bound_candidates", "compact", "transpose",
                    "rules_other", "hash_sort", "sc_activity", "redundant", "col_max", "live_stats_inc",
-                   "cleanup", "singleton_columns", "bounds", "dual_fix", "doubleton", "parallel", "vectors"]
+                   "cleanup", "singleton_columns", "bounds", "dual_fix", "doubleton", "parallel", "vectors",
+                   "entry_bounds"]
 
 
 class GpCsr(C.Structure):
@@ -178,7 +179,7 @@ class GpRoundTrace(C.Structure):
                 ("reserved", C.c_int32)]
 
 
-GP_KT_COUNT = 18
+GP_KT_COUNT = 19
 
 # learned pass scheduler network (gpupre.h GP_MLP_*)
 GP_MLP_IN, GP_MLP_H1, GP_MLP_H2, GP_MLP_OUT = 32, 64, 64, 7

@@ -243,7 +243,8 @@ enum {
   GP_KT_DOUBLETON = 15,       /* doubleton equations                           */
   GP_KT_PARALLEL = 16,        /* parallel rows / columns outside the hashing   */
   GP_KT_VECTORS = 17,         /* vector compaction at materialize              */
-  GP_KT_COUNT = 18
+  GP_KT_ENTRY_BOUNDS = 18,    /* entry-bounds store: changed columns, carry through materialize */
+  GP_KT_COUNT = 19
 };
 
 /* ---- learned pass scheduler (no reference counterpart) --------------------
