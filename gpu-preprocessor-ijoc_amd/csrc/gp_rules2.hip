@@ -2072,11 +2072,318 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
 }
 
 // ---------------------------------------------------------------- doubleton equations
+// ---------------------------------------------------------------- doubleton equations, single row
+// The single-row form (doubleton_equations_scan off,
+// rule_doubleton_equations.cu:1948-2128): one doubleton equation per call,
+// the lowest live equality row with two live entries whose pivot ratio and
+// fill-in pass.  The row is picked by an atomic minimum and its data are then
+// read from that row alone, so the choice never depends on thread timing.
+namespace {
+// fill-in proxy over every entry of the two columns of A^T, live or not
+// (_doubleton_equations_fill_in_proxy_device, :216-243)
+__device__ int32_t db_fill_in_all(const gp_csr& AT, int32_t keep, int32_t elim) {
+  int32_t fill = -1;
+  gp_offset jj = AT.row_ptr[keep], kk = AT.row_ptr[elim];
+  const gp_offset jstop = AT.row_ptr[keep + 1], kstop = AT.row_ptr[elim + 1];
+  if (jstop <= jj || kstop <= kk) return fill;
+  while (jj < jstop && kk < kstop) {
+    const int32_t rj = AT.col_idx[jj], rk = AT.col_idx[kk];
+    if (rj == rk) {
+      ++jj;
+      ++kk;
+    } else if (rk < rj) {
+      ++kk;
+      ++fill;
+    } else {
+      ++jj;
+    }
+  }
+  return fill + static_cast<int32_t>(kstop - kk);
+}
+
+struct DbsArgs {
+  gp_csr A, AT;
+  const uint8_t* keep_row;
+  const uint8_t* keep_col;
+  const double* AL;
+  const double* AU;
+  const int32_t* col_nnz;
+  double zt, bt, ratio_tol;
+  int32_t maxfill;
+};
+
+// row i as a single-row candidate (:273-363): eliminated and kept column
+__device__ bool dbs_candidate(const DbsArgs& g, int64_t i, int32_t& ec, int32_t& kc, double& ev, double& kv) {
+  if (!g.keep_row[i]) return false;
+  const double lhs = g.AL[i], rhs = g.AU[i];
+  if (!isfinite(lhs) || !isfinite(rhs) || fabs(lhs - rhs) > g.bt) return false;
+  int32_t live = 0, c1 = -1, c2 = -1;
+  double v1 = 0.0, v2 = 0.0;
+  for (gp_offset q = g.A.row_ptr[i]; q < g.A.row_ptr[i + 1]; ++q) {
+    const int32_t col = g.A.col_idx[q];
+    const double v = g.A.vals[q];
+    if (g.keep_col[col] && fabs(v) > g.zt) {
+      ++live;
+      if (live == 1) {
+        c1 = col;
+        v1 = v;
+      } else if (live == 2) {
+        c2 = col;
+        v2 = v;
+      } else {
+        return false;
+      }
+    }
+  }
+  if (live != 2) return false;
+  const int32_t* cn = g.col_nnz;
+  const bool i12 = db_integral(v1, v2, g.ratio_tol), i21 = db_integral(v2, v1, g.ratio_tol);
+  if (cn[c1] == 1 && cn[c2] != 1) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else if (cn[c1] != 1 && cn[c2] == 1) {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  } else if (i12 && !i21) {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  } else if (i21 && !i12) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else if (cn[c1] < cn[c2]) {
+    ec = c1; ev = v1; kc = c2; kv = v2;
+  } else {
+    ec = c2; ev = v2; kc = c1; kv = v1;
+  }
+  if (fabs(ev) <= 0.0) return false;
+  const double pr = fabs(kv / ev);
+  if (!(pr <= kDblMaxRatioPivot && pr >= 1.0 / kDblMaxRatioPivot)) return false;
+  if (cn[ec] > cn[kc] + g.maxfill + 1) return false;
+  return db_fill_in_all(g.AT, kc, ec) <= g.maxfill;
+}
+
+__global__ void k_dbs_target(DbsArgs g, int32_t* target) {
+  const int64_t i = gtid();
+  if (i >= g.A.rows) return;
+  int32_t ec, kc;
+  double ev, kv;
+  if (dbs_candidate(g, i, ec, kc, ev, kv)) atomicMin(target, static_cast<int32_t>(i));
+}
+
+struct DbsMeta {
+  int32_t status;  // 0: no candidate, 1: infeasible, 2: applied
+  int32_t row, ec, kc;
+  int64_t at0, at1;  // the eliminated column's entries in A^T
+  double ev, kv, rhs, old_el, old_eu, old_kl, old_ku, kl, ku, elim_obj, alpha, beta, obj;
+};
+
+// the target row's substitution (_kernel_doubleton_equations_apply, :365-461),
+// one lane: mapped bounds of the kept column, then the column and row updates
+__global__ void k_dbs_apply(DbsArgs g, const int32_t* target, double* l, double* u, double* c, uint8_t* keep_row,
+                            uint8_t* keep_col, DbsMeta* meta) {
+  DbsMeta r{};
+  const int32_t row = *target;
+  if (row >= g.A.rows || !dbs_candidate(g, row, r.ec, r.kc, r.ev, r.kv)) {
+    *meta = r;
+    return;
+  }
+  r.row = row;
+  r.rhs = g.AU[row];
+  r.old_el = l[r.ec];
+  r.old_eu = u[r.ec];
+  r.elim_obj = c[r.ec];
+  r.old_kl = l[r.kc];
+  r.old_ku = u[r.kc];
+  r.alpha = -r.kv / r.ev;
+  r.beta = r.rhs / r.ev;
+  double ml = -INFINITY, mu = INFINITY;
+  if (isfinite(r.old_el)) {
+    const double b = (r.old_el - r.beta) / r.alpha;
+    if (r.alpha > 0.0) ml = b;
+    else mu = b;
+  }
+  if (isfinite(r.old_eu)) {
+    const double b = (r.old_eu - r.beta) / r.alpha;
+    if (r.alpha > 0.0) mu = b;
+    else ml = b;
+  }
+  if (ml > mu) {
+    const double t = ml;
+    ml = mu;
+    mu = t;
+  }
+  r.kl = isfinite(ml) ? fmax(r.old_kl, ml) : r.old_kl;
+  r.ku = isfinite(mu) ? fmin(r.old_ku, mu) : r.old_ku;
+  r.at0 = g.AT.row_ptr[r.ec];
+  r.at1 = g.AT.row_ptr[r.ec + 1];
+  if (r.kl > r.ku + g.bt) {
+    r.status = 1;
+    *meta = r;
+    return;
+  }
+  l[r.kc] = r.kl;
+  u[r.kc] = r.ku;
+  c[r.kc] += r.elim_obj * r.alpha;
+  keep_row[row] = 0;
+  keep_col[r.ec] = 0;
+  r.obj = r.elim_obj * r.beta;
+  r.status = 2;
+  *meta = r;
+}
+
+// AL, AU of every row with an entry in the eliminated column: minus that
+// entry (the first for the row) times beta
+__global__ void k_dbs_shift(gp_csr AT, const DbsMeta* meta, double* AL, double* AU) {
+  const int64_t q = meta->at0 + gtid();
+  if (q >= meta->at1) return;
+  const int32_t t = AT.col_idx[q];
+  if (q > meta->at0 && AT.col_idx[q - 1] == t) return;
+  const double shift = AT.vals[q] * meta->beta;
+  AL[t] -= shift;
+  AU[t] -= shift;
+}
+
+__global__ void k_dbs_tape(TapeView v, const DbsMeta* meta) {
+  const DbsMeta& r = *meta;
+  v.types[v.rbase] = GP_RED_DOUBLETON_EQUATION;
+  v.modes[v.rbase] = GP_DUAL_MINIMAL;
+  const int32_t ix[4] = {r.ec, r.kc, r.row, 0};
+  const double vx[12] = {r.ev, r.kv, r.rhs, r.old_el, r.old_eu, r.old_kl, r.old_ku, r.kl, r.ku, r.elim_obj, 0.0, 0.0};
+  for (int k = 0; k < 4; ++k) v.idx[v.ibase + k] = ix[k];
+  for (int k = 0; k < 12; ++k) v.vals[v.vbase + k] = vx[k];
+  v.istarts[v.rbase + 1] = static_cast<gp_offset>(v.ibase + 4);
+  v.vstarts[v.rbase + 1] = static_cast<gp_offset>(v.vbase + 12);
+}
+
+// one row of the rewritten A (_kernel_doubleton_equations_row_counts and
+// _kernel_copy_doubleton_equations_rows, :463-521, 1814-1892): the target row
+// as it is; elsewhere the kept column's entry becomes old + alpha * a_elim,
+// dropped when |.| <= zero_tol and inserted in column order when the row has
+// an eliminated entry and no kept entry; the eliminated column's entries stay
+template <bool kWrite>
+__global__ void k_dbs_rows(gp_csr A, const DbsMeta* meta, double zt, int32_t* counts, const gp_offset* out_ptr,
+                           int32_t* out_col, double* out_val) {
+  const int64_t r = gtid();
+  if (r >= A.rows) return;
+  const int32_t ec = meta->ec, kc = meta->kc;
+  const double alpha = meta->alpha;
+  const gp_offset first = A.row_ptr[r], last = A.row_ptr[r + 1];
+  int64_t w = kWrite ? out_ptr[r] : 0;
+  auto put = [&](int32_t col, double val) {
+    if (kWrite) {
+      out_col[w] = col;
+      out_val[w] = val;
+    }
+    ++w;
+  };
+  if (r == meta->row) {
+    for (gp_offset q = first; q < last; ++q) put(A.col_idx[q], A.vals[q]);
+  } else {
+    bool elim_present = false, keep_present = false;
+    double are = 0.0, old_keep = 0.0;
+    for (gp_offset q = first; q < last; ++q) {
+      if (A.col_idx[q] == ec) {
+        elim_present = true;
+        are = A.vals[q];
+      } else if (A.col_idx[q] == kc) {
+        keep_present = true;
+        old_keep = A.vals[q];
+      }
+    }
+    const double new_keep = old_keep + alpha * are;
+    const bool insert_keep = elim_present && !keep_present && fabs(new_keep) > zt;
+    bool inserted = !insert_keep;
+    for (gp_offset q = first; q < last; ++q) {
+      const int32_t col = A.col_idx[q];
+      if (insert_keep && !inserted && col > kc) {
+        put(kc, new_keep);
+        inserted = true;
+      }
+      if (col == kc) {
+        if (fabs(new_keep) > zt) put(kc, new_keep);
+      } else {
+        put(col, A.vals[q]);
+      }
+    }
+    if (insert_keep && !inserted) put(kc, new_keep);
+  }
+  if (!kWrite) counts[r] = static_cast<int32_t>(w);
+}
+}  // namespace
+
+void rule_doubleton_single_row(gp_plan& plan, const gp_device_lp& lp, const gp_params& p) {
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  // A and A^T sweeps for the counts and the target; c, l, u per column; AL, AU per row
+  TimedScope ts(GP_KT_DOUBLETON, 2.0 * (4.0 * (m + 1) + 12.0 * lp.A.nnz) + 24.0 * n + 16.0 * m);
+  const gp_csr A = plan.has_new_A ? plan.new_A : lp.A;
+  const double zt = p.zero_tol, bt = p.bound_tol;
+  DVec<int32_t> col_nnz(n), target(1);
+  DVec<DbsMeta> meta(1);
+  const PackedMask live_rows(plan.keep_row_mask, m);
+  seg_reduce_tiles(lp.AT, DbColNnzOp{live_rows.view(), plan.keep_col_mask, zt},
+                   reinterpret_cast<uint32_t*>(col_nnz.data()));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, 0, target.data(), m, 1);
+  const DbsArgs g{A, lp.AT, plan.keep_row_mask, plan.keep_col_mask, plan.new_AL, plan.new_AU, col_nnz.data(),
+                  zt, bt, fmax(1.0e-9, bt), p.doubleton_equations_max_fill_in_proxy};
+  hipLaunchKernelGGL(k_dbs_target, dim3(grid_for(m)), dim3(kBlock), 0, 0, g, target.data());
+  hipLaunchKernelGGL(k_dbs_apply, dim3(1), dim3(1), 0, 0, g, static_cast<const int32_t*>(target.data()), plan.new_l,
+                     plan.new_u, plan.new_c, plan.keep_row_mask, plan.keep_col_mask, meta.data());
+  GP_LAUNCH_CHECK("doubleton single row");
+  DbsMeta h;
+  d2h(&h, meta.data(), 1);
+  if (h.status == 0) return;
+  if (h.status == 1) {
+    plan.has_infeasible = 1;
+    return;
+  }
+  if (h.at1 > h.at0) {
+    hipLaunchKernelGGL(k_dbs_shift, dim3(grid_for(h.at1 - h.at0)), dim3(kBlock), 0, 0, lp.AT,
+                       static_cast<const DbsMeta*>(meta.data()), plan.new_AL, plan.new_AU);
+    GP_LAUNCH_CHECK("doubleton single row shift");
+  }
+  if (tape_on(plan, p)) {
+    DeviceTape* tape = plan_tape(plan);
+    tape->reserve_more(1, 4, 12);
+    hipLaunchKernelGGL(k_dbs_tape, dim3(1), dim3(1), 0, 0, tape_view(*tape), static_cast<const DbsMeta*>(meta.data()));
+    GP_LAUNCH_CHECK("doubleton single row tape");
+    tape->commit(1, 4, 12);
+  }
+  DVec<int32_t> cnt(static_cast<size_t>(m) + 1);
+  hipLaunchKernelGGL(k_dbs_rows<false>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A,
+                     static_cast<const DbsMeta*>(meta.data()), zt, cnt.data(), nullptr, nullptr, nullptr);
+  GP_LAUNCH_CHECK("doubleton single row counts");
+  GP_CHECK(hipMemsetAsync(cnt.data() + m, 0, sizeof(int32_t), 0));
+  const int64_t nnz = exclusive_scan_i32_total(cnt.data(), static_cast<int64_t>(m) + 1);
+  if (nnz > INT32_MAX) throw std::length_error("doubleton substitution exceeds int32 offsets");
+  gp_csr out;
+  out.rows = m;
+  out.cols = n;
+  out.nnz = static_cast<gp_offset>(nnz);
+  out.row_ptr = pool_alloc<gp_offset>(static_cast<size_t>(m) + 1);
+  out.col_idx = pool_alloc<int32_t>(static_cast<size_t>(nnz));
+  out.vals = pool_alloc<double>(static_cast<size_t>(nnz));
+  hipLaunchKernelGGL(k_i32_to_offset, dim3(grid_for(m + 1)), dim3(kBlock), 0, 0, cnt.data(), static_cast<int64_t>(m) + 1,
+                     out.row_ptr);
+  hipLaunchKernelGGL(k_dbs_rows<true>, dim3(grid_for(m)), dim3(kBlock), 0, 0, A,
+                     static_cast<const DbsMeta*>(meta.data()), zt, nullptr, out.row_ptr, out.col_idx, out.vals);
+  GP_LAUNCH_CHECK("doubleton single row rewrite");
+  if (plan.has_new_A) {
+    GP_CHECK(hipStreamSynchronize(0));
+    csr_pool_free(plan.new_A);
+  }
+  plan.new_A = out;
+  plan.has_new_A = 1;
+  plan.obj_constant_delta += h.obj;
+  plan.has_row_action = 1;
+  plan.has_col_action = 1;
+  plan.has_change = 1;
+}
+
 void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, const gp_params& p) {
   (void)s;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return;
+  if (!p.doubleton_equations_scan) {
+    rule_doubleton_single_row(plan, lp, p);
+    return;
+  }
   // one pass over A; c, l, u per column; AL, AU per row
   TimedScope ts(GP_KT_DOUBLETON, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 24.0 * n + 16.0 * m);
   const gp_csr A = plan.has_new_A ? plan.new_A : lp.A;

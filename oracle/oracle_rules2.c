@@ -686,10 +686,235 @@ static int dbl_delta_cmp(const void* a, const void* b) {
   return x < y ? -1 : x > y;
 }
 
+/* ======================================================================
+ * doubleton equations, single-row form (doubleton_equations_scan off;
+ * rule_doubleton_equations.cu:1948-2128 with :150-243, 245-571, 1814-1945)
+ * ====================================================================== */
+/* _doubleton_equations_fill_in_proxy_device (:216-243): every entry of the
+ * two columns of A^T counts, live or not */
+static int32_t dbl_fill_in_all(const gp_csr* AT, int32_t keep, int32_t elim) {
+  int32_t fill = -1;
+  gp_offset jj = AT->row_ptr[keep], kk = AT->row_ptr[elim];
+  const gp_offset jstop = AT->row_ptr[keep + 1], kstop = AT->row_ptr[elim + 1];
+  if (jstop <= jj || kstop <= kk) return fill;
+  while (jj < jstop && kk < kstop) {
+    const int32_t rj = AT->col_idx[jj], rk = AT->col_idx[kk];
+    if (rj == rk) {
+      ++jj;
+      ++kk;
+    } else if (rk < rj) {
+      ++kk;
+      ++fill;
+    } else {
+      ++jj;
+    }
+  }
+  return fill + (int32_t)(kstop - kk);
+}
+
+/* The single-row form eliminates one doubleton equation per call: the lowest
+ * live equality row with two live entries whose pivot and fill-in pass
+ * (:273-363).  The reference picks that row by atomicMin and lets every
+ * thread that lowered the minimum write the row's data; here the data are
+ * those of the minimum row, the selection the reference's kernel intends. */
+static int dbl_single_row(gp_plan* plan, const gp_device_lp* lp, const gp_params* p) {
+  const int32_t m = lp->A.rows, n = lp->A.cols;
+  const gp_csr* A = plan->has_new_A ? &plan->new_A : &lp->A;
+  const gp_csr* AT = &lp->AT;
+  const double zt = p->zero_tol, bt = p->bound_tol;
+  const double ratio_tol = fmax(1.0e-9, bt);
+  uint8_t* keep_row = plan->keep_row_mask;
+  uint8_t* keep_col = plan->keep_col_mask;
+  /* live column counts (:245-271) */
+  int32_t* col_nnz = (int32_t*)or_xcalloc((size_t)n, sizeof(int32_t));
+  for (int32_t j = 0; j < n; ++j) {
+    if (!keep_col[j]) continue;
+    int32_t c = 0;
+    for (gp_offset q = AT->row_ptr[j]; q < AT->row_ptr[j + 1]; ++q)
+      if (keep_row[AT->col_idx[q]] && fabs(AT->vals[q]) > zt) ++c;
+    col_nnz[j] = c;
+  }
+  int32_t target = -1, ec = -1, kc = -1;
+  double ev = 0.0, kv = 0.0;
+  for (int32_t i = 0; i < m && target < 0; ++i) {
+    if (!keep_row[i]) continue;
+    const double lhs = plan->new_AL[i], rhs = plan->new_AU[i];
+    if (!isfinite(lhs) || !isfinite(rhs) || fabs(lhs - rhs) > bt) continue;
+    int32_t live = 0, c1 = -1, c2 = -1;
+    double v1 = 0.0, v2 = 0.0;
+    for (gp_offset q = A->row_ptr[i]; q < A->row_ptr[i + 1]; ++q) {
+      const int32_t col = A->col_idx[q];
+      const double v = A->vals[q];
+      if (keep_col[col] && fabs(v) > zt) {
+        ++live;
+        if (live == 1) {
+          c1 = col;
+          v1 = v;
+        } else if (live == 2) {
+          c2 = col;
+          v2 = v;
+        } else {
+          break;
+        }
+      }
+    }
+    if (live != 2) continue;
+    /* _choose_doubleton_equations_columns (:170-214) */
+    const int i12 = dbl_integral(v1, v2, ratio_tol), i21 = dbl_integral(v2, v1, ratio_tol);
+    int32_t e, k;
+    double evv, kvv;
+    if (col_nnz[c1] == 1 && col_nnz[c2] != 1) {
+      e = c1, evv = v1, k = c2, kvv = v2;
+    } else if (col_nnz[c1] != 1 && col_nnz[c2] == 1) {
+      e = c2, evv = v2, k = c1, kvv = v1;
+    } else if (i12 && !i21) {
+      e = c2, evv = v2, k = c1, kvv = v1;
+    } else if (i21 && !i12) {
+      e = c1, evv = v1, k = c2, kvv = v2;
+    } else if (col_nnz[c1] < col_nnz[c2]) {
+      e = c1, evv = v1, k = c2, kvv = v2;
+    } else {
+      e = c2, evv = v2, k = c1, kvv = v1;
+    }
+    /* _is_acceptable_doubleton_equations_pivot (:161-168) */
+    if (fabs(evv) <= 0.0) continue;
+    const double pr = fabs(kvv / evv);
+    if (!(pr <= DBL_MAX_RATIO_PIVOT && pr >= 1.0 / DBL_MAX_RATIO_PIVOT)) continue;
+    if (col_nnz[e] > col_nnz[k] + p->doubleton_equations_max_fill_in_proxy + 1) continue;
+    if (dbl_fill_in_all(AT, k, e) > p->doubleton_equations_max_fill_in_proxy) continue;
+    target = i, ec = e, kc = k, ev = evv, kv = kvv;
+  }
+  free(col_nnz);
+  if (target < 0) return 0;
+  /* _kernel_doubleton_equations_apply (:365-461) */
+  const double rhs = plan->new_AU[target];
+  const double old_el = plan->new_l[ec], old_eu = plan->new_u[ec], elim_obj = plan->new_c[ec];
+  const double old_kl = plan->new_l[kc], old_ku = plan->new_u[kc];
+  const double alpha = -kv / ev, beta = rhs / ev;
+  double ml = -INFINITY, mu = INFINITY;
+  if (isfinite(old_el)) {
+    const double b = (old_el - beta) / alpha;
+    if (alpha > 0.0) ml = b;
+    else mu = b;
+  }
+  if (isfinite(old_eu)) {
+    const double b = (old_eu - beta) / alpha;
+    if (alpha > 0.0) mu = b;
+    else ml = b;
+  }
+  if (ml > mu) {
+    const double tmp = ml;
+    ml = mu;
+    mu = tmp;
+  }
+  const double kl = isfinite(ml) ? fmax(old_kl, ml) : old_kl;
+  const double ku = isfinite(mu) ? fmin(old_ku, mu) : old_ku;
+  if (kl > ku + bt) {
+    plan->has_infeasible = 1;
+    return 0;
+  }
+  /* every row: the first entry of the eliminated column in that row, times beta */
+  for (int32_t t = 0; t < m; ++t) {
+    double shift = 0.0;
+    for (gp_offset q = AT->row_ptr[ec]; q < AT->row_ptr[ec + 1]; ++q) {
+      if (AT->col_idx[q] == t) {
+        shift = AT->vals[q] * beta;
+        break;
+      }
+    }
+    plan->new_AL[t] -= shift;
+    plan->new_AU[t] -= shift;
+  }
+  plan->new_l[kc] = kl;
+  plan->new_u[kc] = ku;
+  plan->new_c[kc] += elim_obj * alpha;
+  keep_row[target] = 0;
+  keep_col[ec] = 0;
+  /* DoubletonEquation record (:2059-2101; capture_scalar_recovery :571-575) */
+  if (tape_on(plan, p)) {
+    const int32_t ix[4] = {ec, kc, target, 0};
+    const double vx[12] = {ev, kv, rhs, old_el, old_eu, old_kl, old_ku, kl, ku, elim_obj, 0.0, 0.0};
+    or_tape_push((or_tape*)plan->tape, GP_RED_DOUBLETON_EQUATION, GP_DUAL_MINIMAL, ix, 4, vx, 12);
+  }
+  /* build_doubleton_scalar_new_A (:1894-1945) with
+   * _kernel_doubleton_equations_row_counts / _kernel_copy_doubleton_equations_rows
+   * (:463-521, 1814-1892): the target row is copied as it is; in every other
+   * row the keep entry becomes old_keep + alpha * a_elim (dropped when
+   * |.| <= zero_tol, inserted in column order when the row had an eliminated
+   * entry but no keep entry); the eliminated column's entries stay */
+  gp_csr out;
+  out.rows = m;
+  out.cols = n;
+  out.row_ptr = (gp_offset*)or_xcalloc((size_t)m + 1, sizeof(gp_offset));
+  out.col_idx = (int32_t*)or_xmalloc(sizeof(int32_t) * (size_t)(A->nnz + m + 1));
+  out.vals = (double*)or_xmalloc(sizeof(double) * (size_t)(A->nnz + m + 1));
+  gp_offset w = 0;
+  for (int32_t r = 0; r < m; ++r) {
+    const gp_offset first = A->row_ptr[r], last = A->row_ptr[r + 1];
+    if (r == target) {
+      for (gp_offset q = first; q < last; ++q) {
+        out.col_idx[w] = A->col_idx[q];
+        out.vals[w++] = A->vals[q];
+      }
+      out.row_ptr[r + 1] = w;
+      continue;
+    }
+    int elim_present = 0, keep_present = 0;
+    double are = 0.0, old_keep = 0.0;
+    for (gp_offset q = first; q < last; ++q) {
+      if (A->col_idx[q] == ec) {
+        elim_present = 1;
+        are = A->vals[q];
+      } else if (A->col_idx[q] == kc) {
+        keep_present = 1;
+        old_keep = A->vals[q];
+      }
+    }
+    const double new_keep = old_keep + alpha * are;
+    const int insert_keep = elim_present && !keep_present && fabs(new_keep) > zt;
+    int inserted = !insert_keep;
+    for (gp_offset q = first; q < last; ++q) {
+      const int32_t col = A->col_idx[q];
+      if (insert_keep && !inserted && col > kc) {
+        out.col_idx[w] = kc;
+        out.vals[w++] = new_keep;
+        inserted = 1;
+      }
+      if (col == kc) {
+        if (fabs(new_keep) > zt) {
+          out.col_idx[w] = kc;
+          out.vals[w++] = new_keep;
+        }
+      } else {
+        out.col_idx[w] = col;
+        out.vals[w++] = A->vals[q];
+      }
+    }
+    if (insert_keep && !inserted) {
+      out.col_idx[w] = kc;
+      out.vals[w++] = new_keep;
+    }
+    out.row_ptr[r + 1] = w;
+  }
+  out.nnz = w;
+  if (plan->has_new_A) or_csr_free(&plan->new_A);
+  plan->new_A = out;
+  plan->has_new_A = 1;
+  plan->obj_constant_delta += elim_obj * beta;
+  plan->has_row_action = 1;
+  plan->has_col_action = 1;
+  plan->has_change = 1;
+  return 0;
+}
+
 int or_rule_doubleton_equations(gp_plan* plan, const gp_device_lp* lp, const gp_stats* s,
                                 const gp_params* p) {
   (void)s;
   if (plan->has_infeasible || plan->has_unbounded) return 0;
+  if (!p->doubleton_equations_scan) {
+    if (lp->A.rows == 0 || lp->A.cols == 0) return 0;
+    return dbl_single_row(plan, lp, p);
+  }
   const int32_t m = lp->A.rows, n = lp->A.cols;
   if (m == 0 || n == 0) return 0;
   const gp_csr* A = plan->has_new_A ? &plan->new_A : &lp->A;
