@@ -174,21 +174,24 @@ def fp64_verdict_leg(seed):
 
 def batched_leg(count, seed, serial=True):
     """Batched presolve of `count` LPs (rows log-uniform in [1e3, 1e5], columns
-    1.5 x rows, uniform degree 6, integer-valued FP64 data as in the large
-    workload) resident in HBM: one gp_preprocess_batch against the plain path
-    presolving the same LPs one after another."""
+    1.5 x rows, uniform degree 6, FP64 N(0,1) coefficients) resident in HBM:
+    one gp_preprocess_batch against the plain path presolving the same LPs one
+    after another.  An instance a rule finds infeasible or unbounded retires
+    alone (reported under "retired")."""
+    import numpy as np
     import gpupre as G
 
     t = time.time()
-    lps = G.batch_workload(count, seed, integral=1)
+    lps = G.batch_workload(count, seed, integral=0)
     dl = [G.DeviceLP(lp) for lp in lps]
     nnz = sum(lp.nnz for lp in lps)
     log(f"batched workload: {count} LPs, {sum(lp.m for lp in lps)} rows, {nnz} nnz "
         f"(generated + uploaded in {time.time() - t:.1f}s)")
     params = G.parity_params()
-    # launch schedule tuned on the assembled direct sum (presolve off assembles it)
+    # launch schedule tuned on the direct sum the batch presolves (without the
+    # instances it retires), which the first batch run assembles
     t = time.time()
-    b0 = G.preprocess_batch(dl, G.parity_params(enable_presolve=0))
+    b0 = G.preprocess_batch(dl, params)
     tune, tuned_ms, default_ms = G.autotune(b0.s.assembled, params, reps=1)
     b0.free()
     G.set_tuning(tune)
@@ -205,6 +208,9 @@ def batched_leg(count, seed, serial=True):
         red = (w.reduced_rows, w.reduced_cols, w.reduced_nnz, w.iterations,
                "infeasible" if w.has_infeasible else "unbounded" if w.has_unbounded else "ok")
         asm, pre = b.s.assemble_seconds, b.s.presolve_seconds
+        st = b.status()
+        retired = {int(k): ("infeasible" if st[k] == 1 else "unbounded") for k in np.nonzero(st)[0]}
+        restarts = b.restarts
         b.free()
     batch_s = min(walls)
     G.set_tuning()
@@ -227,6 +233,7 @@ def batched_leg(count, seed, serial=True):
             "assembly_ms": round(asm * 1e3, 2), "latency_ms_p50": round(batch_s * 1e3, 2),
             "latency_ms_p99": round(batch_s * 1e3, 2),
             "reduced": {"rows": red[0], "cols": red[1], "nnz": red[2], "iterations": red[3], "status": red[4]},
+            "retired": retired, "restarts": restarts,
             "plain_serial": {"lps_per_s": round(count / serial_s, 2), "ms": round(serial_s * 1e3, 2),
                              "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)} if serial else None,
             "speedup_vs_plain": round(serial_s / batch_s, 2) if serial else None,

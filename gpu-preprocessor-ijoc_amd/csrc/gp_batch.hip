@@ -14,8 +14,10 @@
 //    (reduction keeps the block order), and extraction of one reduced
 //    instance as its own LP.
 // The scheduler's decisions (phase changes, gates, limits) are taken for the
-// batch as a whole; an infeasible or unbounded instance ends the batch and is
-// reported in the batch's flags.
+// batch as a whole.  A rule that finds an instance infeasible or unbounded
+// marks that instance (BatchMarks); the batch then starts over without the
+// marked instances, which are reported in the summary's status, so one such
+// instance never ends the batch for the others.
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -28,6 +30,19 @@ namespace gp {
 BatchContext*& batch_context() {
   thread_local BatchContext* ctx = nullptr;
   return ctx;
+}
+
+BatchMarks batch_marks() {
+  BatchMarks b;
+  const BatchContext* bc = batch_context();
+  if (!bc || bc->count <= 0 || !bc->row_map || !bc->col_map || bc->marks.size() == 0) return b;
+  b.row_map = bc->row_map->data();
+  b.col_map = bc->col_map->data();
+  b.row_off = bc->row_off.data();
+  b.col_off = bc->col_off.data();
+  b.count = bc->count;
+  b.marks = const_cast<int32_t*>(bc->marks.data());
+  return b;
 }
 
 namespace {
@@ -264,67 +279,7 @@ int gp_preprocess_batch(const gp_device_lp* instances, int32_t count, const gp_p
     if (count < 1) throw std::invalid_argument("batch needs at least one instance");
     std::memset(out, 0, sizeof(*out));
     const double t0 = now_s();
-    std::vector<int64_t> ro(count + 1, 0), co(count + 1, 0), no(count + 1, 0);
-    double obj0 = 0.0;
-    for (int32_t k = 0; k < count; ++k) {
-      validate_instance(instances[k]);
-      ro[k + 1] = ro[k] + instances[k].A.rows;
-      co[k + 1] = co[k] + instances[k].A.cols;
-      no[k + 1] = no[k] + instances[k].A.nnz;
-    }
-    if (ro[count] > INT32_MAX || co[count] > INT32_MAX || no[count] > INT32_MAX)
-      throw std::length_error("batch exceeds int32 rows, columns or offsets");
-    const int32_t m = static_cast<int32_t>(ro[count]), n = static_cast<int32_t>(co[count]);
-    const gp_offset nnz = static_cast<gp_offset>(no[count]);
-    gp_device_lp& lp = owned.lp;
-    lp.A = gp_csr{m, n, nnz, pool_alloc<gp_offset>(static_cast<size_t>(m) + 1), pool_alloc<int32_t>(nnz),
-                  pool_alloc<double>(nnz)};
-    lp.AT = gp_csr{n, m, nnz, pool_alloc<gp_offset>(static_cast<size_t>(n) + 1), pool_alloc<int32_t>(nnz),
-                   pool_alloc<double>(nnz)};
-    lp.c = pool_alloc<double>(n);
-    lp.l = pool_alloc<double>(n);
-    lp.u = pool_alloc<double>(n);
-    lp.AL = pool_alloc<double>(m);
-    lp.AU = pool_alloc<double>(m);
-    {
-      std::vector<Seg<gp_offset>> rp, tp;
-      std::vector<Seg<int32_t>> ci, ti;
-      std::vector<Seg<double>> av, tv, c, l, u, al, au;
-      for (int32_t k = 0; k < count; ++k) {
-        const gp_device_lp& in = instances[k];
-        const gp_offset nb = static_cast<gp_offset>(no[k]);
-        rp.push_back({in.A.row_ptr, in.A.rows, ro[k], nb});
-        tp.push_back({in.AT.row_ptr, in.AT.rows, co[k], nb});
-        ci.push_back({in.A.col_idx, in.A.nnz, no[k], static_cast<int32_t>(co[k])});
-        ti.push_back({in.AT.col_idx, in.AT.nnz, no[k], static_cast<int32_t>(ro[k])});
-        av.push_back({in.A.vals, in.A.nnz, no[k], 0.0});
-        tv.push_back({in.AT.vals, in.AT.nnz, no[k], 0.0});
-        c.push_back({in.c, in.A.cols, co[k], 0.0});
-        l.push_back({in.l, in.A.cols, co[k], 0.0});
-        u.push_back({in.u, in.A.cols, co[k], 0.0});
-        al.push_back({in.AL, in.A.rows, ro[k], 0.0});
-        au.push_back({in.AU, in.A.rows, ro[k], 0.0});
-        obj0 += in.obj_constant;
-      }
-      seg_copy(rp, lp.A.row_ptr);
-      seg_copy(tp, lp.AT.row_ptr);
-      hipLaunchKernelGGL(k_put_offset, dim3(1), dim3(1), 0, 0, lp.A.row_ptr, static_cast<int64_t>(m), nnz);
-      hipLaunchKernelGGL(k_put_offset, dim3(1), dim3(1), 0, 0, lp.AT.row_ptr, static_cast<int64_t>(n), nnz);
-      GP_LAUNCH_CHECK("batch last offsets");
-      seg_copy(ci, lp.A.col_idx);
-      seg_copy(ti, lp.AT.col_idx);
-      seg_copy(av, lp.A.vals);
-      seg_copy(tv, lp.AT.vals);
-      seg_copy(c, lp.c);
-      seg_copy(l, lp.l);
-      seg_copy(u, lp.u);
-      seg_copy(al, lp.AL);
-      seg_copy(au, lp.AU);
-    }
-    lp.obj_constant = obj0;
-    GP_CHECK(hipDeviceSynchronize());
-    const double t1 = now_s();
-
+    for (int32_t k = 0; k < count; ++k) validate_instance(instances[k]);
     out->count = count;
     out->row_offsets = host_array<int32_t>(count + 1);
     out->col_offsets = host_array<int32_t>(count + 1);
@@ -332,20 +287,107 @@ int gp_preprocess_batch(const gp_device_lp* instances, int32_t count, const gp_p
     out->reduced_col_offsets = host_array<int32_t>(count + 1);
     out->reduced_nnz_offsets = host_array<gp_offset>(count + 1);
     out->obj_constant = host_array<double>(count);
-    for (int32_t k = 0; k <= count; ++k) {
-      out->row_offsets[k] = static_cast<int32_t>(ro[k]);
-      out->col_offsets[k] = static_cast<int32_t>(co[k]);
+    out->status = host_array<int32_t>(count);
+    double assemble = 0.0, presolve = 0.0;
+    // An instance that a rule finds infeasible or unbounded retires alone: the
+    // batch starts over without it (as an empty block) and reports it in
+    // status[]; every other instance is presolved together as before.
+    for (;;) {
+      const double ta = now_s();
+      std::vector<int64_t> ro(count + 1, 0), co(count + 1, 0), no(count + 1, 0);
+      for (int32_t k = 0; k < count; ++k) {
+        const bool on = out->status[k] == 0;
+        ro[k + 1] = ro[k] + (on ? instances[k].A.rows : 0);
+        co[k + 1] = co[k] + (on ? instances[k].A.cols : 0);
+        no[k + 1] = no[k] + (on ? instances[k].A.nnz : 0);
+      }
+      if (ro[count] > INT32_MAX || co[count] > INT32_MAX || no[count] > INT32_MAX)
+        throw std::length_error("batch exceeds int32 rows, columns or offsets");
+      const int32_t m = static_cast<int32_t>(ro[count]), n = static_cast<int32_t>(co[count]);
+      const gp_offset nnz = static_cast<gp_offset>(no[count]);
+      gp_device_lp& lp = owned.lp;
+      lp.A = gp_csr{m, n, nnz, pool_alloc<gp_offset>(static_cast<size_t>(m) + 1), pool_alloc<int32_t>(nnz),
+                    pool_alloc<double>(nnz)};
+      lp.AT = gp_csr{n, m, nnz, pool_alloc<gp_offset>(static_cast<size_t>(n) + 1), pool_alloc<int32_t>(nnz),
+                     pool_alloc<double>(nnz)};
+      lp.c = pool_alloc<double>(n);
+      lp.l = pool_alloc<double>(n);
+      lp.u = pool_alloc<double>(n);
+      lp.AL = pool_alloc<double>(m);
+      lp.AU = pool_alloc<double>(m);
+      double obj0 = 0.0;
+      {
+        std::vector<Seg<gp_offset>> rp, tp;
+        std::vector<Seg<int32_t>> ci, ti;
+        std::vector<Seg<double>> av, tv, c, l, u, al, au;
+        for (int32_t k = 0; k < count; ++k) {
+          if (out->status[k] != 0) continue;
+          const gp_device_lp& in = instances[k];
+          const gp_offset nb = static_cast<gp_offset>(no[k]);
+          rp.push_back({in.A.row_ptr, in.A.rows, ro[k], nb});
+          tp.push_back({in.AT.row_ptr, in.AT.rows, co[k], nb});
+          ci.push_back({in.A.col_idx, in.A.nnz, no[k], static_cast<int32_t>(co[k])});
+          ti.push_back({in.AT.col_idx, in.AT.nnz, no[k], static_cast<int32_t>(ro[k])});
+          av.push_back({in.A.vals, in.A.nnz, no[k], 0.0});
+          tv.push_back({in.AT.vals, in.AT.nnz, no[k], 0.0});
+          c.push_back({in.c, in.A.cols, co[k], 0.0});
+          l.push_back({in.l, in.A.cols, co[k], 0.0});
+          u.push_back({in.u, in.A.cols, co[k], 0.0});
+          al.push_back({in.AL, in.A.rows, ro[k], 0.0});
+          au.push_back({in.AU, in.A.rows, ro[k], 0.0});
+          obj0 += in.obj_constant;
+        }
+        seg_copy(rp, lp.A.row_ptr);
+        seg_copy(tp, lp.AT.row_ptr);
+        hipLaunchKernelGGL(k_put_offset, dim3(1), dim3(1), 0, 0, lp.A.row_ptr, static_cast<int64_t>(m), nnz);
+        hipLaunchKernelGGL(k_put_offset, dim3(1), dim3(1), 0, 0, lp.AT.row_ptr, static_cast<int64_t>(n), nnz);
+        GP_LAUNCH_CHECK("batch last offsets");
+        seg_copy(ci, lp.A.col_idx);
+        seg_copy(ti, lp.AT.col_idx);
+        seg_copy(av, lp.A.vals);
+        seg_copy(tv, lp.AT.vals);
+        seg_copy(c, lp.c);
+        seg_copy(l, lp.l);
+        seg_copy(u, lp.u);
+        seg_copy(al, lp.AL);
+        seg_copy(au, lp.AU);
+      }
+      lp.obj_constant = obj0;
+      GP_CHECK(hipDeviceSynchronize());
+      const double tb = now_s();
+      assemble += tb - ta;
+
+      for (int32_t k = 0; k <= count; ++k) {
+        out->row_offsets[k] = static_cast<int32_t>(ro[k]);
+        out->col_offsets[k] = static_cast<int32_t>(co[k]);
+      }
+      ctx.count = count;
+      ctx.row_off.resize(count + 1);
+      ctx.col_off.resize(count + 1);
+      ctx.marks.resize(count);
+      h2d(ctx.row_off.data(), out->row_offsets, count + 1);
+      h2d(ctx.col_off.data(), out->col_offsets, count + 1);
+      GP_CHECK(hipMemsetAsync(ctx.marks.data(), 0, sizeof(int32_t) * static_cast<size_t>(count), 0));
+      ctx.obj.assign(count, 0.0);
+      batch_context() = &ctx;
+      const int prc = gp_preprocess(&lp, params, &out->whole);
+      batch_context() = nullptr;
+      if (prc != GP_OK) throw Error(prc, std::string("batched presolve: ") + gp_last_error());
+      presolve += now_s() - tb;
+      if (!(out->whole.has_infeasible || out->whole.has_unbounded)) break;
+      std::vector<int32_t> marks(static_cast<size_t>(count));
+      d2h(marks.data(), ctx.marks.data(), marks.size());
+      bool retired = false;
+      for (int32_t k = 0; k < count; ++k) {
+        if (marks[k] == 0 || out->status[k] != 0) continue;
+        out->status[k] = (marks[k] & 1) ? 1 : 2;
+        retired = true;
+      }
+      if (!retired) break;  // a stage without instance marks ended the batch: reported as a whole
+      gp_free_summary(&out->whole);
+      owned.free();
+      ++out->restarts;
     }
-    ctx.count = count;
-    ctx.row_off.resize(count + 1);
-    ctx.col_off.resize(count + 1);
-    h2d(ctx.row_off.data(), out->row_offsets, count + 1);
-    h2d(ctx.col_off.data(), out->col_offsets, count + 1);
-    ctx.obj.assign(count, 0.0);
-    batch_context() = &ctx;
-    const int prc = gp_preprocess(&lp, params, &out->whole);
-    batch_context() = nullptr;
-    if (prc != GP_OK) throw Error(prc, std::string("batched presolve: ") + gp_last_error());
     const double t2 = now_s();
 
     // split: instance k's rows / columns in the reduced direct sum
@@ -363,11 +405,12 @@ int gp_preprocess_batch(const gp_device_lp* instances, int32_t count, const gp_p
     d2h(out->reduced_col_offsets, rc.data(), count + 1);
     d2h(out->reduced_nnz_offsets, rn.data(), count + 1);
     for (int32_t k = 0; k < count; ++k) out->obj_constant[k] = instances[k].obj_constant + ctx.obj[k];
-    out->assembled = lp;
+    out->assembled = owned.lp;
     owned.lp = gp_device_lp{};
-    out->assemble_seconds = t1 - t0;
-    out->presolve_seconds = t2 - t1;
+    out->assemble_seconds = assemble;
+    out->presolve_seconds = presolve;
     out->total_seconds = now_s() - t0;
+    (void)t2;
   });
   batch_context() = nullptr;
   if (rc != GP_OK) {
@@ -446,6 +489,7 @@ void gp_free_batch_summary(gp_batch_summary* s) {
   std::free(s->reduced_col_offsets);
   std::free(s->reduced_nnz_offsets);
   std::free(s->obj_constant);
+  std::free(s->status);
   std::memset(s, 0, sizeof(*s));
 }
 

@@ -749,9 +749,41 @@ struct BatchContext {
   const DVec<int32_t>* row_map = nullptr;  // physical -> original row of the direct sum
   const DVec<int32_t>* col_map = nullptr;
   DVec<int32_t> row_off, col_off;          // [count+1] original offsets (device)
+  DVec<int32_t> marks;                     // [count] infeasible / unbounded marks (device)
   std::vector<double> obj;                 // per-instance objective deltas
 };
 BatchContext*& batch_context();
+// Instances a rule finds infeasible (bit 1) or unbounded (bit 2) in a batched
+// presolve: the kernels that raise those flags also mark the instance that
+// holds the row or column (current line -> original line -> instance), so the
+// batch can retire exactly those instances.  Inactive (marks null) outside a
+// batch.
+struct BatchMarks {
+  const int32_t* row_map = nullptr;  // current -> original row of the direct sum
+  const int32_t* col_map = nullptr;
+  const int32_t* row_off = nullptr;  // [count+1] original offsets
+  const int32_t* col_off = nullptr;
+  int32_t count = 0;
+  int32_t* marks = nullptr;  // [count]
+  __device__ void row(int64_t i, int32_t what) const {
+    if (marks) put(row_off, row_map[i], what);
+  }
+  __device__ void col(int64_t j, int32_t what) const {
+    if (marks) put(col_off, col_map[j], what);
+  }
+  // the last instance whose first line is at or before `line` (empty
+  // instances share their offset with the next one)
+  __device__ void put(const int32_t* off, int32_t line, int32_t what) const {
+    int32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (off[mid] <= line) lo = mid;
+      else hi = mid;
+    }
+    atomicOr(marks + lo, what);
+  }
+};
+BatchMarks batch_marks();  // gp_batch.hip: the current batch's marks, inactive outside a batch
 // Objective delta of a rule: the fixed-order det_sum of contrib[0..n).  In a
 // batch the contributions are also summed per instance, in item order: item i
 // belongs to physical line item_line[i] (or i when item_line is null), a row

@@ -126,7 +126,7 @@ void rule_fixed_variables(gp_plan& plan, const gp_device_lp& lp, const gp_params
 namespace {
 __global__ void k_empty_cols(int32_t n, const uint8_t* keep_col, const uint8_t* empty, const double* c,
                              const double* l, const double* u, double bound_tol, double zero_tol, uint8_t* fixed,
-                             double* val, int32_t* flags) {
+                             double* val, int32_t* flags, BatchMarks bm) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   fixed[j] = 0;
@@ -135,10 +135,12 @@ __global__ void k_empty_cols(int32_t n, const uint8_t* keep_col, const uint8_t* 
   const double cj = c[j], lj = l[j], uj = u[j];
   if (lj > uj + bound_tol) {
     wave_flag(flags + 0, 1);
+    bm.col(j, 1);
     return;
   }
   if ((cj > zero_tol && !isfinite(lj)) || (cj < -zero_tol && !isfinite(uj))) {
     wave_flag(flags + 1, 1);
+    bm.col(j, 2);
     return;
   }
   double v = 0.0;
@@ -163,7 +165,7 @@ void rule_empty_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
   Flags flags;
   hipLaunchKernelGGL(k_empty_cols, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, plan.keep_col_mask, s.empty_col_mask,
                      plan.new_c, plan.new_l, plan.new_u, p.bound_tol, p.zero_tol, fixed.data(), val.data(),
-                     flags.d.data());
+                     flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("empty columns");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -186,11 +188,15 @@ void rule_empty_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
 // ---------------------------------------------------------------- empty rows
 namespace {
 __global__ void k_empty_rows(int32_t m, const uint8_t* keep_row, const double* AL, const double* AU,
-                             const int32_t* row_nnz, double tol, int32_t* flags) {
+                             const int32_t* row_nnz, double tol, int32_t* flags, BatchMarks bm) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m || !(keep_row[i] && row_nnz[i] == 0)) return;
-  if (AL[i] <= tol && AU[i] >= -tol) wave_flag(flags + 1, 1);
-  else wave_flag(flags + 0, 1);
+  if (AL[i] <= tol && AU[i] >= -tol) {
+    wave_flag(flags + 1, 1);
+  } else {
+    wave_flag(flags + 0, 1);
+    bm.row(i, 1);
+  }
 }
 __global__ void k_empty_rows_apply(int32_t m, uint8_t* keep_row, const double* AL, const double* AU,
                                    const int32_t* row_nnz, double tol) {
@@ -208,7 +214,7 @@ void rule_empty_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, c
   TimedScope ts(GP_KT_CLEANUP, 21.0 * m);
   Flags flags;
   hipLaunchKernelGGL(k_empty_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, plan.keep_row_mask, plan.new_AL,
-                     plan.new_AU, s.row_nnz, p.feasibility_tol, flags.d.data());
+                     plan.new_AU, s.row_nnz, p.feasibility_tol, flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("empty rows");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -245,10 +251,13 @@ __global__ void k_singleton_rows(int32_t m, int32_t n, const uint8_t* keep_row, 
   remove[i] = 1;
 }
 __global__ void k_singleton_check(int32_t m, int32_t n, const double* cl, const double* cu, const double* l,
-                                  const double* u, const uint8_t* remove, double tol, int32_t* flags) {
+                                  const double* u, const uint8_t* remove, double tol, int32_t* flags, BatchMarks bm) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < n) {
-    if (cl[j] > cu[j] + tol) wave_flag(flags + 0, 1);
+    if (cl[j] > cu[j] + tol) {
+      wave_flag(flags + 0, 1);
+      bm.col(j, 1);
+    }
     if (cl[j] != l[j] || cu[j] != u[j]) wave_flag(flags + 2, 1);
   }
   if (j < m && remove[j]) wave_flag(flags + 1, 1);
@@ -322,7 +331,7 @@ void rule_singleton_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   Flags flags;
   const int32_t mn = m > n ? m : n;
   hipLaunchKernelGGL(k_singleton_check, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, cl.data(), cu.data(),
-                     plan.new_l, plan.new_u, remove.data(), p.bound_tol, flags.d.data());
+                     plan.new_l, plan.new_u, remove.data(), p.bound_tol, flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("singleton rows check");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -374,6 +383,7 @@ struct RedundantFold {
   uint8_t* dl;
   uint8_t* du;
   int32_t* flags;
+  BatchMarks bm;
   struct State {
     Act lo, hi;
   };
@@ -402,6 +412,7 @@ struct RedundantFold {
     const double rmin = s.lo.value(), rmax = s.hi.value();
     if ((lf && rmax < al - tol) || (uf && rmin > au + tol)) {
       wave_flag(flags + 0, 1);
+      bm.row(i, 1);
       return;
     }
     const bool li = !lf || rmin >= al - tol;
@@ -486,7 +497,7 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
                                      changed);
   launch_row_fold(lp.A, lu.data(), live_cols.view(),
                   RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
-                                dl.data(), du.data(), flags.d.data()},
+                                dl.data(), du.data(), flags.d.data(), batch_marks()},
                   store);
   GP_LAUNCH_CHECK("redundant rows");
   }
@@ -654,7 +665,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
                               const double* cl_in, const double* cu_in, const double* ref_l, const double* ref_u,
                               const double* c, const double* vmax, int preserve, double zero_tol, double ft,
                               double relax, double* fl, double* fu, uint8_t* lch, uint8_t* uch, uint8_t* fixed,
-                              double* fval, int32_t* flags) {
+                              double* fval, int32_t* flags, BatchMarks bm) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   if (!keep_col[j]) {
@@ -674,6 +685,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   if (isfinite(cand_u)) eu = cmin(eu, cand_u);
   if (el > eu + ft) {
     wave_flag(flags + 0, 1);
+    bm.col(j, 1);
     return;
   }
   const bool was_free = !isfinite(ref_l[j]) && !isfinite(ref_u[j]);
@@ -687,6 +699,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   if (isfinite(cand_l) && cand_l > lj && isfinite(uj)) {
     if (cand_l >= uj + ft) {
       wave_flag(flags + 0, 1);
+      bm.col(j, 1);
       return;
     }
     if (cand_l >= uj || (uj - cand_l) * vmax[j] <= ft) {
@@ -707,6 +720,7 @@ __global__ void k_bs_finalize(int32_t n, const uint8_t* keep_col, const double* 
   if (!fx && isfinite(cand_u) && cand_u < uj && isfinite(new_l)) {
     if (cand_u <= new_l - ft) {
       wave_flag(flags + 0, 1);
+      bm.col(j, 1);
       return;
     }
     if (cand_u <= new_l || (cand_u - new_l) * vmax[j] <= ft) {
@@ -830,7 +844,7 @@ __global__ void k_bs_apply(int32_t n, const double* fl, const double* fu, const 
 
 // fixed-column row shift and removal of rows left without live entries
 __global__ void k_bs_fixed_rows(gp_csr A, const uint8_t* fixed, const double* fval, const uint8_t* keep_col,
-                                double ft, uint8_t* keep_row, double* AL, double* AU, int32_t* flags) {
+                                double ft, uint8_t* keep_row, double* AL, double* AU, int32_t* flags, BatchMarks bm) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= A.rows || !keep_row[i]) return;
   RowAcc acc;
@@ -847,8 +861,12 @@ __global__ void k_bs_fixed_rows(gp_csr A, const uint8_t* fixed, const double* fv
   AL[i] = al;
   AU[i] = au;
   if (live == 0) {
-    if (al <= ft && au >= -ft) keep_row[i] = 0;
-    else wave_flag(flags + 3, 1);
+    if (al <= ft && au >= -ft) {
+      keep_row[i] = 0;
+    } else {
+      wave_flag(flags + 3, 1);
+      bm.row(i, 1);
+    }
   }
 }
 // The same shift and removal as k_bs_fixed_rows through the LDS row fold:
@@ -866,6 +884,7 @@ struct FixedRowsFold {
   double* AL;
   double* AU;
   int32_t* flags;
+  BatchMarks bm;
   struct State {
     RowAcc acc;
     int32_t fixed;
@@ -893,8 +912,12 @@ struct FixedRowsFold {
       AU[i] = au;
     }
     if (row_nnz[i] - s.fixed == 0) {
-      if (al <= ft && au >= -ft) keep_row_out[i] = 0;
-      else wave_flag(flags + 3, 1);
+      if (al <= ft && au >= -ft) {
+        keep_row_out[i] = 0;
+      } else {
+        wave_flag(flags + 3, 1);
+        bm.row(i, 1);
+      }
     }
   }
 };
@@ -993,7 +1016,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
                        plan.new_u, cl.data(), cu.data(), ref_l, ref_u, plan.new_c, vmax_d,
                        p.bound_strengthening_preserve_free_zero_cost ? 1 : 0, p.zero_tol, p.feasibility_tol,
                        p.bound_strengthening_upper_relaxation_factor, fl.data(), fu.data(), lch.data(), uch.data(),
-                       fixed.data(), fval.data(), flags.d.data());
+                       fixed.data(), fval.data(), flags.d.data(), batch_marks());
     GP_LAUNCH_CHECK("bound finalize");
   }
   const int32_t* f = flags.read();
@@ -1041,11 +1064,11 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
       const PackedMask fixed_bits(fixed.data(), n);
       launch_row_fold(lp.A, fv.data(), fixed_bits.view(),
                       FixedRowsFold{plan.keep_row_mask, s.row_nnz, p.feasibility_tol, plan.keep_row_mask,
-                                    plan.new_AL, plan.new_AU, f3.d.data()});
+                                    plan.new_AL, plan.new_AU, f3.d.data(), batch_marks()});
     } else {
       hipLaunchKernelGGL(k_bs_fixed_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, lp.A, fixed.data(), fval.data(),
                          plan.keep_col_mask, p.feasibility_tol, plan.keep_row_mask, plan.new_AL, plan.new_AU,
-                         f3.d.data());
+                         f3.d.data(), batch_marks());
     }
     GP_LAUNCH_CHECK("bound fixed rows");
     if (f3.read()[3]) {

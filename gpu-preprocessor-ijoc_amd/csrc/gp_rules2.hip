@@ -214,7 +214,8 @@ __device__ bool sc_rest_bounds(const ScAct& act, double a, double lj, double uj,
 __global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const uint8_t* keep_row,
                              const uint8_t* keep_col, const uint8_t* smask, const int32_t* srow, const double* sval,
                              const double* c, const double* l, const double* u, const double* AL, const double* AU,
-                             double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags) {
+                             double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags,
+                             BatchMarks bm) {
   const int64_t j = gtid();
   if (j >= n || !keep_col[j] || !smask[j]) return;
   const int32_t row = srow[j];
@@ -231,6 +232,7 @@ __global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const u
       (cj < -zt && a < -zt && !isfinite(lhs) && !isfinite(uj)) ||
       (cj < -zt && a > zt && !isfinite(rhs) && !isfinite(uj))) {
     wave_flag(&flags[1], 1);
+    bm.col(j, 2);
     return;
   }
   if (!rest_ok) return;
@@ -510,7 +512,7 @@ struct DfSidesOp {
 __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint32_t* sides, const double* c,
                                 const double* l, const double* u, double zt, double bt, int32_t maxdeg,
                                 const int32_t* live_col_nnz, uint8_t* fixed, uint8_t* inf, uint8_t* ffix,
-                                double* val, double* contrib, int32_t* flags) {
+                                double* val, double* contrib, int32_t* flags, BatchMarks bm) {
   const int64_t j = gtid();
   if (j >= AT.rows) return;
   fixed[j] = 0;
@@ -522,6 +524,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint32
   const double lj = l[j], uj = u[j];
   if (lj > uj + bt) {
     wave_flag(&flags[0], 1);
+    bm.col(j, 1);
     return;
   }
   const gp_offset a0 = AT.row_ptr[j], a1 = AT.row_ptr[j + 1];
@@ -535,6 +538,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint32
     if (!down) {
       if (!isfinite(lj)) {
         wave_flag(&flags[0], 2);
+        bm.col(j, 2);
         return;
       }
       v = lj;
@@ -544,6 +548,7 @@ __global__ void k_df_candidates(gp_csr AT, const uint8_t* keep_col, const uint32
     if (!up) {
       if (!isfinite(uj)) {
         wave_flag(&flags[0], 2);
+        bm.col(j, 2);
         return;
       }
       v = uj;
@@ -917,7 +922,7 @@ struct PrRecs {
 
 __global__ void k_pr_groups(int32_t m, const uint64_t* hash, const int32_t* ids, gp_csr A, const uint8_t* keep_row,
                             const uint8_t* keep_col, double* AL, double* AU, double ctol, double itol,
-                            uint8_t* row_delete, PrRecs rec, int32_t* flags) {
+                            uint8_t* row_delete, PrRecs rec, int32_t* flags, BatchMarks bm) {
   const int64_t gs = gtid();
   if (gs >= m) return;
   if (gs > 0 && hash[gs - 1] == hash[gs]) return;
@@ -964,6 +969,7 @@ __global__ void k_pr_groups(int32_t m, const uint64_t* hash, const int32_t* ids,
     scaled_interval(AL[row], AU[row], r, &sl, &su);
     if (cmax(rl, sl) > cmin(ru, su) + itol) {
       wave_flag(&flags[0], 1);
+      bm.row(row, 1);
       return;
     }
     ++step;
@@ -1116,7 +1122,7 @@ struct PcOut {
 
 __global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids, gp_csr AT, const uint8_t* keep_col,
                             const uint8_t* keep_row, const double* c, double* l, double* u, double zt, double ctol,
-                            double otol, double active_max, PcOut o, int32_t* flags) {
+                            double otol, double active_max, PcOut o, int32_t* flags, BatchMarks bm) {
   const int64_t gs = gtid();
   if (gs >= n) return;
   if (gs > 0 && hash[gs - 1] == hash[gs]) return;
@@ -1177,6 +1183,7 @@ __global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids,
         const double v = xk_lo ? sl : su;
         if (!isfinite(v)) {
           wave_flag(&flags[0], 1);
+          bm.col(k, 2);
           return;
         }
         o.fixed[k] = 1;
@@ -1188,6 +1195,7 @@ __global__ void k_pc_groups(int32_t n, const uint64_t* hash, const int32_t* ids,
         const double v = xj_lo ? tl : tu;
         if (!isfinite(v)) {
           wave_flag(&flags[0], 1);
+          bm.col(j, 2);
           return;
         }
         o.fixed[j] = 1;
@@ -1396,7 +1404,7 @@ struct DbPairs {
 };
 
 __global__ void k_db_apply(DbPairs P, DbCand cand, double* l, double* u, double* c, const double* AU,
-                           uint8_t* keep_row, uint8_t* keep_col, double bt, int32_t* flags) {
+                           uint8_t* keep_row, uint8_t* keep_col, double bt, int32_t* flags, BatchMarks bm) {
   const int64_t t = gtid();
   if (t >= P.k) return;
   const int32_t row = P.rows[t], kc = cand.keep[row], ec = cand.elim[row];
@@ -1433,6 +1441,7 @@ __global__ void k_db_apply(DbPairs P, DbCand cand, double* l, double* u, double*
   P.obj_f[t] = 0.0;
   if (kl > ku + bt) {
     wave_flag(&flags[1], 1);
+    bm.row(row, 1);
     return;
   }
   if (isfinite(kl) && isfinite(ku) && ku <= kl + bt) {
@@ -1754,7 +1763,7 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act, plan.keep_row_mask,
                      plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row,
                      s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
-                     p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data());
+                     p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("singleton columns propose");
   const int32_t* pf = flags.read();
   if (pf[1]) {
@@ -1853,7 +1862,8 @@ void rule_dual_fix(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, con
     hipLaunchKernelGGL(k_df_candidates, dim3(grid_for(n)), dim3(kBlock), 0, 0, lp.AT, plan.keep_col_mask,
                        static_cast<const uint32_t*>(sides.data()), plan.new_c, plan.new_l, plan.new_u, p.zero_tol,
                        p.bound_tol, p.dual_fix_max_nonzero_cost_col_degree, s.live_masks ? s.col_nnz : nullptr,
-                       fixed.data(), inf.data(), ffix.data(), val.data(), contrib.data(), flags.d.data());
+                       fixed.data(), inf.data(), ffix.data(), val.data(), contrib.data(), flags.d.data(),
+                       batch_marks());
   }
   GP_LAUNCH_CHECK("dual fix candidates");
   const int32_t* f = flags.read();
@@ -1958,7 +1968,7 @@ void rule_parallel_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s
   PrRecs rec{step.data(), kept.data(), rvals.data()};
   hipLaunchKernelGGL(k_pr_groups, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, hash.data(), ids.data(), lp.A,
                      plan.keep_row_mask, keep_col, plan.new_AL, plan.new_AU, p.zero_tol, p.feasibility_tol,
-                     row_delete.data(), rec, flags.d.data());
+                     row_delete.data(), rec, flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("parallel rows groups");
   if (flags.read()[0]) {
     plan.has_infeasible = 1;
@@ -2028,7 +2038,7 @@ void rule_parallel_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stats
   PcOut o{col_delete.data(), fixed.data(), fval.data(), merge_to.data(), mrec.data()};
   hipLaunchKernelGGL(k_pc_groups, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, hash.data(), ids.data(), lp.AT,
                      plan.keep_col_mask, plan.keep_row_mask, plan.new_c, plan.new_l, plan.new_u, zt, ctol, otol,
-                     active_max, o, flags.d.data());
+                     active_max, o, flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("parallel columns groups");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -2176,7 +2186,7 @@ struct DbsMeta {
 // the target row's substitution (_kernel_doubleton_equations_apply, :365-461),
 // one lane: mapped bounds of the kept column, then the column and row updates
 __global__ void k_dbs_apply(DbsArgs g, const int32_t* target, double* l, double* u, double* c, uint8_t* keep_row,
-                            uint8_t* keep_col, DbsMeta* meta) {
+                            uint8_t* keep_col, DbsMeta* meta, BatchMarks bm) {
   DbsMeta r{};
   const int32_t row = *target;
   if (row >= g.A.rows || !dbs_candidate(g, row, r.ec, r.kc, r.ev, r.kv)) {
@@ -2214,6 +2224,7 @@ __global__ void k_dbs_apply(DbsArgs g, const int32_t* target, double* l, double*
   r.at1 = g.AT.row_ptr[r.ec + 1];
   if (r.kl > r.ku + g.bt) {
     r.status = 1;
+    bm.row(row, 1);
     *meta = r;
     return;
   }
@@ -2323,7 +2334,7 @@ void rule_doubleton_single_row(gp_plan& plan, const gp_device_lp& lp, const gp_p
                   zt, bt, fmax(1.0e-9, bt), p.doubleton_equations_max_fill_in_proxy};
   hipLaunchKernelGGL(k_dbs_target, dim3(grid_for(m)), dim3(kBlock), 0, 0, g, target.data());
   hipLaunchKernelGGL(k_dbs_apply, dim3(1), dim3(1), 0, 0, g, static_cast<const int32_t*>(target.data()), plan.new_l,
-                     plan.new_u, plan.new_c, plan.keep_row_mask, plan.keep_col_mask, meta.data());
+                     plan.new_u, plan.new_c, plan.keep_row_mask, plan.keep_col_mask, meta.data(), batch_marks());
   GP_LAUNCH_CHECK("doubleton single row");
   DbsMeta h;
   d2h(&h, meta.data(), 1);
@@ -2428,7 +2439,7 @@ void rule_doubleton_equations(gp_plan& plan, const gp_device_lp& lp, const gp_st
   GP_LAUNCH_CHECK("doubleton rows");
   DbPairs P{k, rows.data(), rec.data(), subst.data(), fixedp.data(), obj_s.data(), obj_f.data()};
   hipLaunchKernelGGL(k_db_apply, dim3(grid_for(k)), dim3(kBlock), 0, 0, P, cand, plan.new_l, plan.new_u, plan.new_c,
-                     plan.new_AU, plan.keep_row_mask, plan.keep_col_mask, bt, flags.d.data());
+                     plan.new_AU, plan.keep_row_mask, plan.keep_col_mask, bt, flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("doubleton apply");
   const int32_t* f = flags.read();
   if (f[1]) {

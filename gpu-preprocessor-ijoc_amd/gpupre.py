@@ -209,7 +209,8 @@ class GpBatchSummary(C.Structure):
                 ("row_offsets", C.POINTER(C.c_int32)), ("col_offsets", C.POINTER(C.c_int32)),
                 ("reduced_row_offsets", C.POINTER(C.c_int32)), ("reduced_col_offsets", C.POINTER(C.c_int32)),
                 ("reduced_nnz_offsets", C.POINTER(OFFSET)), ("obj_constant", C.POINTER(C.c_double)),
-                ("assemble_seconds", C.c_double), ("presolve_seconds", C.c_double), ("total_seconds", C.c_double)]
+                ("assemble_seconds", C.c_double), ("presolve_seconds", C.c_double), ("total_seconds", C.c_double),
+                ("status", C.POINTER(C.c_int32)), ("restarts", C.c_int32)]
 
 
 class GpTuning(C.Structure):
@@ -932,6 +933,14 @@ class BatchSummary:
 
     def obj_constants(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.s.obj_constant, shape=(self.s.count,)).copy()
+
+    def status(self) -> np.ndarray:
+        """Per instance: 0 presolved in the batch, 1 infeasible, 2 unbounded (retired alone)."""
+        return np.ctypeslib.as_array(self.s.status, shape=(self.s.count,)).copy()
+
+    @property
+    def restarts(self) -> int:
+        return self.s.restarts
 
     def whole(self) -> "Summary":
         """The direct-sum summary as a Summary view (owned by this batch)."""

@@ -373,6 +373,8 @@ typedef struct gp_batch_summary {
   double assemble_seconds;
   double presolve_seconds;
   double total_seconds;
+  int32_t* status;             /* host [count]: 0 presolved in the batch, 1 infeasible, 2 unbounded */
+  int32_t restarts;            /* times the batch started over without retired instances */
 } gp_batch_summary;
 
 /* Host view of a recovery record. */
@@ -531,7 +533,10 @@ int gp_autotune(const gp_device_lp* lp, const gp_params* params, int32_t reps, g
 
 /* ---- batched presolve (no reference counterpart) ------------------------- */
 
-/* Presolve `count` device LPs (caller-owned, unmodified) as one batch. */
+/* Presolve `count` device LPs (caller-owned, unmodified) as one batch.  An
+ * instance that a rule finds infeasible or unbounded retires alone: the batch
+ * starts over without it, it becomes an empty block of the direct sum (the
+ * offsets describe the direct sum actually presolved) and status[k] says why. */
 int gp_preprocess_batch(const gp_device_lp* instances, int32_t count, const gp_params* params,
                         gp_batch_summary* out);
 /* Postsolve of the whole batch: x, y, z are device arrays over the reduced
