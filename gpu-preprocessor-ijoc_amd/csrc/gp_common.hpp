@@ -493,7 +493,8 @@ void launch_row_fold(const gp_csr& A, const double2* lu, Bits keep_col, const P&
 // (l[j], u[j]) interleaved, so a sweep gathers both bounds with one load.
 struct PackedBounds {
   DVec<double2> lu;
-  PackedBounds(const double* l, const double* u, int32_t n);  // gp_primitives.hip
+  // also copies l and u into l_copy / u_copy when given (gp_primitives.hip)
+  PackedBounds(const double* l, const double* u, int32_t n, double* l_copy = nullptr, double* u_copy = nullptr);
   const double2* data() const { return lu.data(); }
 };
 
@@ -532,6 +533,8 @@ StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, double
                      DVec<uint32_t>& changed);  // gp_primitives.hip
 // After a writer fold: the current pairs become the snap.
 void store_commit(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol);
+// The same, taking over lu's buffer as the snap instead of copying it (lu is left empty).
+void store_commit(EntryBounds* eb, const gp_csr& A, PackedBounds& lu, double zero_tol);
 
 // Stable ascending sort of (key, value) pairs on bits [0, end_bit) of the key,
 // in place (gp_sort.hip).
@@ -699,10 +702,12 @@ void live_stats_all_live(gp_stats& s, const gp_device_lp& lp, int64_t* device_to
 // Bring live statistics from the masks (old_rows, old_cols) they were last
 // computed for to the plan's current masks, visiting only deleted lines.
 // `scratch` holds 4 int32.  Returns false (stats then need a full refresh)
-// when a mask went from 0 to 1 or a count went negative.
-bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, const uint8_t* old_rows,
-                       const uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
-                       int32_t* scratch);
+// when a mask went from 0 to 1 or a count went negative.  With `sync`, the
+// touched bytes arrive all zero and are left all zero, and old_rows / old_cols
+// are the caller's snapshot, left equal to the plan's masks (no copy, no fill).
+bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, uint8_t* old_rows,
+                       uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
+                       int32_t* scratch, bool sync = false);
 void row_activity_summary(const gp_csr& A, const double* l, const double* u, double zero_tol,
                           const uint8_t* keep_row, const uint8_t* keep_col, double* min_fin,
                           double* max_fin, int32_t* min_ninf, int32_t* max_pinf);

@@ -238,19 +238,18 @@ struct Deferred {
   gp_plan plan{};
   gp_stats stats{};
   DVec<uint8_t> entry_cols;
-  DVec<double> reference_bounds;
   DVec<int64_t> live_total;
   // masks the statistics were last computed for, and incremental-update scratch
   DVec<uint8_t> snap_rows, snap_cols, touched_rows, touched_cols;
   DVec<int32_t> inc_scratch;
   bool active = false, pending = false, stats_dirty = false, nnz_dirty = false, have_snapshot = false;
-  bool fresh = false;  // plan masks still all ones
+  bool fresh = false;          // plan masks still all ones
+  bool entry_current = false;  // entry_cols equals the plan's column mask
   void reset() {
     if (active) {
       plan_destroy(plan);
       stats_destroy(stats);
       entry_cols.reset();
-      reference_bounds.reset();
       live_total.reset();
       snap_rows.reset();
       snap_cols.reset();
@@ -258,7 +257,7 @@ struct Deferred {
       touched_cols.reset();
       inc_scratch.reset();
     }
-    active = pending = stats_dirty = nnz_dirty = have_snapshot = fresh = false;
+    active = pending = stats_dirty = nnz_dirty = have_snapshot = fresh = entry_current = false;
   }
 };
 
@@ -286,7 +285,6 @@ struct Scheduler {
     stats_create(cur.lp.A.rows, cur.lp.A.cols, d.stats);
     const int32_t n = cur.lp.A.cols;
     d.entry_cols.resize(n ? n : 1);
-    d.reference_bounds.resize(2 * static_cast<size_t>(n) + 1);
     d.live_total.resize(1);
     d.plan.rule_entry_keep_col = d.entry_cols.data();
     d.active = d.stats_dirty = d.nnz_dirty = d.fresh = true;
@@ -297,11 +295,13 @@ struct Scheduler {
   void refresh_stats() {
     if (!d.stats_dirty) return;
     const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
-    bool done = false;
-    if (d.have_snapshot && p->incremental_live_stats)
-      done = update_live_stats(d.stats, cur.lp, d.plan, d.snap_rows.data(), d.snap_cols.data(),
-                               d.touched_rows.data(), d.touched_cols.data(), d.live_total.data(),
-                               d.inc_scratch.data());
+    bool done = false, synced = false;
+    if (d.have_snapshot && p->incremental_live_stats) {
+      // an accepted update leaves the snapshot equal to the masks
+      done = synced = update_live_stats(d.stats, cur.lp, d.plan, d.snap_rows.data(), d.snap_cols.data(),
+                                        d.touched_rows.data(), d.touched_cols.data(), d.live_total.data(),
+                                        d.inc_scratch.data(), true);
+    }
     if (!done && d.fresh) {
       live_stats_all_live(d.stats, cur.lp, d.live_total.data());
       done = true;
@@ -313,12 +313,16 @@ struct Scheduler {
         d.snap_cols.resize(n ? n : 1);
         d.touched_rows.resize(m ? m : 1);
         d.touched_cols.resize(n ? n : 1);
+        fill_u8(d.touched_rows.data(), 0, m ? m : 1);  // updates keep them zero from here on
+        fill_u8(d.touched_cols.data(), 0, n ? n : 1);
         d.inc_scratch.resize(4);
         d.have_snapshot = true;
       }
     }
-    d2d(d.snap_rows.data(), d.plan.keep_row_mask, static_cast<size_t>(m));
-    d2d(d.snap_cols.data(), d.plan.keep_col_mask, static_cast<size_t>(n));
+    if (!synced) {
+      d2d(d.snap_rows.data(), d.plan.keep_row_mask, static_cast<size_t>(m));
+      d2d(d.snap_cols.data(), d.plan.keep_col_mask, static_cast<size_t>(n));
+    }
     d.stats_dirty = false;
     d.fresh = false;
   }
@@ -340,7 +344,12 @@ struct Scheduler {
     refresh_stats();
     gp_plan& plan = d.plan;
     const int32_t n = cur.lp.A.cols;
-    d2d(d.entry_cols.data(), plan.keep_col_mask, n);
+    // the column mask at the rule's entry; only a rule with a column action
+    // deletes columns, so the copy is redone after such a rule only
+    if (!d.entry_current) {
+      d2d(d.entry_cols.data(), plan.keep_col_mask, n);
+      d.entry_current = true;
+    }
     plan.has_change = plan.has_row_action = plan.has_col_action = 0;
     plan.has_redundant_row_deletion = plan.has_row_deletion = 0;
     plan.obj_constant_delta = 0.0;
@@ -348,6 +357,7 @@ struct Scheduler {
     out->has_infeasible = plan.has_infeasible;
     out->has_unbounded = plan.has_unbounded;
     terminal = plan.has_infeasible || plan.has_unbounded;
+    if (plan.has_col_action || terminal) d.entry_current = false;
     const bool changed = plan.has_change && !terminal;
     if (changed) {
       if (plan.has_row_deletion || plan.has_col_action) ++deletion_serial;
@@ -548,12 +558,13 @@ void run_presolve(const gp_device_lp& lp, const gp_params& p, gp_summary& out) {
         bool bounds_changed = false;
         if (on(p.enable_bound_strengthening, GP_RULE_BOUND_STRENGTHENING) && !s.terminal && !s.time_exceeded()) {
           s.prepare_deferred();
-          const int32_t n = s.cur.lp.A.cols;
-          if (n) {
-            d2d(s.d.reference_bounds.data(), s.d.plan.new_l, n);
-            d2d(s.d.reference_bounds.data() + n, s.d.plan.new_u, n);
-            s.d.plan.propagation_reference_l = s.d.reference_bounds.data();
-            s.d.plan.propagation_reference_u = s.d.reference_bounds.data() + n;
+          // The reference bounds are the plan's bounds as this call starts
+          // (the propagation phase, gpu_presolve.cu:2456-2470, copies them).  The
+          // rule reads them before it writes any bound, so here it reads the
+          // plan's own arrays.
+          if (s.cur.lp.A.cols) {
+            s.d.plan.propagation_reference_l = s.d.plan.new_l;
+            s.d.plan.propagation_reference_u = s.d.plan.new_u;
           }
           bounds_changed = s.deferred_rule(true, GP_RULE_BOUND_STRENGTHENING);
         }
@@ -917,8 +928,10 @@ int gp_update_live_stats(gp_stats* stats, const gp_device_lp* lp, const gp_plan*
     DVec<int32_t> scratch(4);
     const int64_t before = stats->live_nnz;
     h2d(total.data(), &before, 1);
-    const bool ok = update_live_stats(*stats, *lp, *plan, old_keep_row, old_keep_col, touched_rows.data(),
-                                      touched_cols.data(), total.data(), scratch.data());
+    // the caller's masks are read only (no sync)
+    const bool ok = update_live_stats(*stats, *lp, *plan, const_cast<uint8_t*>(old_keep_row),
+                                      const_cast<uint8_t*>(old_keep_col), touched_rows.data(), touched_cols.data(),
+                                      total.data(), scratch.data());
     *accepted = ok ? 1 : 0;
     if (ok) stats->live_nnz = read1(total.data());
   });

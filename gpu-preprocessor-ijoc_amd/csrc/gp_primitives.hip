@@ -1514,16 +1514,32 @@ __device__ inline void finalize_line(const gp_csr& M, int64_t r, bool active, co
   coefficient[r] = v;
 }
 
-__global__ void k_inc_finalize(gp_csr A, gp_csr AT, const uint8_t* old_rows, const uint8_t* old_cols,
-                               const uint8_t* keep_rows, const uint8_t* keep_cols, const uint8_t* touched_rows,
-                               const uint8_t* touched_cols, gp_stats s, int32_t* invalid) {
+// With `sync`, line k's snapshot byte takes its current mask and its touched
+// byte is cleared: the kernel is the last to read either at index k.
+__global__ void k_inc_finalize(gp_csr A, gp_csr AT, uint8_t* old_rows, uint8_t* old_cols, const uint8_t* keep_rows,
+                               const uint8_t* keep_cols, uint8_t* touched_rows, uint8_t* touched_cols, gp_stats s,
+                               int32_t* invalid, int sync) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k < A.rows && ((old_rows[k] && !keep_rows[k]) || touched_rows[k]))
-    finalize_line(A, k, keep_rows[k] != 0, keep_cols, s.row_nnz, s.empty_row_mask, s.singleton_row_mask,
-                  s.singleton_row_col, s.singleton_row_val, invalid);
-  if (k < AT.rows && ((old_cols[k] && !keep_cols[k]) || touched_cols[k]))
-    finalize_line(AT, k, keep_cols[k] != 0, keep_rows, s.col_nnz, s.empty_col_mask, s.singleton_column_mask,
-                  s.singleton_column_row, s.singleton_column_val, invalid);
+  if (k < A.rows) {
+    const uint8_t was = old_rows[k], now = keep_rows[k], t = touched_rows[k];
+    if ((was && !now) || t)
+      finalize_line(A, k, now != 0, keep_cols, s.row_nnz, s.empty_row_mask, s.singleton_row_mask,
+                    s.singleton_row_col, s.singleton_row_val, invalid);
+    if (sync) {
+      if (was != now) old_rows[k] = now;
+      if (t) touched_rows[k] = 0;
+    }
+  }
+  if (k < AT.rows) {
+    const uint8_t was = old_cols[k], now = keep_cols[k], t = touched_cols[k];
+    if ((was && !now) || t)
+      finalize_line(AT, k, now != 0, keep_rows, s.col_nnz, s.empty_col_mask, s.singleton_column_mask,
+                    s.singleton_column_row, s.singleton_column_val, invalid);
+    if (sync) {
+      if (was != now) old_cols[k] = now;
+      if (t) touched_cols[k] = 0;
+    }
+  }
 }
 
 // a rejected update leaves the live total as it was
@@ -1537,14 +1553,16 @@ __global__ void k_inc_total(int64_t* total, const unsigned long long* removed, i
 }
 }  // namespace
 
-bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, const uint8_t* old_rows,
-                       const uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
-                       int32_t* scratch) {
+bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan, uint8_t* old_rows,
+                       uint8_t* old_cols, uint8_t* touched_rows, uint8_t* touched_cols, int64_t* device_total,
+                       int32_t* scratch, bool sync) {
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (!plan.keep_row_mask || !plan.keep_col_mask || !device_total) return false;
   TimedScope ts(GP_KT_LIVE_STATS_INC, 2.0 * (m + n) + 2.0 * (m + n) + 4.0 * (m + n));
-  if (m) GP_CHECK(hipMemsetAsync(touched_rows, 0, static_cast<size_t>(m), 0));
-  if (n) GP_CHECK(hipMemsetAsync(touched_cols, 0, static_cast<size_t>(n), 0));
+  if (!sync) {
+    if (m) GP_CHECK(hipMemsetAsync(touched_rows, 0, static_cast<size_t>(m), 0));
+    if (n) GP_CHECK(hipMemsetAsync(touched_cols, 0, static_cast<size_t>(n), 0));
+  }
   GP_CHECK(hipMemsetAsync(scratch, 0, 4 * sizeof(int32_t), 0));
   unsigned long long* removed = reinterpret_cast<unsigned long long*>(scratch);
   int32_t* invalid = scratch + 2;
@@ -1557,7 +1575,7 @@ bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
   const int64_t most = m > n ? m : n;
   if (most)
     hipLaunchKernelGGL(k_inc_finalize, dim3(grid_for(most)), dim3(kBlock), 0, 0, lp.A, lp.AT, old_rows, old_cols,
-                       plan.keep_row_mask, plan.keep_col_mask, touched_rows, touched_cols, s, invalid);
+                       plan.keep_row_mask, plan.keep_col_mask, touched_rows, touched_cols, s, invalid, sync ? 1 : 0);
   hipLaunchKernelGGL(k_inc_total, dim3(1), dim3(1), 0, 0, device_total, removed, invalid);
   GP_LAUNCH_CHECK("incremental live stats");
   return read1(invalid) == 0;
@@ -1565,15 +1583,23 @@ bool update_live_stats(gp_stats& s, const gp_device_lp& lp, const gp_plan& plan,
 
 // ---- interleaved bounds ----------------------------------------------------------------
 namespace {
-__global__ void k_pack_bounds(const double* l, const double* u, int32_t n, double2* lu) {
+__global__ void k_pack_bounds(const double* l, const double* u, int32_t n, double2* lu, double* l_copy,
+                              double* u_copy) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j < n) lu[j] = make_double2(l[j], u[j]);
+  if (j >= n) return;
+  const double lj = l[j], uj = u[j];
+  lu[j] = make_double2(lj, uj);
+  if (l_copy) {
+    l_copy[j] = lj;
+    u_copy[j] = uj;
+  }
 }
 }  // namespace
 
-PackedBounds::PackedBounds(const double* l, const double* u, int32_t n) : lu(static_cast<size_t>(n > 0 ? n : 1)) {
+PackedBounds::PackedBounds(const double* l, const double* u, int32_t n, double* l_copy, double* u_copy)
+    : lu(static_cast<size_t>(n > 0 ? n : 1)) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu.data());
+  hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu.data(), l_copy, u_copy);
   GP_LAUNCH_CHECK("pack bounds");
 }
 
@@ -1625,6 +1651,17 @@ void store_commit(EntryBounds* eb, const gp_csr& A, const double2* lu, double ze
     eb->snap.resize(static_cast<size_t>(A.cols));
     d2d(eb->snap.data(), lu, static_cast<size_t>(A.cols));
   }
+  eb->col_idx = A.col_idx;
+  eb->nnz = static_cast<int64_t>(A.nnz);
+  eb->n = A.cols;
+  eb->zero_tol = zero_tol;
+  eb->valid = true;
+}
+
+void store_commit(EntryBounds* eb, const gp_csr& A, PackedBounds& lu, double zero_tol) {
+  if (!eb || A.nnz <= 0 || A.cols <= 0) return;
+  std::swap(eb->snap, lu.lu);
+  lu.lu.reset();
   eb->col_idx = A.col_idx;
   eb->nnz = static_cast<int64_t>(A.nnz);
   eb->n = A.cols;
@@ -1761,7 +1798,7 @@ extern "C" int gp_row_activity_summary_noalloc(const gp_csr* A, const double* l,
     uint32_t* words = reinterpret_cast<uint32_t*>(base + kScratchBoundsBytes);
     const int32_t n = A->cols;
     if (n > 0) {
-      hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu);
+      hipLaunchKernelGGL(k_pack_bounds, dim3(grid_for(n)), dim3(kBlock), 0, 0, l, u, n, lu, nullptr, nullptr);
       if (keep_col)
         hipLaunchKernelGGL(k_pack_bits, dim3(grid_for(((static_cast<int64_t>(n) + 31) / 32) * 32)), dim3(kBlock), 0, 0,
                            keep_col, static_cast<int64_t>(n), words);

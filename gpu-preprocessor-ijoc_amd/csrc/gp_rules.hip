@@ -233,40 +233,61 @@ void rule_empty_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& s, c
 
 // ---------------------------------------------------------------- singleton rows
 namespace {
-__global__ void k_singleton_rows(int32_t m, int32_t n, const uint8_t* keep_row, const uint8_t* singleton,
-                                 const int32_t* scol, const double* sval, const double* AL, const double* AU,
-                                 double zero_tol, double* cl, double* cu, uint8_t* remove) {
+// Candidate bounds exist only for the columns of live singleton rows: the
+// seed pass marks each such column and copies its current bounds, the next
+// pass tightens them.  Every other column keeps its bounds, as the full
+// copy-and-tighten of the restatement leaves them.
+__global__ void k_singleton_seed(int32_t m, int32_t n, const uint8_t* keep_row, const uint8_t* singleton,
+                                 const int32_t* scol, const double* sval, double zero_tol, const double* l,
+                                 const double* u, double* cl, double* cu, uint8_t* touched, uint8_t* remove) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  remove[i] = 0;
-  if (!(keep_row[i] && singleton[i])) return;
+  uint8_t r = 0;
+  if (keep_row[i] && singleton[i]) {
+    const int32_t j = scol[i];
+    if (j >= 0 && j < n && fabs(sval[i]) > zero_tol) {
+      // rows sharing a column store the same values
+      touched[j] = 1;
+      cl[j] = l[j];
+      cu[j] = u[j];
+      r = 1;
+    }
+  }
+  remove[i] = r;
+}
+__global__ void k_singleton_rows(int32_t m, const int32_t* scol, const double* sval, const double* AL,
+                                 const double* AU, const uint8_t* remove, double* cl, double* cu) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m || !remove[i]) return;
   const int32_t j = scol[i];
   const double a = sval[i];
-  if (j < 0 || j >= n || !(fabs(a) > zero_tol)) return;
   const double lo = AL[i], hi = AU[i];
   const double il = a > 0.0 ? lo / a : hi / a;
   const double iu = a > 0.0 ? hi / a : lo / a;
   atomic_total_max(cl + j, il);
   atomic_total_min(cu + j, iu);
-  remove[i] = 1;
 }
 __global__ void k_singleton_check(int32_t m, int32_t n, const double* cl, const double* cu, const double* l,
-                                  const double* u, const uint8_t* remove, double tol, int32_t* flags, BatchMarks bm) {
+                                  const double* u, const uint8_t* touched, const uint8_t* remove, double tol,
+                                  int32_t* flags, BatchMarks bm) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < n) {
-    if (cl[j] > cu[j] + tol) {
+    const double lj = l[j], uj = u[j];
+    const bool t = touched[j] != 0;
+    const double lo = t ? cl[j] : lj, hi = t ? cu[j] : uj;
+    if (lo > hi + tol) {
       wave_flag(flags + 0, 1);
       bm.col(j, 1);
     }
-    if (cl[j] != l[j] || cu[j] != u[j]) wave_flag(flags + 2, 1);
+    if (lo != lj || hi != uj) wave_flag(flags + 2, 1);
   }
   if (j < m && remove[j]) wave_flag(flags + 1, 1);
 }
-__global__ void k_singleton_commit(int32_t m, int32_t n, const double* cl, const double* cu, const uint8_t* remove,
-                                   uint8_t* keep_row, double* l, double* u) {
+__global__ void k_singleton_commit(int32_t m, int32_t n, const double* cl, const double* cu, const uint8_t* touched,
+                                   const uint8_t* remove, uint8_t* keep_row, double* l, double* u) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j < m && remove[j]) keep_row[j] = 0;
-  if (j < n) {
+  if (j < n && touched[j]) {
     l[j] = cl[j];
     u[j] = cu[j];
   }
@@ -318,20 +339,22 @@ void rule_singleton_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0) return;
-  // singleton mask, column and value, keep_row, AL, AU per row; l, u per column
-  TimedScope ts(GP_KT_CLEANUP, 30.0 * m + 16.0 * n);
+  // singleton mask, column and value, keep_row, AL, AU per row; l, u and a mark per column
+  TimedScope ts(GP_KT_CLEANUP, 30.0 * m + 17.0 * n);
   DVec<double> cl(n ? n : 1), cu(n ? n : 1);
-  DVec<uint8_t> remove(m);
-  d2d(cl.data(), plan.new_l, n);
-  d2d(cu.data(), plan.new_u, n);
-  hipLaunchKernelGGL(k_singleton_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, n, plan.keep_row_mask,
-                     s.singleton_row_mask, s.singleton_row_col, s.singleton_row_val, plan.new_AL, plan.new_AU,
-                     p.zero_tol, cl.data(), cu.data(), remove.data());
+  DVec<uint8_t> remove(m), touched(n ? n : 1);
+  fill_u8(touched.data(), 0, n);
+  hipLaunchKernelGGL(k_singleton_seed, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, n, plan.keep_row_mask,
+                     s.singleton_row_mask, s.singleton_row_col, s.singleton_row_val, p.zero_tol, plan.new_l,
+                     plan.new_u, cl.data(), cu.data(), touched.data(), remove.data());
+  hipLaunchKernelGGL(k_singleton_rows, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, s.singleton_row_col,
+                     s.singleton_row_val, plan.new_AL, plan.new_AU, remove.data(), cl.data(), cu.data());
   GP_LAUNCH_CHECK("singleton rows");
   Flags flags;
   const int32_t mn = m > n ? m : n;
   hipLaunchKernelGGL(k_singleton_check, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, cl.data(), cu.data(),
-                     plan.new_l, plan.new_u, remove.data(), p.bound_tol, flags.d.data(), batch_marks());
+                     plan.new_l, plan.new_u, touched.data(), remove.data(), p.bound_tol, flags.d.data(),
+                     batch_marks());
   GP_LAUNCH_CHECK("singleton rows check");
   const int32_t* f = flags.read();
   if (f[0]) {
@@ -341,7 +364,7 @@ void rule_singleton_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
       append_deleted_rows(plan_tape(plan), m, remove.data(), s.singleton_row_col, plan.new_AL, plan.new_AU,
                           s.singleton_row_val, true);
     hipLaunchKernelGGL(k_singleton_commit, dim3(grid_for(mn)), dim3(kBlock), 0, 0, m, n, cl.data(), cu.data(),
-                       remove.data(), plan.keep_row_mask, plan.new_l, plan.new_u);
+                       touched.data(), remove.data(), plan.keep_row_mask, plan.new_l, plan.new_u);
     GP_LAUNCH_CHECK("singleton rows commit");
     plan.has_row_action = 1;
     plan.has_row_deletion = plan.has_row_deletion || f[1] != 0;
@@ -932,9 +955,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   DVec<int32_t> min_ninf(m), max_pinf(m);
   DVec<double> cl(n), cu(n), fl(n), fu(n), fval(n);
   DVec<uint8_t> lch(n), uch(n), fixed(n);
-  d2d(cl.data(), plan.new_l, n);
-  d2d(cu.data(), plan.new_u, n);
-  const PackedBounds lu(plan.new_l, plan.new_u, n);
+  // candidate bounds start as the current ones
+  PackedBounds lu(plan.new_l, plan.new_u, n, cl.data(), cu.data());
   // the support-row search (tape only) reads each row's summary as one record
   DVec<BsRow> row_rec(tape_on(plan, p) && s.row_nnz ? static_cast<size_t>(m) : 0);
   // the sweep refreshes the entry-bounds store, and the candidates read each entry's pair from it
@@ -946,7 +968,7 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
                        plan.new_AU, s.row_nnz, view);
-  store_commit(store, lp.A, lu.data(), p.zero_tol);
+  store_commit(store, lp.A, lu, p.zero_tol);  // lu's buffer becomes the snap
   const double2* entry_lu = store->pairs.data();
   {
     const PackedMask live_cols(keep_col, n);
