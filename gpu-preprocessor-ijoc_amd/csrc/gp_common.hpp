@@ -547,7 +547,10 @@ struct RowVal {
   int32_t pad;
   double val;
 };
-void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit = 32);
+// The sorted pairs are left where the last pass put them: in keys / vals or
+// in k2 / v2 (which receive the scratch); *keys_out / *vals_out point there.
+void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit, DVec<uint32_t>& k2,
+                           DVec<RowVal>& v2, const uint32_t** keys_out, const RowVal** vals_out);
 void gather_compact(double* dst, const double* src, const int32_t* old_to_new, int32_t n_old);
 void gather_compact_i32(int32_t* dst, const int32_t* src, const int32_t* old_to_new, int32_t n_old);
 int32_t count_u8(const uint8_t* mask, int32_t n);

@@ -686,9 +686,13 @@ void transpose_csr(const gp_csr& A, gp_csr& AT) {
       default: hipLaunchKernelGGL(k_transpose_keys<64>, dim3(grid), dim3(kBlock), 0, 0, A.row_ptr, A.col_idx, A.vals, m, keys.data(), rv.data()); break;
     }
     GP_LAUNCH_CHECK("transpose keys");
-    sort_pairs_u32_rowval(keys.data(), rv.data(), nnz, bits_for(n));
-    hipLaunchKernelGGL(k_transpose_out, dim3(grid_for(nnz)), dim3(kBlock), 0, 0, keys.data(),
-                       static_cast<const RowVal*>(rv.data()), nnz, n, AT.col_idx, AT.vals, AT.row_ptr);
+    DVec<uint32_t> keys2;
+    DVec<RowVal> rv2;
+    const uint32_t* sorted_keys = nullptr;
+    const RowVal* sorted_rv = nullptr;
+    sort_pairs_u32_rowval(keys.data(), rv.data(), nnz, bits_for(n), keys2, rv2, &sorted_keys, &sorted_rv);
+    hipLaunchKernelGGL(k_transpose_out, dim3(grid_for(nnz)), dim3(kBlock), 0, 0, sorted_keys, sorted_rv, nnz, n,
+                       AT.col_idx, AT.vals, AT.row_ptr);
     GP_LAUNCH_CHECK("transpose columns");
     return;
   }

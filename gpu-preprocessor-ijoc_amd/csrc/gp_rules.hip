@@ -652,9 +652,13 @@ struct ColMaxCache {
   DVec<uint8_t> keep_row;
 };
 
-__global__ void k_cm_dirty(gp_csr A, const uint8_t* keep_row, const uint8_t* keep_row_old, uint8_t* dirty) {
+// marks the columns of rows whose mask changed and brings the cached mask up to date
+__global__ void k_cm_dirty(gp_csr A, const uint8_t* keep_row, uint8_t* keep_row_old, uint8_t* dirty) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= A.rows || keep_row[i] == keep_row_old[i]) return;
+  if (i >= A.rows) return;
+  const uint8_t now = keep_row[i];
+  if (now == keep_row_old[i]) return;
+  keep_row_old[i] = now;
   for (gp_offset p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) dirty[A.col_idx[p]] = 1;
 }
 
@@ -1000,8 +1004,9 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
       cm.vmax.resize(n);
       cm.keep_row.resize(m);
     }
-    bool full = !cm.valid;
+    bool full = !cm.valid, dirty_ran = false;
     if (!full && masked) {
+      dirty_ran = true;
       DVec<uint8_t> dirty(n);
       DVec<int32_t> list(n);
       fill_u8(dirty.data(), 0, n);
@@ -1029,7 +1034,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
         GP_LAUNCH_CHECK("col max abs");
       }
     }
-    if (masked) d2d(cm.keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
+    // k_cm_dirty brought the cached mask up to date when it ran
+    if (masked && !dirty_ran) d2d(cm.keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
     cm.valid = true;
     const double* vmax_d = cm.vmax.data();
     const double* ref_l = plan.propagation_reference_l ? plan.propagation_reference_l : (lp.l ? lp.l : plan.new_l);

@@ -143,26 +143,36 @@ struct ScCache {
   DVec<uint8_t> keep_row, keep_col;
 };
 
+// marks the rows of columns whose bounds or mask changed; the cached column
+// mask is brought up to date on the way
 __global__ void k_sc_dirty_cols(gp_csr AT, const double2* lu, const double2* lu_old, const uint8_t* keep_col,
-                                const uint8_t* keep_col_old, uint8_t* dirty) {
+                                uint8_t* keep_col_old, uint8_t* dirty) {
   const int64_t j = gtid();
   if (j >= AT.rows) return;
   const double2 a = lu[j], b = lu_old[j];
+  const uint8_t now = keep_col[j];
+  const bool mask_same = now == keep_col_old[j];
+  if (!mask_same) keep_col_old[j] = now;
   const bool same = __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
-                    __double_as_longlong(a.y) == __double_as_longlong(b.y) && keep_col[j] == keep_col_old[j];
+                    __double_as_longlong(a.y) == __double_as_longlong(b.y) && mask_same;
   if (same) return;
   for (gp_offset p = AT.row_ptr[j]; p < AT.row_ptr[j + 1]; ++p) dirty[AT.col_idx[p]] = 1;
 }
 
 // dirty rows appended to the list with one atomic per wavefront (any order:
 // listed rows are refreshed independently)
-__global__ void k_sc_dirty_rows(int32_t m, const uint8_t* keep_row, const uint8_t* keep_row_old, uint8_t* dirty,
+// (also brings the cached row mask up to date)
+__global__ void k_sc_dirty_rows(int32_t m, const uint8_t* keep_row, uint8_t* keep_row_old, uint8_t* dirty,
                                 int32_t* list, int32_t* count) {
   const int64_t i = gtid();
   bool d = false;
   if (i < m) {
-    if (keep_row[i] != keep_row_old[i]) dirty[i] = 1;
+    const uint8_t now = keep_row[i];
     d = dirty[i] != 0;
+    if (now != keep_row_old[i]) {
+      keep_row_old[i] = now;
+      d = true;
+    }
   }
   const unsigned long long b = __ballot(d);
   if (!b) return;
@@ -1719,8 +1729,9 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   {
     const PackedMask live_col(plan.keep_col_mask, n);
     PackedBounds lu(plan.new_l, plan.new_u, n);
-    bool full = !cache->valid;
+    bool full = !cache->valid, dirty_ran = false;
     if (!full) {
+      dirty_ran = true;
       // rows touched since the last call; few of them -> recompute those on one lane each
       DVec<uint8_t> dirty(m);
       DVec<int32_t> list(m);
@@ -1755,8 +1766,11 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
       store_commit(store, lp.A, lu.data(), p.zero_tol);
     }
     std::swap(cache->lu, lu.lu);
-    d2d(cache->keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
-    d2d(cache->keep_col.data(), plan.keep_col_mask, static_cast<size_t>(n));
+    if (!dirty_ran) {
+      // the dirty-row kernels bring the cached masks up to date when they run
+      d2d(cache->keep_row.data(), plan.keep_row_mask, static_cast<size_t>(m));
+      d2d(cache->keep_col.data(), plan.keep_col_mask, static_cast<size_t>(n));
+    }
     cache->valid = true;
   }
   const ScAct* row_act = cache->act.data();

@@ -151,8 +151,22 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const K* __restric
   }
 }
 
+// With `out`, the sorted data stays where the last pass put it: out->keys /
+// out->vals point there and out holds the scratch buffers (no copy back).
+template <class K, class V>
+struct SortOut {
+  DVec<K> k2;
+  DVec<V> v2;
+  const K* keys = nullptr;
+  const V* vals = nullptr;
+};
+
 template <class K, class V, bool kHasValues>
-void radix_sort(K* keys, V* vals, int64_t n, int end_bit) {
+void radix_sort(K* keys, V* vals, int64_t n, int end_bit, SortOut<K, V>* out = nullptr) {
+  if (out) {
+    out->keys = keys;
+    out->vals = vals;
+  }
   if (n <= 1 || end_bit <= 0) return;
   if (n > INT32_MAX) throw std::length_error("radix sort input exceeds int32 items");
   const int key_bits = static_cast<int>(8 * sizeof(K));
@@ -179,6 +193,13 @@ void radix_sort(K* keys, V* vals, int64_t n, int end_bit) {
     std::swap(kin, kout);
     if (kHasValues) std::swap(vin, vout);
   }
+  if (out) {
+    out->keys = kin;
+    out->vals = vin;
+    out->k2 = std::move(k2);
+    out->v2 = std::move(v2);
+    return;
+  }
   // an odd number of passes leaves the result in the scratch buffers
   if (kin != keys) d2d(keys, kin, static_cast<size_t>(n));
   if (kHasValues && vin != vals) d2d(vals, vin, static_cast<size_t>(n));
@@ -198,8 +219,14 @@ void sort_pairs_u64_f64(uint64_t* keys, double* vals, int64_t n, int end_bit) {
   radix_sort<uint64_t, double, true>(keys, vals, n, end_bit);
 }
 
-void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit) {
-  radix_sort<uint32_t, RowVal, true>(keys, vals, n, end_bit);
+void sort_pairs_u32_rowval(uint32_t* keys, RowVal* vals, int64_t n, int end_bit, DVec<uint32_t>& k2,
+                           DVec<RowVal>& v2, const uint32_t** keys_out, const RowVal** vals_out) {
+  SortOut<uint32_t, RowVal> out;
+  radix_sort<uint32_t, RowVal, true>(keys, vals, n, end_bit, &out);
+  k2 = std::move(out.k2);
+  v2 = std::move(out.v2);
+  *keys_out = out.keys;
+  *vals_out = out.vals;
 }
 
 }  // namespace gp
