@@ -361,12 +361,14 @@ int32_t build_old_to_new(const uint8_t* keep, int32_t n, int32_t* old_to_new);
 double det_sum(const double* x, int64_t n);
 void fill_u8(uint8_t* p, uint8_t v, int64_t n);
 // What one row fold sees of the entry-bounds store (struct EntryBounds
-// below): an entry with |a| > zero_tol whose column is not marked changed
-// reads its stored pair; every other entry gathers lu[col], and a refreshing
-// (writer) fold stores the gathered pair of such an entry back.
+// below): an entry with |a| > zero_tol whose column is marked current reads
+// its stored pair; every other entry gathers lu[col], and a refreshing
+// (writer) fold stores the gathered pair of such an entry back.  A current
+// column is also live under the fold's column mask, so its entries skip the
+// mask lookup.
 struct StoreView {
   double2* pairs = nullptr;  // null: no store, every pair gathered
-  Bits changed{nullptr};     // columns whose pair differs from the store's snap (null: all)
+  Bits current{nullptr};     // live columns whose pair equals the store's snap (null: none)
   double zero_tol = 0.0;
   bool write = false;
 };
@@ -429,8 +431,9 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(gp_csr A, const double2*
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const gp_offset q = c0 + k * kFoldRows + static_cast<gp_offset>(threadIdx.x);
-        used[k] = q < c1 && keep_col[j[k]] && pol.use(a[k]);
-        stored[k] = store.pairs && fabs(a[k]) > store.zero_tol && !store.changed[j[k]];
+        const bool cur = store.pairs && store.current.w && store.current[j[k]];
+        used[k] = q < c1 && (cur || keep_col[j[k]]) && pol.use(a[k]);
+        stored[k] = cur && fabs(a[k]) > store.zero_tol;
       }
       double2 b[kPer];
 #pragma unroll
@@ -527,10 +530,11 @@ struct EntryBounds {
   }
 };
 // The view of `eb` (may be null) for one fold over A with the current pairs
-// lu; `changed` receives the changed-column bits.  A refreshing (writer) fold
+// lu and the fold's column mask keep_col (null: every column live);
+// `current` receives the current-column bits.  A refreshing (writer) fold
 // starts the store over when it holds nothing for A; a reader then gathers.
-StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol, bool writer,
-                     DVec<uint32_t>& changed);  // gp_primitives.hip
+StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, const uint8_t* keep_col, double zero_tol,
+                     bool writer, DVec<uint32_t>& current);  // gp_primitives.hip
 // After a writer fold: the current pairs become the snap.
 void store_commit(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol);
 // The same, taking over lu's buffer as the snap instead of copying it (lu is left empty).

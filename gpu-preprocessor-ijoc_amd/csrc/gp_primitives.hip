@@ -1608,13 +1608,15 @@ PackedBounds::PackedBounds(const double* l, const double* u, int32_t n, double* 
 }
 
 namespace {
-// bit j: column j's current pair differs from the snap (bitwise)
-__global__ void k_pairs_changed(const double2* lu, const double2* snap, int32_t n, uint32_t* words) {
+// bit j: column j is live (keep null: every column) and its pair equals the snap (bitwise)
+__global__ void k_pairs_current(const double2* lu, const double2* snap, const uint8_t* keep, int32_t n,
+                                uint32_t* words) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool c = false;
   if (j < n) {
     const double2 a = lu[j], b = snap[j];
-    c = __double_as_longlong(a.x) != __double_as_longlong(b.x) || __double_as_longlong(a.y) != __double_as_longlong(b.y);
+    c = __double_as_longlong(a.x) == __double_as_longlong(b.x) && __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+        (!keep || keep[j]);
   }
   const unsigned long long bal = __ballot(c);
   const int lane = static_cast<int>(__lane_id());
@@ -1622,23 +1624,23 @@ __global__ void k_pairs_changed(const double2* lu, const double2* snap, int32_t 
 }
 }  // namespace
 
-StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, double zero_tol, bool writer,
-                     DVec<uint32_t>& changed) {
+StoreView store_view(EntryBounds* eb, const gp_csr& A, const double2* lu, const uint8_t* keep_col, double zero_tol,
+                     bool writer, DVec<uint32_t>& current) {
   StoreView v;
   if (!eb || A.nnz <= 0 || A.cols <= 0) return v;
   const int32_t n = A.cols;
   if (eb->holds(A, zero_tol)) {
-    // current pairs, the snap, a bit out per column
-    TimedScope ts(GP_KT_ENTRY_BOUNDS, 32.0 * n + n / 8.0);
-    changed.resize(static_cast<size_t>((n + 31) / 32));
-    hipLaunchKernelGGL(k_pairs_changed, dim3(grid_for((static_cast<int64_t>(n) + 31) / 32 * 32)), dim3(kBlock), 0, 0,
-                       lu, eb->snap.data(), n, changed.data());
-    GP_LAUNCH_CHECK("changed pairs");
-    v.changed = Bits{changed.data()};
+    // current pairs, the snap, the mask, a bit out per column
+    TimedScope ts(GP_KT_ENTRY_BOUNDS, (keep_col ? 33.0 : 32.0) * n + n / 8.0);
+    current.resize(static_cast<size_t>((n + 31) / 32));
+    hipLaunchKernelGGL(k_pairs_current, dim3(grid_for((static_cast<int64_t>(n) + 31) / 32 * 32)), dim3(kBlock), 0, 0,
+                       lu, eb->snap.data(), keep_col, n, current.data());
+    GP_LAUNCH_CHECK("current pairs");
+    v.current = Bits{current.data()};
   } else if (writer) {
     eb->drop();
     eb->pairs.resize(static_cast<size_t>(A.nnz));
-    v.changed = Bits{nullptr};  // every column: gather and store
+    v.current = Bits{nullptr};  // no column: gather and store
   } else {
     return v;
   }

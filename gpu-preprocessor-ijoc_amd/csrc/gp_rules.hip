@@ -515,9 +515,9 @@ void rule_redundant_rows(gp_plan& plan, const gp_device_lp& lp, const gp_stats& 
   // A, (l, u) and a mask bit per column, keep_row, row_nnz, AL, AU, 3 flags out per row
   TimedScope ts(GP_KT_REDUNDANT, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * lp.A.cols + lp.A.cols / 8.0 + 24.0 * m);
   // the entry-bounds store (refreshed by bound strengthening's sweep right before) for unchanged columns
-  DVec<uint32_t> changed;
-  const StoreView store = store_view(s.live_masks ? plan_store(plan) : nullptr, lp.A, lu.data(), p.zero_tol, false,
-                                     changed);
+  DVec<uint32_t> current;
+  const StoreView store = store_view(s.live_masks ? plan_store(plan) : nullptr, lp.A, lu.data(), plan.keep_col_mask,
+                                     p.zero_tol, false, current);
   launch_row_fold(lp.A, lu.data(), live_cols.view(),
                   RedundantFold{plan.keep_row_mask, s.row_nnz, plan.new_AL, plan.new_AU, p.bound_tol, full.data(),
                                 dl.data(), du.data(), flags.d.data(), batch_marks()},
@@ -967,8 +967,8 @@ void rule_bound_strengthening(gp_plan& plan, const gp_device_lp& lp, const gp_st
   EntryBounds* store = plan_store(plan);
   // without live masks every column counts as live: start the store over, so it covers them all
   if (!keep_col) store->drop();
-  DVec<uint32_t> changed;
-  const StoreView view = store_view(store, lp.A, lu.data(), p.zero_tol, true, changed);
+  DVec<uint32_t> current;
+  const StoreView view = store_view(store, lp.A, lu.data(), keep_col, p.zero_tol, true, current);
   row_activity_summary(lp.A, lu.data(), p.zero_tol, plan.keep_row_mask, keep_col, min_fin.data(), max_fin.data(),
                        min_ninf.data(), max_pinf.data(), row_rec.size() ? row_rec.data() : nullptr, plan.new_AL,
                        plan.new_AU, s.row_nnz, view);

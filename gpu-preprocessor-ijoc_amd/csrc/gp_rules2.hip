@@ -1754,8 +1754,8 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
     if (full) {
       // every live row is folded: the sweep refreshes the entry-bounds store
       EntryBounds* store = plan_store(plan);
-      DVec<uint32_t> changed;
-      const StoreView view = store_view(store, lp.A, lu.data(), p.zero_tol, true, changed);
+      DVec<uint32_t> current;
+      const StoreView view = store_view(store, lp.A, lu.data(), plan.keep_col_mask, p.zero_tol, true, current);
       {
         // A, (l, u) and a mask bit per column, keep_row, 40 bytes out per row
         TimedScope ts(GP_KT_SC_ACTIVITY, 4.0 * (m + 1) + 12.0 * lp.A.nnz + 16.0 * n + n / 8.0 + 41.0 * m);
