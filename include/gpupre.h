@@ -359,7 +359,9 @@ enum {
  * Instance k occupies rows [row_offsets[k], row_offsets[k+1]) and columns
  * [col_offsets[k], col_offsets[k+1]) of the direct sum, and the same-numbered
  * reduced_* ranges of whole.reduced_lp.  Scheduling decisions are taken for
- * the batch; whole.has_infeasible / has_unbounded flag the batch. */
+ * the batch.  An instance found infeasible or unbounded is retired (status[],
+ * restarts); whole.has_infeasible / has_unbounded flag the batch only when a
+ * stage that marks no instance ends it. */
 typedef struct gp_batch_summary {
   int32_t count;
   gp_summary whole;            /* presolve of the direct sum (record, reduced LP) */
