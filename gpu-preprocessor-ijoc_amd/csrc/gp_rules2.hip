@@ -221,18 +221,20 @@ __device__ bool sc_rest_bounds(const ScAct& act, double a, double lj, double uj,
   return !isnan(*rmin) && !isnan(*rmax);
 }
 
-__global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const uint8_t* keep_row,
-                             const uint8_t* keep_col, const uint8_t* smask, const int32_t* srow, const double* sval,
-                             const double* c, const double* l, const double* u, const double* AL, const double* AU,
-                             double zt, double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags,
+__global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const uint8_t* keep_col,
+                             const uint8_t* smask, const int32_t* srow, const double* sval, const double* c,
+                             const double* l, const double* u, const double* AL, const double* AU, double zt,
+                             double tol, uint8_t* action, double* side, int32_t* owner, int32_t* flags,
                              BatchMarks bm) {
   const int64_t j = gtid();
   if (j >= n || !keep_col[j] || !smask[j]) return;
   const int32_t row = srow[j];
-  if (row < 0 || row >= m || !keep_row[row]) return;
+  if (row < 0 || row >= m) return;
   const double a = sval[j];
   if (fabs(a) <= zt) return;
   const double lhs = AL[row], rhs = AU[row], cj = c[j], lj = l[j], uj = u[j];
+  // A deleted row has live = 0 in the cache (the sweep and the row refresh
+  // both write it so), so this also returns for every row with keep_row 0.
   const ScAct act = row_act[row];
   if (act.live <= 1) return;
   double rmin = 0.0, rmax = 0.0;
@@ -1774,9 +1776,8 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
     cache->valid = true;
   }
   const ScAct* row_act = cache->act.data();
-  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act, plan.keep_row_mask,
-                     plan.keep_col_mask, s.singleton_column_mask, s.singleton_column_row,
-                     s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
+  hipLaunchKernelGGL(k_sc_propose, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, m, row_act, plan.keep_col_mask,
+                     s.singleton_column_mask, s.singleton_column_row, s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.zero_tol,
                      p.bound_tol, action.data(), side.data(), owner.data(), flags.d.data(), batch_marks());
   GP_LAUNCH_CHECK("singleton columns propose");
   const int32_t* pf = flags.read();
