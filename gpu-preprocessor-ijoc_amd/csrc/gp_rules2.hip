@@ -232,13 +232,18 @@ __global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const u
   if (row < 0 || row >= m) return;
   const double a = sval[j];
   if (fabs(a) <= zt) return;
-  const double lhs = AL[row], rhs = AU[row], cj = c[j], lj = l[j], uj = u[j];
+  const double cj = c[j], lj = l[j], uj = u[j];
   // A deleted row has live = 0 in the cache (the sweep and the row refresh
   // both write it so), so this also returns for every row with keep_row 0.
   const ScAct act = row_act[row];
   if (act.live <= 1) return;
   double rmin = 0.0, rmax = 0.0;
   const bool rest_ok = sc_rest_bounds(act, a, lj, uj, &rmin, &rmax);
+  // The row sides matter only to the unboundedness test, which needs a cost
+  // against an infinite bound, and to a column whose rest of row is bounded:
+  // every other column returns here without gathering them.
+  if (!rest_ok && !(cj > zt && !isfinite(lj)) && !(cj < -zt && !isfinite(uj))) return;
+  const double lhs = AL[row], rhs = AU[row];
   if ((cj > zt && a > zt && !isfinite(lhs) && !isfinite(lj)) ||
       (cj > zt && a < -zt && !isfinite(rhs) && !isfinite(lj)) ||
       (cj < -zt && a < -zt && !isfinite(lhs) && !isfinite(uj)) ||
