@@ -297,6 +297,20 @@ __global__ void k_sc_propose(int32_t n, int32_t m, const ScAct* row_act, const u
   wave_flag(&flags[4], 1);
 }
 
+__global__ void k_sc_clear(int32_t m, int32_t n, uint8_t* row_delete, uint8_t* lower_change, uint8_t* upper_change,
+                           uint8_t* col_delete, double* contrib) {
+  const int64_t i = gtid();
+  if (i < m) {
+    row_delete[i] = 0;
+    lower_change[i] = 0;
+    upper_change[i] = 0;
+  }
+  if (i < n) {
+    col_delete[i] = 0;
+    contrib[i] = 0.0;
+  }
+}
+
 __global__ void k_sc_stage(int32_t m, int32_t n, int32_t* owner, const uint8_t* action, const double* side,
                            const double* sval, const double* c, const double* l, const double* u, const double* AL,
                            const double* AU, double tol, uint8_t* row_delete, uint8_t* lower_change,
@@ -1714,11 +1728,6 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
   DVec<double> AL_new(m), AU_new(m), side(n), contrib(n);
   DVec<int32_t> owner(m);
   DVec<uint8_t> action(n), row_delete(m), lo(m), up(m), col_delete(n);
-  fill_u8(row_delete.data(), 0, m);
-  fill_u8(lo.data(), 0, m);
-  fill_u8(up.data(), 0, m);
-  fill_u8(col_delete.data(), 0, n);
-  GP_CHECK(hipMemsetAsync(contrib.data(), 0, sizeof(double) * n, 0));
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(m)), dim3(kBlock), 0, 0, owner.data(), INT_MAX, m);
   GP_LAUNCH_CHECK("singleton columns owner");
   RuleCaches& rc = rule_caches(plan);
@@ -1791,6 +1800,9 @@ void rule_singleton_columns(gp_plan& plan, const gp_device_lp& lp, const gp_stat
     return;
   }
   if (!pf[4]) return;  // no proposal
+  // the stage's outputs start cleared; a call without proposals never needs them
+  hipLaunchKernelGGL(k_sc_clear, dim3(grid_for(m > n ? m : n)), dim3(kBlock), 0, 0, m, n, row_delete.data(), lo.data(),
+                     up.data(), col_delete.data(), contrib.data());
   hipLaunchKernelGGL(k_sc_stage, dim3(grid_for(m)), dim3(kBlock), 0, 0, m, n, owner.data(), action.data(), side.data(),
                      s.singleton_column_val, plan.new_c, plan.new_l, plan.new_u, plan.new_AL, plan.new_AU, p.bound_tol,
                      row_delete.data(), lo.data(), up.data(), col_delete.data(), AL_new.data(), AU_new.data(),
